@@ -1,0 +1,269 @@
+#!/usr/bin/env python3
+"""Benchmark of the batched SPD path on one MI355X.
+
+Default workload (the headline metric): 50,000 SPD 64x64 systems,
+A = Q diag(s) Q^T with Q Haar-orthogonal (QR of a Gaussian) and s log-spaced
+so cond(A) = kappa, log10(kappa) ~ U[2, 6]; 4 right-hand sides ~ N(0,1) in the
+reference's Right-side layout (B is 4 x 64 per system).  One step = one
+mixed-precision batched POSV (FP32 factor + FP64 refinement, FP64 fallback)
+over the whole batch; the plain FP64 batched POSV on the same inputs is timed
+the same way.  value = mixed solves/s.
+
+Inputs are generated on the GPU with torch (plumbing only) and staged W+K
+times in HBM, so the timed loop holds exactly K library calls back to back,
+bracketed by synchronisation, timed with HIP events on the library's stream.
+The CPU baseline (the oracle's OpenMP FP64 POSV restatement, and LAPACK when a
+library is loadable) runs on a bounded sample of the same inputs, which also
+yields the per-solve backward errors of the GPU results.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c4]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "kblas-gpu_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+CONFIGS = {
+    "c1": dict(op="potrf", n=32, batch=10000, nrhs=0, gen="shift",
+               workload="dpotrf_batch_strided n=32 x 10000, A=B*B^T+32I"),
+    "c2": dict(op="posv", n=16, batch=100000, nrhs=1, gen="shift",
+               workload="dposv_batch_strided n=16 x 100000, 1 RHS, A=B*B^T+16I"),
+    "c4": dict(op="sposv", n=64, batch=50000, nrhs=4, gen="cond",
+               workload="dsposv_batch_strided n=64 x 50000, 4 RHS, cond in [1e2,1e6], mixed FP32+IR vs FP64"),
+}
+
+
+def gen_inputs(torch, cfg, seed):
+    n, bc, nrhs = cfg["n"], cfg["batch"], cfg["nrhs"]
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    dev = "cuda"
+    if cfg["gen"] == "shift":
+        Bm = torch.rand((bc, n, n), generator=g, device=dev, dtype=torch.float64) * 2 - 1
+        A = torch.bmm(Bm, Bm.transpose(1, 2).contiguous())
+        A = 0.5 * (A + A.transpose(1, 2))
+        A += n * torch.eye(n, device=dev, dtype=torch.float64)
+        kappa = None
+    else:
+        Gm = torch.randn((bc, n, n), generator=g, device=dev, dtype=torch.float64)
+        Q, _ = torch.linalg.qr(Gm)
+        lk = 2.0 + 4.0 * torch.rand((bc, 1), generator=g, device=dev, dtype=torch.float64)
+        t = torch.rand((bc, n), generator=g, device=dev, dtype=torch.float64)
+        t[:, 0] = 0.0
+        t[:, -1] = 1.0
+        s = torch.pow(10.0, -lk * t)
+        A = torch.bmm(Q * s[:, None, :], Q.transpose(1, 2).contiguous())
+        A = 0.5 * (A + A.transpose(1, 2))
+        kappa = torch.pow(10.0, lk[:, 0])
+        del Gm, Q
+    A = A.contiguous()                       # symmetric: row- and column-major agree
+    B = None
+    if nrhs:
+        # Right side (reference layout): B is nrhs x n, ldb = nrhs, column-major
+        B = torch.randn((bc, n, nrhs), generator=g, device=dev, dtype=torch.float64).contiguous()
+    return A, B, kappa
+
+
+def time_steps(torch, call, K, W):
+    for i in range(W):
+        call(i)
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(s)
+    for i in range(K):
+        call(W + i)
+    e1.record(s)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return e0.elapsed_time(e1) / 1000.0, wall
+
+
+def cpu_baseline(cfg, A_s, B_s, X_gpu, cores):
+    """Oracle (FP64 POSV restatement, OpenMP over the sample) on host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    L = po.lib()
+    L.orc_set_threads(cores)
+    n, nrhs = cfg["n"], cfg["nrhs"]
+    bs = A_s.shape[0]
+    res = {}
+    # accuracy of the GPU results on the sample, against the FP64 oracle metrics
+    if X_gpu is not None:
+        be = np.array([po.solve_berr("R", A_s[i].T, X_gpu[i].T, B_s[i].T) for i in range(bs)])
+        res["berr_max"] = float(be.max())
+        res["berr_median"] = float(np.median(be))
+    A = np.ascontiguousarray(A_s)
+    reps, t_best = 0, None
+    t_end = time.perf_counter() + 12.0
+    while reps < 5 and (reps == 0 or time.perf_counter() < t_end):
+        Aw = A.copy()
+        if cfg["op"] == "potrf":
+            t0 = time.perf_counter()
+            L.orc_potrf_batch(n, po.ptr(Aw), n, n * n, bs, None)
+        else:
+            Bw = np.ascontiguousarray(B_s).copy()
+            t0 = time.perf_counter()
+            L.orc_posv_batch(b"R", nrhs, n, po.ptr(Aw), n, n * n, po.ptr(Bw), nrhs, nrhs * n, bs, None)
+        dt = time.perf_counter() - t0
+        t_best = dt if t_best is None else min(t_best, dt)
+        reps += 1
+    res["value"] = bs / t_best
+    lap = None
+    if L.orc_lapack_available():
+        Aw = A.copy()
+        t0 = time.perf_counter()
+        if cfg["op"] == "potrf":
+            L.orc_lapack_potrf_batch(n, po.ptr(Aw), n, n * n, bs, None)
+        else:
+            Bw = np.ascontiguousarray(B_s).copy()
+            L.orc_lapack_posv_batch(b"R", nrhs, n, po.ptr(Aw), n, n * n, po.ptr(Bw), nrhs, nrhs * n, bs, None)
+        lap = {"value": bs / (time.perf_counter() - t0), "library": L.orc_lapack_name().decode()}
+    res["lapack"] = lap
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c4", choices=sorted(CONFIGS))
+    ap.add_argument("--cpu-sample", type=int, default=0, help="matrices in the CPU-baseline sample (0 = auto)")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (the batched SPD path shards by matrix with "
+              "no exchange; see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import pykblas
+
+    cfg = CONFIGS[args.config]
+    n, bc, nrhs = cfg["n"], cfg["batch"], cfg["nrhs"]
+    K, W = args.steps, args.warmup
+    print("bench: config %s (%s), generating inputs" % (args.config, cfg["workload"]), file=sys.stderr, flush=True)
+    A0, B0, kappa = gen_inputs(torch, cfg, 1234 + int(args.config[1]))
+    torch.cuda.synchronize()
+    h = pykblas.Handle(torch.cuda.current_stream().cuda_stream)
+    lib = h.lib
+    info = torch.zeros(bc, dtype=torch.int32, device="cuda")
+    iters = torch.zeros(bc, dtype=torch.int32, device="cuda")
+    NA = n * n
+    out = {}
+
+    if cfg["op"] == "potrf":
+        As = [A0.clone() for _ in range(K + W)]
+
+        def call(i):
+            st = lib.kblasDpotrf_batch_strided(h.h, b"L", n, As[i].data_ptr(), n, NA, bc, info.data_ptr())
+            assert st == pykblas.KBLAS_Success, st
+        t, wall = time_steps(torch, call, K, W)
+        per_unit_bytes = 8 * n * (n + 1)
+        value = bc * K / t
+        metric, unit = "factorizations/s (batched DPOTRF n=%d)" % n, "factorizations/s"
+        dtype = "f64"
+        X_sample = None
+        del As
+    elif cfg["op"] == "posv":
+        As = [A0.clone() for _ in range(K + W)]
+        Bs = [B0.clone() for _ in range(K + W)]
+
+        def call(i):
+            st = lib.kblasDposv_batch_strided(h.h, b"R", b"L", nrhs, n, As[i].data_ptr(), n, NA,
+                                              Bs[i].data_ptr(), nrhs, nrhs * n, bc, info.data_ptr())
+            assert st == pykblas.KBLAS_Success, st
+        t, wall = time_steps(torch, call, K, W)
+        per_unit_bytes = 8 * n * (n + 1) + 16 * n * nrhs
+        value = bc * K / t
+        metric, unit = "solves/s (batched DPOSV n=%d, %d RHS)" % (n, nrhs), "solves/s"
+        dtype = "f64"
+        X_sample = Bs[0]
+        del As
+    else:
+        # mixed precision: A and B read-only, X written (A only on fallback)
+        As = [A0.clone() for _ in range(K + W)]
+        Xs = torch.empty_like(B0)
+
+        def call_mixed(i):
+            st = lib.kblasDsposv_batch_strided(h.h, b"R", b"L", nrhs, n, As[i].data_ptr(), n, NA,
+                                               B0.data_ptr(), nrhs, nrhs * n, Xs.data_ptr(), nrhs, nrhs * n,
+                                               bc, info.data_ptr(), iters.data_ptr())
+            assert st == pykblas.KBLAS_Success, st
+        print("bench: timing mixed FP32+IR POSV", file=sys.stderr, flush=True)
+        t, wall = time_steps(torch, call_mixed, K, W)
+        it_h = iters.cpu().numpy()
+        X_sample = Xs
+        del As
+        # plain FP64 POSV on the same inputs
+        As = [A0.clone() for _ in range(K + W)]
+        Bs = [B0.clone() for _ in range(K + W)]
+
+        def call_fp64(i):
+            st = lib.kblasDposv_batch_strided(h.h, b"R", b"L", nrhs, n, As[i].data_ptr(), n, NA,
+                                              Bs[i].data_ptr(), nrhs, nrhs * n, bc, info.data_ptr())
+            assert st == pykblas.KBLAS_Success, st
+        print("bench: timing FP64 POSV", file=sys.stderr, flush=True)
+        t64, _ = time_steps(torch, call_fp64, K, W)
+        X64 = Bs[0]
+        del As
+        per_unit_bytes = 4 * n * (n + 1) + 16 * n * nrhs     # A lower read once (FP64), B in, X out
+        value = bc * K / t
+        metric = "solves/s (batched POSV n=64 mixed FP32+IR, bwd err<=1e-13)"
+        unit = "solves/s"
+        dtype = "f32+f64"
+        out["fp64_posv"] = {"value": bc * K / t64, "ms_per_step": 1000 * t64 / K,
+                            "achieved_GBs": (8 * n * (n + 1) + 16 * n * nrhs) * bc * K / t64 / 1e9}
+        out["mixed_over_fp64"] = (bc * K / t) / (bc * K / t64)
+        hist = {}
+        for v in it_h.tolist():
+            hist[str(v)] = hist.get(str(v), 0) + 1
+        out["iter_hist"] = hist
+        out["fallbacks"] = int((it_h < 0).sum())
+
+    ms = 1000.0 * t / K
+    achieved = per_unit_bytes * bc / (t / K) / 1e9
+    line = {
+        "metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": dtype, "data": "synthetic (seeded, generated on the GPU)",
+        "config": {"workload": cfg["workload"], "batch": bc, "n": n, "nrhs": nrhs},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_unit": per_unit_bytes},
+    }
+    line.update(out)
+    torch.cuda.synchronize()
+    if not args.no_cpu:
+        cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+        bs = args.cpu_sample or {"c1": 20000, "c2": 50000, "c4": 2000}[args.config]
+        bs = min(bs, bc)
+        A_s = A0[:bs].cpu().numpy()
+        B_s = B0[:bs].cpu().numpy() if B0 is not None else None
+        Xg = X_sample[:bs].cpu().numpy() if X_sample is not None else None
+        print("bench: CPU baseline on %d matrices, %d threads" % (bs, cores), file=sys.stderr, flush=True)
+        cb = cpu_baseline(cfg, A_s, B_s, Xg, cores)
+        line["cpu_baseline"] = {"value": cb["value"], "unit": unit, "cores": cores, "kind": "port",
+                                "sample": "%d of the %d systems, FP64 oracle POSV/POTRF restatement, OpenMP"
+                                          % (bs, bc),
+                                "lapack": cb.get("lapack")}
+        if "berr_max" in cb:
+            line["berr_max_sample"] = cb["berr_max"]
+            line["berr_median_sample"] = cb["berr_median"]
+    h.close()
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,91 @@
+/*
+ * kblas_batch_ext.h -- entry points with no counterpart in the reference.
+ *
+ * The reference has no non-uniform POTRF/POTRS/POSV and no mixed-precision
+ * POSV (its non-uniform support stops at MAGMA-backed TRSM/SYRK/GEMM,
+ * src/batch_triangular/Xtrsm_batch_drivers.cuh:277-367).  These follow the
+ * reference's own `_vbatch` naming for non-uniform batched routines
+ * (include/batch_qr.h:272, include/batch_transpose.h:33) and LAPACK dsposv's
+ * ITER convention for the mixed-precision solver.
+ *
+ * Non-uniform ("vbatch") arguments: n, m, lda, ldb, ldx are DEVICE int arrays
+ * of length batchCount; A, B, X are DEVICE arrays of device pointers; max_n /
+ * max_m are host hints (0 = unknown).  The whole batch runs in one persistent
+ * launch that pulls matrices from a size-sorted queue built on the device; the
+ * queue lives in the handle's device workspace (query, allocate, call).
+ */
+#ifndef KBLAS_BATCH_EXT_H_
+#define KBLAS_BATCH_EXT_H_
+
+#include "kblas.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- non-uniform Cholesky family ---------------------------------------- */
+void kblas_potrf_vbatch_wsquery(kblasHandle_t handle, int max_n, int batchCount);
+void kblas_posv_vbatch_wsquery(kblasHandle_t handle, int max_n, int batchCount);
+
+int kblasDpotrf_vbatch(kblasHandle_t handle, char uplo, int* n, int max_n,
+                       double** A, int* lda, int batchCount, int* info_array);
+int kblasSpotrf_vbatch(kblasHandle_t handle, char uplo, int* n, int max_n,
+                       float** A, int* lda, int batchCount, int* info_array);
+/* B[i] is m[i] x n[i] as in the uniform calls: side 'R' solves X*A = B with A
+ * of order n[i] (m[i] right-hand sides, the reference's layout); side 'L'
+ * solves A*X = B with A of order m[i] (n[i] right-hand sides, LAPACK's).     */
+int kblasDpotrs_vbatch(kblasHandle_t handle, char side, char uplo, int* m, int* n,
+                       int max_m, int max_n, double** A, int* lda, double** B, int* ldb,
+                       int batchCount);
+int kblasDposv_vbatch(kblasHandle_t handle, char side, char uplo, int* m, int* n,
+                      int max_m, int max_n, double** A, int* lda, double** B, int* ldb,
+                      int batchCount, int* info_array);
+int kblasSposv_vbatch(kblasHandle_t handle, char side, char uplo, int* m, int* n,
+                      int max_m, int max_n, float** A, int* lda, float** B, int* ldb,
+                      int batchCount, int* info_array);
+
+/* ---- mixed precision POSV (LAPACK dsposv semantics, batched) -------------
+ * FP32 Cholesky and triangular sweeps, FP64 residual refinement until
+ * ||r||inf <= ||x||inf * ||A||inf * eps64 * sqrt(n) (at most 30 steps), FP64
+ * Cholesky fallback when the FP32 pivot is not positive or refinement stalls.
+ * A is read-only unless the fallback ran (then A holds L, as in dsposv).
+ * B is read-only; X receives the solution (X may equal B with ldx == ldb).
+ * iter[i] >= 0: refinement steps used; -3: FP32 factor failed; -31: no
+ * convergence (both then solved in FP64).  info[i] as in dposv.
+ * side / m / n exactly as kblasDposv_batch_strided.                           */
+int kblasDsposv_batch_strided(kblasHandle_t handle, char side, char uplo,
+                              const int m, const int n,
+                              double* A, int lda, long strideA,
+                              const double* B, int ldb, long strideB,
+                              double* X, int ldx, long strideX,
+                              int batchCount, int* info_array, int* iter_array);
+int kblasDsposv_batch(kblasHandle_t handle, char side, char uplo,
+                      const int m, const int n,
+                      double** A_array, int lda, const double** B_array, int ldb,
+                      double** X_array, int ldx,
+                      int batchCount, int* info_array, int* iter_array);
+int kblasDsposv_vbatch(kblasHandle_t handle, char side, char uplo, int* m, int* n,
+                       int max_m, int max_n, double** A, int* lda, const double** B, int* ldb,
+                       double** X, int* ldx, int batchCount, int* info_array, int* iter_array);
+
+#ifdef __cplusplus
+}  /* extern "C" */
+
+inline int kblas_potrf_batch(kblasHandle_t h, char uplo, int* n, int max_n, double** A, int* lda,
+                             int bc, int* info)
+{ return kblasDpotrf_vbatch(h, uplo, n, max_n, A, lda, bc, info); }
+inline int kblas_potrf_batch(kblasHandle_t h, char uplo, int* n, int max_n, float** A, int* lda,
+                             int bc, int* info)
+{ return kblasSpotrf_vbatch(h, uplo, n, max_n, A, lda, bc, info); }
+inline int kblas_posv_batch(kblasHandle_t h, char side, char uplo, int* m, int* n, int max_m, int max_n,
+                            double** A, int* lda, double** B, int* ldb, int bc, int* info)
+{ return kblasDposv_vbatch(h, side, uplo, m, n, max_m, max_n, A, lda, B, ldb, bc, info); }
+inline int kblas_posv_batch(kblasHandle_t h, char side, char uplo, int* m, int* n, int max_m, int max_n,
+                            float** A, int* lda, float** B, int* ldb, int bc, int* info)
+{ return kblasSposv_vbatch(h, side, uplo, m, n, max_m, max_n, A, lda, B, ldb, bc, info); }
+inline int kblas_potrs_batch(kblasHandle_t h, char side, char uplo, int* m, int* n, int max_m, int max_n,
+                             double** A, int* lda, double** B, int* ldb, int bc)
+{ return kblasDpotrs_vbatch(h, side, uplo, m, n, max_m, max_n, A, lda, B, ldb, bc); }
+#endif
+
+#endif /* KBLAS_BATCH_EXT_H_ */

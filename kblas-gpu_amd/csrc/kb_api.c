@@ -1,0 +1,476 @@
+/* kb_api.c -- the kblas{S,D}{potrf,potrs,posv,trsm,syrk,gemm}_batch* C entry points.
+ *
+ * Front-end behaviour mirrors the reference front-ends
+ * (src/batch_triangular/X{potrf,potrs,posv,trsm,syrk,gemm}_batch.cu): workspace
+ * sufficiency check (KBLAS_InsufficientWorkspace), Upper / Unit variants
+ * rejected with KBLAS_NotImplemented, launch failures reported as
+ * KBLAS_UnknownError.  Added here (the reference does no argument checking):
+ * negative sizes, too-small leading dimensions / strides, NULL operands and
+ * negative batch counts return KBLAS_Error_WrongInput before anything is
+ * launched; batchCount == 0 or an empty matrix returns KBLAS_Success (info is
+ * zeroed when the order is 0).  Nothing here allocates or synchronises.
+ */
+#include <stdlib.h>
+#include <string.h>
+
+#include "kb_internal.h"
+
+#define IMAX(a, b) ((a) > (b) ? (a) : (b))
+
+static int upl(char c) { return c == 'L' || c == 'l' ? 0 : (c == 'U' || c == 'u') ? 1 : -1; }
+static int sideof(char c) { return c == 'L' || c == 'l' ? 0 : (c == 'R' || c == 'r') ? 1 : -1; }
+static int transof(char c)
+{
+    if (c == 'N' || c == 'n') return 0;
+    if (c == 'T' || c == 't' || c == 'C' || c == 'c') return 1;
+    return -1;
+}
+static int diagof(char c) { return c == 'N' || c == 'n' ? 0 : (c == 'U' || c == 'u') ? 1 : -1; }
+
+static kb_mat_t mat_strided(const void* p, int ld, long stride)
+{
+    kb_mat_t m;
+    m.base = (void*)p; m.stride = stride; m.ptrs = NULL; m.lds = NULL; m.ld = ld;
+    return m;
+}
+static kb_mat_t mat_ptrs(const void* pp, int ld)
+{
+    kb_mat_t m;
+    m.base = NULL; m.stride = 0; m.ptrs = (void* const*)pp; m.lds = NULL; m.ld = ld;
+    return m;
+}
+static kb_mat_t mat_vptrs(const void* pp, const int* lds)
+{
+    kb_mat_t m;
+    m.base = NULL; m.stride = 0; m.ptrs = (void* const*)pp; m.lds = lds; m.ld = 0;
+    return m;
+}
+
+static int launched(int hip_err)
+{
+    if (hip_err == 0) return KBLAS_Success;
+    if (hip_err == (int)hipErrorNotSupported) return KBLAS_NotImplemented;
+    return KBLAS_UnknownError;
+}
+
+static void zero_info(kblasHandle_t h, int* info, int bc)
+{
+    if (info && bc > 0) kb_launch_fill_int(info, 0, bc, h->stream);
+}
+
+/* ============================== POTRF ================================== */
+static int potrf_core(kblasHandle_t h, char prec, char uplo, int n, kb_mat_t A,
+                      long strideA, int strided, int bc, int* info)
+{
+    KB_CHECK_HANDLE(h);
+    int u = upl(uplo);
+    if (u < 0 || n < 0 || bc < 0 || A.ld < IMAX(1, n)) return KBLAS_Error_WrongInput;
+    if (u == 1) return KBLAS_NotImplemented;
+    if (bc == 0) return KBLAS_Success;
+    if (strided ? (!A.base || strideA < (long)A.ld * n) : !A.ptrs) return KBLAS_Error_WrongInput;
+    if (n == 0) { zero_info(h, info, bc); return KBLAS_Success; }
+    if (n <= KB_REG_MAX_N)
+        return launched(kb_launch_reg_chol(prec, KB_MODE_POTRF, n, 0, 0, A, A, info, bc, h->stream));
+    return launched(kb_launch_blk_chol(prec, KB_MODE_POTRF, n, 0, 0, A, A, info, bc, h->stream));
+}
+
+int kblasDpotrf_batch(kblasHandle_t h, char uplo, const int n, double** A, int lda, int bc, int* info)
+{ return potrf_core(h, 'd', uplo, n, mat_ptrs(A, lda), 0, 0, bc, info); }
+int kblasSpotrf_batch(kblasHandle_t h, char uplo, const int n, float** A, int lda, int bc, int* info)
+{ return potrf_core(h, 's', uplo, n, mat_ptrs(A, lda), 0, 0, bc, info); }
+int kblasDpotrf_batch_strided(kblasHandle_t h, char uplo, const int n, double* A, int lda, long sA,
+                              int bc, int* info)
+{ return potrf_core(h, 'd', uplo, n, mat_strided(A, lda, sA), sA, 1, bc, info); }
+int kblasSpotrf_batch_strided(kblasHandle_t h, char uplo, const int n, float* A, int lda, long sA,
+                              int bc, int* info)
+{ return potrf_core(h, 's', uplo, n, mat_strided(A, lda, sA), sA, 1, bc, info); }
+
+/* ========================= POTRS / POSV ================================= */
+/* B is m x n.  side Right: X A = B, A of order n, m right-hand sides (the
+ * reference's only side, src/batch_triangular/Xpotrs_batch_drivers.cuh:40-43).
+ * side Left: A X = B, A of order m, n right-hand sides (LAPACK layout). */
+static int solve_core(kblasHandle_t h, char prec, int mode, char side, char uplo, int m, int n,
+                      kb_mat_t A, long strideA, kb_mat_t B, long strideB, int strided,
+                      int bc, int* info)
+{
+    KB_CHECK_HANDLE(h);
+    int sd = sideof(side), u = upl(uplo);
+    if (sd < 0 || u < 0 || m < 0 || n < 0 || bc < 0) return KBLAS_Error_WrongInput;
+    int order = sd == 0 ? m : n;
+    int nrhs = sd == 0 ? n : m;
+    if (A.ld < IMAX(1, order) || B.ld < IMAX(1, m)) return KBLAS_Error_WrongInput;
+    if (u == 1) return KBLAS_NotImplemented;
+    if (bc == 0) return KBLAS_Success;
+    if (strided) {
+        if (!A.base || !B.base || strideB < (long)B.ld * n || strideA < 0) return KBLAS_Error_WrongInput;
+        if (mode == KB_MODE_POSV && strideA < (long)A.ld * order) return KBLAS_Error_WrongInput;
+    } else if (!A.ptrs || !B.ptrs) {
+        return KBLAS_Error_WrongInput;
+    }
+    if (order == 0) { if (mode == KB_MODE_POSV) zero_info(h, info, bc); return KBLAS_Success; }
+    if (nrhs == 0 && mode == KB_MODE_POTRS) return KBLAS_Success;
+    if (order <= KB_REG_MAX_N)
+        return launched(kb_launch_reg_chol(prec, mode, order, nrhs, sd == 0, A, B, info, bc, h->stream));
+    return launched(kb_launch_blk_chol(prec, mode, order, nrhs, sd == 0, A, B, info, bc, h->stream));
+}
+
+int kblasDpotrs_batch(kblasHandle_t h, char side, char uplo, const int m, const int n,
+                      const double** A, int lda, double** B, int ldb, int bc)
+{ return solve_core(h, 'd', KB_MODE_POTRS, side, uplo, m, n, mat_ptrs(A, lda), 0, mat_ptrs(B, ldb), 0, 0, bc, NULL); }
+int kblasSpotrs_batch(kblasHandle_t h, char side, char uplo, const int m, const int n,
+                      const float** A, int lda, float** B, int ldb, int bc)
+{ return solve_core(h, 's', KB_MODE_POTRS, side, uplo, m, n, mat_ptrs(A, lda), 0, mat_ptrs(B, ldb), 0, 0, bc, NULL); }
+int kblasDpotrs_batch_strided(kblasHandle_t h, char side, char uplo, const int m, const int n,
+                              const double* A, int lda, long sA, double* B, int ldb, long sB, int bc)
+{ return solve_core(h, 'd', KB_MODE_POTRS, side, uplo, m, n, mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB, 1, bc, NULL); }
+int kblasSpotrs_batch_strided(kblasHandle_t h, char side, char uplo, const int m, const int n,
+                              const float* A, int lda, long sA, float* B, int ldb, long sB, int bc)
+{ return solve_core(h, 's', KB_MODE_POTRS, side, uplo, m, n, mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB, 1, bc, NULL); }
+
+int kblasDposv_batch(kblasHandle_t h, char side, char uplo, const int m, const int n,
+                     double** A, int lda, double** B, int ldb, int bc, int* info)
+{ return solve_core(h, 'd', KB_MODE_POSV, side, uplo, m, n, mat_ptrs(A, lda), 0, mat_ptrs(B, ldb), 0, 0, bc, info); }
+int kblasSposv_batch(kblasHandle_t h, char side, char uplo, const int m, const int n,
+                     float** A, int lda, float** B, int ldb, int bc, int* info)
+{ return solve_core(h, 's', KB_MODE_POSV, side, uplo, m, n, mat_ptrs(A, lda), 0, mat_ptrs(B, ldb), 0, 0, bc, info); }
+int kblasDposv_batch_strided(kblasHandle_t h, char side, char uplo, const int m, const int n,
+                             double* A, int lda, long sA, double* B, int ldb, long sB, int bc, int* info)
+{ return solve_core(h, 'd', KB_MODE_POSV, side, uplo, m, n, mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB, 1, bc, info); }
+int kblasSposv_batch_strided(kblasHandle_t h, char side, char uplo, const int m, const int n,
+                             float* A, int lda, long sA, float* B, int ldb, long sB, int bc, int* info)
+{ return solve_core(h, 's', KB_MODE_POSV, side, uplo, m, n, mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB, 1, bc, info); }
+
+/* ======================= mixed-precision POSV ============================ */
+static int sposv_core(kblasHandle_t h, char side, char uplo, int m, int n, kb_mat_t A, long sA,
+                      kb_mat_t B, long sB, kb_mat_t X, long sX, int strided, int bc,
+                      int* info, int* iter)
+{
+    KB_CHECK_HANDLE(h);
+    int sd = sideof(side), u = upl(uplo);
+    if (sd < 0 || u < 0 || m < 0 || n < 0 || bc < 0) return KBLAS_Error_WrongInput;
+    int order = sd == 0 ? m : n;
+    int nrhs = sd == 0 ? n : m;
+    if (A.ld < IMAX(1, order) || B.ld < IMAX(1, m) || X.ld < IMAX(1, m)) return KBLAS_Error_WrongInput;
+    if (u == 1) return KBLAS_NotImplemented;
+    if (bc == 0) return KBLAS_Success;
+    if (strided) {
+        if (!A.base || !B.base || !X.base || sA < (long)A.ld * order || sB < 0 ||
+            sX < (long)X.ld * n) return KBLAS_Error_WrongInput;
+    } else if (!A.ptrs || !B.ptrs || !X.ptrs) {
+        return KBLAS_Error_WrongInput;
+    }
+    if (order == 0 || nrhs == 0) {
+        zero_info(h, info, bc);
+        if (iter) kb_launch_fill_int(iter, 0, bc, h->stream);
+        if (order == 0 || nrhs == 0) return KBLAS_Success;
+    }
+    if (order <= KB_REG_MAX_N)
+        return launched(kb_launch_reg_sposv(order, nrhs, sd == 0, A, B, X, info, iter, bc, h->stream));
+    return launched(kb_launch_blk_sposv(order, nrhs, sd == 0, A, B, X, info, iter, bc, h->stream));
+}
+
+int kblasDsposv_batch_strided(kblasHandle_t h, char side, char uplo, const int m, const int n,
+                              double* A, int lda, long sA, const double* B, int ldb, long sB,
+                              double* X, int ldx, long sX, int bc, int* info, int* iter)
+{
+    return sposv_core(h, side, uplo, m, n, mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB,
+                      mat_strided(X, ldx, sX), sX, 1, bc, info, iter);
+}
+int kblasDsposv_batch(kblasHandle_t h, char side, char uplo, const int m, const int n,
+                      double** A, int lda, const double** B, int ldb, double** X, int ldx,
+                      int bc, int* info, int* iter)
+{
+    return sposv_core(h, side, uplo, m, n, mat_ptrs(A, lda), 0, mat_ptrs(B, ldb), 0,
+                      mat_ptrs(X, ldx), 0, 0, bc, info, iter);
+}
+
+/* ================================ TRSM =================================== */
+static int trsm_core(kblasHandle_t h, char prec, char side, char uplo, char trans, char diag,
+                     int m, int n, double alpha, kb_mat_t A, long sA, kb_mat_t B, long sB,
+                     int strided, int bc)
+{
+    KB_CHECK_HANDLE(h);
+    int sd = sideof(side), u = upl(uplo), t = transof(trans), dg = diagof(diag);
+    if (sd < 0 || u < 0 || t < 0 || dg < 0 || m < 0 || n < 0 || bc < 0) return KBLAS_Error_WrongInput;
+    int ka = sd == 0 ? m : n;
+    if (A.ld < IMAX(1, ka) || B.ld < IMAX(1, m)) return KBLAS_Error_WrongInput;
+    /* src/batch_triangular/Xtrsm_batch_drivers.cuh:64-67 */
+    if (u == 1 || dg == 1) return KBLAS_NotImplemented;
+    if (bc == 0 || m == 0 || n == 0) return KBLAS_Success;
+    if (strided) {
+        if (!A.base || !B.base || sA < 0 || sB < (long)B.ld * n) return KBLAS_Error_WrongInput;
+    } else if (!A.ptrs || !B.ptrs) {
+        return KBLAS_Error_WrongInput;
+    }
+    return launched(kb_launch_trsm(prec, sd == 0, t, dg, m, n, alpha, A, B, NULL, NULL, bc, h->stream));
+}
+
+int kblasDtrsm_batch(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m,
+                     const int n, const double alpha, const double** A, int lda, double** B, int ldb, int bc)
+{ return trsm_core(h, 'd', side, uplo, trans, diag, m, n, alpha, mat_ptrs(A, lda), 0, mat_ptrs(B, ldb), 0, 0, bc); }
+int kblasStrsm_batch(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m,
+                     const int n, const float alpha, const float** A, int lda, float** B, int ldb, int bc)
+{ return trsm_core(h, 's', side, uplo, trans, diag, m, n, alpha, mat_ptrs(A, lda), 0, mat_ptrs(B, ldb), 0, 0, bc); }
+int kblasDtrsm_batch_strided(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m,
+                             const int n, const double alpha, const double* A, int lda, long sA,
+                             double* B, int ldb, long sB, int bc)
+{ return trsm_core(h, 'd', side, uplo, trans, diag, m, n, alpha, mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB, 1, bc); }
+int kblasStrsm_batch_strided(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m,
+                             const int n, const float alpha, const float* A, int lda, long sA,
+                             float* B, int ldb, long sB, int bc)
+{ return trsm_core(h, 's', side, uplo, trans, diag, m, n, alpha, mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB, 1, bc); }
+
+/* ================================ SYRK =================================== */
+static int syrk_core(kblasHandle_t h, char prec, char uplo, char trans, int m, int n, double alpha,
+                     kb_mat_t A, long sA, double beta, kb_mat_t B, long sB, int strided, int bc)
+{
+    KB_CHECK_HANDLE(h);
+    int u = upl(uplo), t = transof(trans);
+    if (u < 0 || t < 0 || m < 0 || n < 0 || bc < 0) return KBLAS_Error_WrongInput;
+    if (A.ld < IMAX(1, t ? n : m) || B.ld < IMAX(1, m)) return KBLAS_Error_WrongInput;
+    /* src/batch_triangular/Xsyrk_batch_drivers.cuh:133-136 */
+    if (u == 1) return KBLAS_NotImplemented;
+    if (bc == 0 || m == 0) return KBLAS_Success;
+    if (strided) {
+        if (!A.base || !B.base || sA < 0 || sB < (long)B.ld * m) return KBLAS_Error_WrongInput;
+    } else if (!A.ptrs || !B.ptrs) {
+        return KBLAS_Error_WrongInput;
+    }
+    return launched(kb_launch_syrk(prec, t, m, n, alpha, A, beta, B, NULL, NULL, bc, h->stream));
+}
+
+int kblasDsyrk_batch(kblasHandle_t h, char uplo, char trans, const int m, const int n, const double alpha,
+                     const double** A, int lda, const double beta, double** B, int ldb, int bc)
+{ return syrk_core(h, 'd', uplo, trans, m, n, alpha, mat_ptrs(A, lda), 0, beta, mat_ptrs(B, ldb), 0, 0, bc); }
+int kblasSsyrk_batch(kblasHandle_t h, char uplo, char trans, const int m, const int n, const float alpha,
+                     const float** A, int lda, const float beta, float** B, int ldb, int bc)
+{ return syrk_core(h, 's', uplo, trans, m, n, alpha, mat_ptrs(A, lda), 0, beta, mat_ptrs(B, ldb), 0, 0, bc); }
+int kblasDsyrk_batch_strided(kblasHandle_t h, char uplo, char trans, const int m, const int n,
+                             const double alpha, const double* A, int lda, long sA,
+                             const double beta, double* B, int ldb, long sB, int bc)
+{ return syrk_core(h, 'd', uplo, trans, m, n, alpha, mat_strided(A, lda, sA), sA, beta, mat_strided(B, ldb, sB), sB, 1, bc); }
+int kblasSsyrk_batch_strided(kblasHandle_t h, char uplo, char trans, const int m, const int n,
+                             const float alpha, const float* A, int lda, long sA,
+                             const float beta, float* B, int ldb, long sB, int bc)
+{ return syrk_core(h, 's', uplo, trans, m, n, alpha, mat_strided(A, lda, sA), sA, beta, mat_strided(B, ldb, sB), sB, 1, bc); }
+
+/* ================================ GEMM =================================== */
+static int gemm_core(kblasHandle_t h, char prec, char ta, char tb, int m, int n, int k, double alpha,
+                     kb_mat_t A, long sA, kb_mat_t B, long sB, double beta, kb_mat_t C, long sC,
+                     int strided, int bc)
+{
+    KB_CHECK_HANDLE(h);
+    int a = transof(ta), b = transof(tb);
+    if (a < 0 || b < 0 || m < 0 || n < 0 || k < 0) return KBLAS_Error_WrongInput;
+    /* src/batch_triangular/Xgemm_batch_core.cuh:181-182 rejects batchCount < 1 */
+    if (bc < 0) return KBLAS_Error_WrongInput;
+    if (A.ld < IMAX(1, a ? k : m) || B.ld < IMAX(1, b ? n : k) || C.ld < IMAX(1, m))
+        return KBLAS_Error_WrongInput;
+    if (bc == 0 || m == 0 || n == 0) return KBLAS_Success;
+    if (strided) {
+        if (!A.base || !B.base || !C.base || sA < 0 || sB < 0 || sC < (long)C.ld * n)
+            return KBLAS_Error_WrongInput;
+    } else if (!A.ptrs || !B.ptrs || !C.ptrs) {
+        return KBLAS_Error_WrongInput;
+    }
+    return launched(kb_launch_gemm(prec, a, b, m, n, k, alpha, A, B, beta, C, NULL, NULL, NULL, bc, h->stream));
+}
+
+int kblasDgemm_batch(kblasHandle_t h, char ta, char tb, const int m, const int n, const int k,
+                     const double alpha, const double** A, int lda, const double** B, int ldb,
+                     const double beta, double** C, int ldc, int bc)
+{ return gemm_core(h, 'd', ta, tb, m, n, k, alpha, mat_ptrs(A, lda), 0, mat_ptrs(B, ldb), 0, beta, mat_ptrs(C, ldc), 0, 0, bc); }
+int kblasSgemm_batch(kblasHandle_t h, char ta, char tb, const int m, const int n, const int k,
+                     const float alpha, const float** A, int lda, const float** B, int ldb,
+                     const float beta, float** C, int ldc, int bc)
+{ return gemm_core(h, 's', ta, tb, m, n, k, alpha, mat_ptrs(A, lda), 0, mat_ptrs(B, ldb), 0, beta, mat_ptrs(C, ldc), 0, 0, bc); }
+int kblasDgemm_batch_strided(kblasHandle_t h, char ta, char tb, const int m, const int n, const int k,
+                             const double alpha, const double* A, int lda, long sA,
+                             const double* B, int ldb, long sB, const double beta,
+                             double* C, int ldc, long sC, int bc)
+{ return gemm_core(h, 'd', ta, tb, m, n, k, alpha, mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB, beta, mat_strided(C, ldc, sC), sC, 1, bc); }
+int kblasSgemm_batch_strided(kblasHandle_t h, char ta, char tb, const int m, const int n, const int k,
+                             const float alpha, const float* A, int lda, long sA,
+                             const float* B, int ldb, long sB, const float beta,
+                             float* C, int ldc, long sC, int bc)
+{ return gemm_core(h, 's', ta, tb, m, n, k, alpha, mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB, beta, mat_strided(C, ldc, sC), sC, 1, bc); }
+
+/* ============================ non-uniform ================================ */
+/* Per-matrix sizes live on the device, so only host-visible arguments are
+ * validated here; a matrix with a non-positive size is skipped by the
+ * kernels (its info is 0). */
+static int vcheck(kblasHandle_t h, int bc, const void* a, const void* b)
+{
+    if (!h) return KBLAS_Error_NotInitialized;
+    if (bc < 0) return KBLAS_Error_WrongInput;
+    if (bc > 0 && (!a || !b)) return KBLAS_Error_WrongInput;
+    return KBLAS_Success;
+}
+
+int kblasDgemm_vbatch(kblasHandle_t h, char ta, char tb, int* m, int* n, int* k, int max_m, int max_n,
+                      int max_k, const double alpha, const double** A, int* lda, const double** B,
+                      int* ldb, const double beta, double** C, int* ldc, int bc)
+{
+    int st = vcheck(h, bc, A, C);
+    if (st != KBLAS_Success) return st;
+    int a = transof(ta), b = transof(tb);
+    if (a < 0 || b < 0 || !m || !n || !k || !lda || !ldb || !ldc || !B) return KBLAS_Error_WrongInput;
+    if (bc == 0) return KBLAS_Success;
+    (void)max_m; (void)max_n; (void)max_k;
+    return launched(kb_launch_gemm('d', a, b, 0, 0, 0, alpha, mat_vptrs(A, lda), mat_vptrs(B, ldb), beta,
+                                   mat_vptrs(C, ldc), m, n, k, bc, h->stream));
+}
+int kblasSgemm_vbatch(kblasHandle_t h, char ta, char tb, int* m, int* n, int* k, int max_m, int max_n,
+                      int max_k, const float alpha, const float** A, int* lda, const float** B,
+                      int* ldb, const float beta, float** C, int* ldc, int bc)
+{
+    int st = vcheck(h, bc, A, C);
+    if (st != KBLAS_Success) return st;
+    int a = transof(ta), b = transof(tb);
+    if (a < 0 || b < 0 || !m || !n || !k || !lda || !ldb || !ldc || !B) return KBLAS_Error_WrongInput;
+    if (bc == 0) return KBLAS_Success;
+    (void)max_m; (void)max_n; (void)max_k;
+    return launched(kb_launch_gemm('s', a, b, 0, 0, 0, alpha, mat_vptrs(A, lda), mat_vptrs(B, ldb), beta,
+                                   mat_vptrs(C, ldc), m, n, k, bc, h->stream));
+}
+
+static int syrk_v(kblasHandle_t h, char prec, char uplo, char trans, int* m, int* n, double alpha,
+                  void* A, int* lda, double beta, void* B, int* ldb, int bc)
+{
+    int st = vcheck(h, bc, A, B);
+    if (st != KBLAS_Success) return st;
+    int u = upl(uplo), t = transof(trans);
+    if (u < 0 || t < 0 || !m || !n || !lda || !ldb) return KBLAS_Error_WrongInput;
+    if (u == 1) return KBLAS_NotImplemented;
+    if (bc == 0) return KBLAS_Success;
+    return launched(kb_launch_syrk(prec, t, 0, 0, alpha, mat_vptrs(A, lda), beta, mat_vptrs(B, ldb),
+                                   m, n, bc, h->stream));
+}
+int kblasDsyrk_vbatch(kblasHandle_t h, char uplo, char trans, int* m, int* n, int max_m, int max_n,
+                      double alpha, double** A, int* lda, double beta, double** B, int* ldb, int bc)
+{ (void)max_m; (void)max_n; return syrk_v(h, 'd', uplo, trans, m, n, alpha, A, lda, beta, B, ldb, bc); }
+int kblasSsyrk_vbatch(kblasHandle_t h, char uplo, char trans, int* m, int* n, int max_m, int max_n,
+                      float alpha, float** A, int* lda, float beta, float** B, int* ldb, int bc)
+{ (void)max_m; (void)max_n; return syrk_v(h, 's', uplo, trans, m, n, alpha, A, lda, beta, B, ldb, bc); }
+
+static int trsm_v(kblasHandle_t h, char prec, char side, char uplo, char trans, char diag, int* m, int* n,
+                  double alpha, void* A, int* lda, void* B, int* ldb, int bc)
+{
+    int st = vcheck(h, bc, A, B);
+    if (st != KBLAS_Success) return st;
+    int sd = sideof(side), u = upl(uplo), t = transof(trans), dg = diagof(diag);
+    if (sd < 0 || u < 0 || t < 0 || dg < 0 || !m || !n || !lda || !ldb) return KBLAS_Error_WrongInput;
+    if (u == 1 || dg == 1) return KBLAS_NotImplemented;
+    if (bc == 0) return KBLAS_Success;
+    return launched(kb_launch_trsm(prec, sd == 0, t, dg, 0, 0, alpha, mat_vptrs(A, lda), mat_vptrs(B, ldb),
+                                   m, n, bc, h->stream));
+}
+int kblasDtrsm_vbatch(kblasHandle_t h, char side, char uplo, char trans, char diag, int* m, int* n,
+                      int max_m, int max_n, double alpha, double** A, int* lda, double** B, int* ldb, int bc)
+{ (void)max_m; (void)max_n; return trsm_v(h, 'd', side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, bc); }
+int kblasStrsm_vbatch(kblasHandle_t h, char side, char uplo, char trans, char diag, int* m, int* n,
+                      int max_m, int max_n, float alpha, float** A, int* lda, float** B, int* ldb, int bc)
+{ (void)max_m; (void)max_n; return trsm_v(h, 's', side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, bc); }
+
+/* ---- non-uniform Cholesky family: workspace = the size-sorted queue ---- */
+void kb_ws_need_vbatch(kb_ws_state* ws, int batchCount)
+{
+    memset(ws, 0, sizeof(*ws));
+    ws->d_data = kb_vchol_ws_bytes(batchCount > 0 ? batchCount : 0);
+}
+
+void kblas_potrf_vbatch_wsquery(kblasHandle_t h, int max_n, int batchCount)
+{
+    (void)max_n;
+    if (!h) return;
+    kb_ws_state need;
+    kb_ws_need_vbatch(&need, batchCount);
+    kb_ws_request(h, &need);
+}
+void kblas_posv_vbatch_wsquery(kblasHandle_t h, int max_n, int batchCount)
+{ kblas_potrf_vbatch_wsquery(h, max_n, batchCount); }
+
+static int vchol(kblasHandle_t h, char prec, int mode, char side, char uplo, int* m, int* n, int max_n,
+                 void* A, int* lda, void* B, int* ldb, void* X, int* ldx, int bc, int* info, int* iter)
+{
+    if (!h) return KBLAS_Error_NotInitialized;
+    int u = upl(uplo);
+    int sd = mode == KB_MODE_POTRF ? 1 : sideof(side);
+    if (u < 0 || sd < 0 || bc < 0 || !n || !A || !lda) return KBLAS_Error_WrongInput;
+    if (mode != KB_MODE_POTRF && (!m || !B || !ldb)) return KBLAS_Error_WrongInput;
+    if (mode == KB_MODE_SPOSV && (!X || !ldx)) return KBLAS_Error_WrongInput;
+    if (u == 1) return KBLAS_NotImplemented;
+    if (bc == 0) return KBLAS_Success;
+    kb_ws_state need;
+    kb_ws_need_vbatch(&need, bc);
+    if (!kb_ws_sufficient(&need, &h->allocated)) return KBLAS_InsufficientWorkspace;
+    kb_mat_t Am = mat_vptrs(A, lda);
+    kb_mat_t Bm = B ? mat_vptrs(B, ldb) : Am;
+    kb_mat_t Xm = X ? mat_vptrs(X, ldx) : Bm;
+    return launched(kb_launch_vchol(prec, mode, m, n, sd == 0, Am, Bm, Xm, info, iter, bc, max_n,
+                                    h->d_data, h->stream));
+}
+
+int kblasDpotrf_vbatch(kblasHandle_t h, char uplo, int* n, int max_n, double** A, int* lda, int bc, int* info)
+{ return vchol(h, 'd', KB_MODE_POTRF, 'R', uplo, NULL, n, max_n, A, lda, NULL, NULL, NULL, NULL, bc, info, NULL); }
+int kblasSpotrf_vbatch(kblasHandle_t h, char uplo, int* n, int max_n, float** A, int* lda, int bc, int* info)
+{ return vchol(h, 's', KB_MODE_POTRF, 'R', uplo, NULL, n, max_n, A, lda, NULL, NULL, NULL, NULL, bc, info, NULL); }
+int kblasDpotrs_vbatch(kblasHandle_t h, char side, char uplo, int* m, int* n, int max_m, int max_n,
+                       double** A, int* lda, double** B, int* ldb, int bc)
+{ (void)max_m; return vchol(h, 'd', KB_MODE_POTRS, side, uplo, m, n, max_n, A, lda, B, ldb, NULL, NULL, bc, NULL, NULL); }
+int kblasDposv_vbatch(kblasHandle_t h, char side, char uplo, int* m, int* n, int max_m, int max_n,
+                      double** A, int* lda, double** B, int* ldb, int bc, int* info)
+{ (void)max_m; return vchol(h, 'd', KB_MODE_POSV, side, uplo, m, n, max_n, A, lda, B, ldb, NULL, NULL, bc, info, NULL); }
+int kblasSposv_vbatch(kblasHandle_t h, char side, char uplo, int* m, int* n, int max_m, int max_n,
+                      float** A, int* lda, float** B, int* ldb, int bc, int* info)
+{ (void)max_m; return vchol(h, 's', KB_MODE_POSV, side, uplo, m, n, max_n, A, lda, B, ldb, NULL, NULL, bc, info, NULL); }
+int kblasDsposv_vbatch(kblasHandle_t h, char side, char uplo, int* m, int* n, int max_m, int max_n,
+                       double** A, int* lda, const double** B, int* ldb, double** X, int* ldx, int bc,
+                       int* info, int* iter)
+{ (void)max_m; return vchol(h, 'm', KB_MODE_SPOSV, side, uplo, m, n, max_n, A, lda, (void*)B, ldb, X, ldx, bc, info, iter); }
+
+/* ====================== workspace queries (all zero) ===================== */
+/* The uniform paths run without any workspace: no pointer arrays are built
+ * (the reference needs them for its cuBLAS GEMM steps,
+ * src/workspace_queries.cu:188-238).  The queries are kept for source
+ * compatibility of the query -> allocate -> call protocol. */
+void kblas_gemm_batch_strided_wsquery(kblasHandle_t h, int bc) { (void)h; (void)bc; }
+void kblas_gemm_batch_nonuniform_wsquery(kblasHandle_t h) { (void)h; }
+void kblas_syrk_batch_wsquery(kblasHandle_t h, const int m, int bc) { (void)h; (void)m; (void)bc; }
+void kblas_syrk_batch_nonuniform_wsquery(kblasHandle_t h) { (void)h; }
+void kblas_trsm_batch_wsquery(kblasHandle_t h, char side, int m, int n, int bc) { (void)h; (void)side; (void)m; (void)n; (void)bc; }
+void kblas_trsm_batch_strided_wsquery(kblasHandle_t h, char side, int m, int n, int bc) { (void)h; (void)side; (void)m; (void)n; (void)bc; }
+void kblas_trsm_batch_nonuniform_wsquery(kblasHandle_t h) { (void)h; }
+void kblas_potrf_batch_wsquery(kblasHandle_t h, const int n, int bc) { (void)h; (void)n; (void)bc; }
+void kblas_potrf_batch_strided_wsquery(kblasHandle_t h, const int n, int bc) { (void)h; (void)n; (void)bc; }
+void kblas_potrs_batch_wsquery(kblasHandle_t h, const int m, const int n, int bc) { (void)h; (void)m; (void)n; (void)bc; }
+void kblas_potrs_batch_strided_wsquery(kblasHandle_t h, const int m, const int n, int bc) { (void)h; (void)m; (void)n; (void)bc; }
+void kblas_posv_batch_wsquery(kblasHandle_t h, char side, const int m, const int n, int bc) { (void)h; (void)side; (void)m; (void)n; (void)bc; }
+void kblas_posv_batch_strided_wsquery(kblasHandle_t h, char side, const int m, const int n, int bc) { (void)h; (void)side; (void)m; (void)n; (void)bc; }
+
+/* ============ complex precisions: link-compatible, not implemented ======= */
+#define KB_NI return KBLAS_NotImplemented
+int kblasCgemm_batch(kblasHandle_t h, char ta, char tb, const int m, const int n, const int k, const hipFloatComplex alpha, const hipFloatComplex** A, int lda, const hipFloatComplex** B, int ldb, const hipFloatComplex beta, hipFloatComplex** C, int ldc, int bc) { KB_NI; }
+int kblasZgemm_batch(kblasHandle_t h, char ta, char tb, const int m, const int n, const int k, const hipDoubleComplex alpha, const hipDoubleComplex** A, int lda, const hipDoubleComplex** B, int ldb, const hipDoubleComplex beta, hipDoubleComplex** C, int ldc, int bc) { KB_NI; }
+int kblasCgemm_batch_strided(kblasHandle_t h, char ta, char tb, const int m, const int n, const int k, const hipFloatComplex alpha, const hipFloatComplex* A, int lda, long sA, const hipFloatComplex* B, int ldb, long sB, const hipFloatComplex beta, hipFloatComplex* C, int ldc, long sC, int bc) { KB_NI; }
+int kblasZgemm_batch_strided(kblasHandle_t h, char ta, char tb, const int m, const int n, const int k, const hipDoubleComplex alpha, const hipDoubleComplex* A, int lda, long sA, const hipDoubleComplex* B, int ldb, long sB, const hipDoubleComplex beta, hipDoubleComplex* C, int ldc, long sC, int bc) { KB_NI; }
+int kblasCsyrk_batch(kblasHandle_t h, char uplo, char trans, const int m, const int n, const hipFloatComplex alpha, const hipFloatComplex** A, int lda, const hipFloatComplex beta, hipFloatComplex** B, int ldb, int bc) { KB_NI; }
+int kblasZsyrk_batch(kblasHandle_t h, char uplo, char trans, const int m, const int n, const hipDoubleComplex alpha, const hipDoubleComplex** A, int lda, const hipDoubleComplex beta, hipDoubleComplex** B, int ldb, int bc) { KB_NI; }
+int kblasCsyrk_batch_strided(kblasHandle_t h, char uplo, char trans, const int m, const int n, const hipFloatComplex alpha, const hipFloatComplex* A, int lda, long sA, const hipFloatComplex beta, hipFloatComplex* B, int ldb, long sB, int bc) { KB_NI; }
+int kblasZsyrk_batch_strided(kblasHandle_t h, char uplo, char trans, const int m, const int n, const hipDoubleComplex alpha, const hipDoubleComplex* A, int lda, long sA, const hipDoubleComplex beta, hipDoubleComplex* B, int ldb, long sB, int bc) { KB_NI; }
+int kblasCtrsm_batch(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n, const hipFloatComplex alpha, const hipFloatComplex** A, int lda, hipFloatComplex** B, int ldb, int bc) { KB_NI; }
+int kblasZtrsm_batch(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n, const hipDoubleComplex alpha, const hipDoubleComplex** A, int lda, hipDoubleComplex** B, int ldb, int bc) { KB_NI; }
+int kblasCtrsm_batch_strided(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n, const hipFloatComplex alpha, const hipFloatComplex* A, int lda, long sA, hipFloatComplex* B, int ldb, long sB, int bc) { KB_NI; }
+int kblasZtrsm_batch_strided(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n, const hipDoubleComplex alpha, const hipDoubleComplex* A, int lda, long sA, hipDoubleComplex* B, int ldb, long sB, int bc) { KB_NI; }
+int kblasCpotrf_batch(kblasHandle_t h, char uplo, const int n, hipFloatComplex** A, int lda, int bc, int* info) { KB_NI; }
+int kblasZpotrf_batch(kblasHandle_t h, char uplo, const int n, hipDoubleComplex** A, int lda, int bc, int* info) { KB_NI; }
+int kblasCpotrf_batch_strided(kblasHandle_t h, char uplo, const int n, hipFloatComplex* A, int lda, long sA, int bc, int* info) { KB_NI; }
+int kblasZpotrf_batch_strided(kblasHandle_t h, char uplo, const int n, hipDoubleComplex* A, int lda, long sA, int bc, int* info) { KB_NI; }
+int kblasCpotrs_batch(kblasHandle_t h, char side, char uplo, const int m, const int n, const hipFloatComplex** A, int lda, hipFloatComplex** B, int ldb, int bc) { KB_NI; }
+int kblasZpotrs_batch(kblasHandle_t h, char side, char uplo, const int m, const int n, const hipDoubleComplex** A, int lda, hipDoubleComplex** B, int ldb, int bc) { KB_NI; }
+int kblasCpotrs_batch_strided(kblasHandle_t h, char side, char uplo, const int m, const int n, const hipFloatComplex* A, int lda, long sA, hipFloatComplex* B, int ldb, long sB, int bc) { KB_NI; }
+int kblasZpotrs_batch_strided(kblasHandle_t h, char side, char uplo, const int m, const int n, const hipDoubleComplex* A, int lda, long sA, hipDoubleComplex* B, int ldb, long sB, int bc) { KB_NI; }
+int kblasCposv_batch(kblasHandle_t h, char side, char uplo, const int m, const int n, hipFloatComplex** A, int lda, hipFloatComplex** B, int ldb, int bc, int* info) { KB_NI; }
+int kblasZposv_batch(kblasHandle_t h, char side, char uplo, const int m, const int n, hipDoubleComplex** A, int lda, hipDoubleComplex** B, int ldb, int bc, int* info) { KB_NI; }
+int kblasCposv_batch_strided(kblasHandle_t h, char side, char uplo, const int m, const int n, hipFloatComplex* A, int lda, long sA, hipFloatComplex* B, int ldb, long sB, int bc, int* info) { KB_NI; }
+int kblasZposv_batch_strided(kblasHandle_t h, char side, char uplo, const int m, const int n, hipDoubleComplex* A, int lda, long sA, hipDoubleComplex* B, int ldb, long sB, int bc, int* info) { KB_NI; }

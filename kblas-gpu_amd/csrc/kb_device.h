@@ -1,0 +1,141 @@
+// kb_device.h -- device-side building blocks for the gfx950 batched SPD kernels.
+//
+// Lane-group broadcasts replace the reference's sub-warp shuffles
+// (include/kblas_operators.h:84-105, `shfl(x, lane, width)` built from two
+// 32-bit __shfl_sync for FP64).  On CDNA4 a 16-lane group is one DPP row, so
+// the broadcast is a single `v_mov_b64_dpp row_newbcast:J`; a 32-lane group
+// uses ds_swizzle in bit-mask mode (no LDS storage, no bank conflicts); a
+// 64-lane group has one matrix per wavefront, so the source lane is
+// wave-uniform and v_readlane puts the value in an SGPR that every FMA can
+// read directly.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define KB_DEV __device__ __forceinline__
+
+// ---------------------------------------------------------------------------
+// Batch operand: strided base, array of device pointers, or per-matrix leading
+// dimensions.  One code path serves the reference's strided and pointer-array
+// entry points and the non-uniform ones.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct BatchArg {
+    T* base;             // strided mode: first matrix
+    long stride;         // strided mode: elements between matrices
+    T* const* ptrs;      // pointer-array mode (device array of device pointers)
+    const int* lds;      // non-uniform leading dimensions (device), or nullptr
+    int ld;              // uniform leading dimension
+    KB_DEV T* get(long b) const { return ptrs ? ptrs[b] : base + b * stride; }
+    KB_DEV int ldim(long b) const { return lds ? lds[b] : ld; }
+};
+
+// ---------------------------------------------------------------------------
+// bit casts
+// ---------------------------------------------------------------------------
+KB_DEV int f2i(float x) { return __builtin_bit_cast(int, x); }
+KB_DEV float i2f(int x) { return __builtin_bit_cast(float, x); }
+KB_DEV int2 d2i2(double x) { return __builtin_bit_cast(int2, x); }
+KB_DEV double i22d(int2 x) { return __builtin_bit_cast(double, x); }
+
+// ---------------------------------------------------------------------------
+// Broadcast lane J of each G-lane group to the whole group (J compile-time).
+// ---------------------------------------------------------------------------
+template <int G, int J>
+KB_DEV double bcast(double x)
+{
+    static_assert(J >= 0 && J < G, "lane out of group");
+    if constexpr (G == 16) {
+        return __builtin_amdgcn_update_dpp(0.0, x, 0x150 + J, 0xF, 0xF, false);
+    } else if constexpr (G == 32) {
+        int2 v = d2i2(x);
+        v.x = __builtin_amdgcn_ds_swizzle(v.x, (J << 5));
+        v.y = __builtin_amdgcn_ds_swizzle(v.y, (J << 5));
+        return i22d(v);
+    } else {
+        int2 v = d2i2(x);
+        v.x = __builtin_amdgcn_readlane(v.x, J);
+        v.y = __builtin_amdgcn_readlane(v.y, J);
+        return i22d(v);
+    }
+}
+
+template <int G, int J>
+KB_DEV float bcast(float x)
+{
+    static_assert(J >= 0 && J < G, "lane out of group");
+    if constexpr (G == 16) {
+        return __builtin_amdgcn_update_dpp(0.0f, x, 0x150 + J, 0xF, 0xF, false);
+    } else if constexpr (G == 32) {
+        return i2f(__builtin_amdgcn_ds_swizzle(f2i(x), (J << 5)));
+    } else {
+        return i2f(__builtin_amdgcn_readlane(f2i(x), J));
+    }
+}
+
+// Broadcast with a lane index that becomes a compile-time constant once the
+// enclosing loop is fully unrolled (the switch folds away).  G = 64 reads an
+// SGPR-uniform lane directly.
+#define KB_BC16(i) case i: return bcast<16, i>(x);
+#define KB_BC32(i) case i: return bcast<32, i>(x);
+#define KB_REP16(M, o) M(o+0) M(o+1) M(o+2) M(o+3) M(o+4) M(o+5) M(o+6) M(o+7) \
+    M(o+8) M(o+9) M(o+10) M(o+11) M(o+12) M(o+13) M(o+14) M(o+15)
+
+template <int G, typename T>
+KB_DEV T bcastk(T x, int j)
+{
+    if constexpr (G == 16) {
+        switch (j) { KB_REP16(KB_BC16, 0) default: return x; }
+    } else if constexpr (G == 32) {
+        switch (j) { KB_REP16(KB_BC32, 0) KB_REP16(KB_BC32, 16) default: return x; }
+    } else {
+        if constexpr (sizeof(T) == 8) {
+            int2 v = d2i2(x);
+            v.x = __builtin_amdgcn_readlane(v.x, j);
+            v.y = __builtin_amdgcn_readlane(v.y, j);
+            return i22d(v);
+        } else {
+            return i2f(__builtin_amdgcn_readlane(f2i(x), j));
+        }
+    }
+}
+
+// Runtime-lane broadcast within a G-lane group (ds_bpermute); used off the hot loops.
+template <int G, typename T>
+KB_DEV T bcast_rt(T x, int j)
+{
+    int lane = __lane_id();
+    int src = (lane & ~(G - 1)) + j;
+    if constexpr (sizeof(T) == 8) {
+        int2 v = d2i2(x);
+        v.x = __builtin_amdgcn_ds_bpermute(src << 2, v.x);
+        v.y = __builtin_amdgcn_ds_bpermute(src << 2, v.y);
+        return i22d(v);
+    } else {
+        return i2f(__builtin_amdgcn_ds_bpermute(src << 2, f2i(x)));
+    }
+}
+
+// Group-wide max (all lanes of the G-lane group get the result).
+template <int G, typename T>
+KB_DEV T group_max(T x)
+{
+#pragma unroll
+    for (int off = 1; off < G; off <<= 1) {
+        T y = __shfl_xor(x, off, 64);
+        x = (y > x || y != y) ? y : x;   // NaN wins so a failed solve is never reported converged
+    }
+    return x;
+}
+
+template <int G>
+KB_DEV int group_all(int pred)
+{
+#pragma unroll
+    for (int off = 1; off < G; off <<= 1) pred &= __shfl_xor(pred, off, 64);
+    return pred;
+}
+
+template <typename T> KB_DEV T kb_sqrt(T x);
+template <> KB_DEV double kb_sqrt<double>(double x) { return __builtin_sqrt(x); }
+template <> KB_DEV float kb_sqrt<float>(float x) { return __builtin_sqrtf(x); }

@@ -1,0 +1,79 @@
+/* kb_launch.h -- C interface between the host library (C) and the HIP kernels.
+ *
+ * Every launcher is asynchronous on the given stream, allocates nothing and
+ * returns a hipError_t value (0 = hipSuccess).  Argument validation is done by
+ * the host layer before these are called.
+ */
+#ifndef KB_LAUNCH_H
+#define KB_LAUNCH_H
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* A batch operand: strided (base + b*stride), pointer array (ptrs[b]) and a
+ * uniform (ld) or per-matrix (lds[b]) leading dimension.  All device memory. */
+typedef struct {
+    void* base;
+    long stride;
+    void* const* ptrs;
+    const int* lds;
+    int ld;
+} kb_mat_t;
+
+enum { KB_MODE_POTRF = 0, KB_MODE_POSV = 1, KB_MODE_POTRS = 2 };
+
+/* Largest order served by the register-resident kernels. */
+#define KB_REG_MAX_N 64
+
+/* Register-resident uniform Cholesky family (n <= 64).
+ * prec 'd' or 's'.  side_left: 1 = A X = B (B is n x nrhs), 0 = X A = B
+ * (B is nrhs x n, the reference's Right side). */
+int kb_launch_reg_chol(char prec, int mode, int n, int nrhs, int side_left,
+                       kb_mat_t A, kb_mat_t B, int* info, int batch, hipStream_t s);
+
+/* Mixed precision POSV (FP32 factor + FP64 refinement + FP64 fallback), n <= 64.
+ * A read-only unless the fallback ran (then it holds L, as LAPACK dsposv).   */
+int kb_launch_reg_sposv(int n, int nrhs, int side_left, kb_mat_t A, kb_mat_t B, kb_mat_t X,
+                        int* info, int* iter, int batch, hipStream_t s);
+
+/* Blocked one-workgroup-per-matrix Cholesky family for any n (used for
+ * n > KB_REG_MAX_N); same arguments as kb_launch_reg_chol. */
+int kb_launch_blk_chol(char prec, int mode, int n, int nrhs, int side_left,
+                       kb_mat_t A, kb_mat_t B, int* info, int batch, hipStream_t s);
+int kb_launch_blk_sposv(int n, int nrhs, int side_left, kb_mat_t A, kb_mat_t B, kb_mat_t X,
+                        int* info, int* iter, int batch, hipStream_t s);
+
+/* Batched GEMM C := alpha op(A) op(B) + beta C.  vm/vn/vk: optional device
+ * arrays of per-matrix sizes (non-uniform); then m/n/k are upper bounds or 0. */
+int kb_launch_gemm(char prec, int transA, int transB, int m, int n, int k, double alpha,
+                   kb_mat_t A, kb_mat_t B, double beta, kb_mat_t C,
+                   const int* vm, const int* vn, const int* vk, int batch, hipStream_t s);
+/* Batched lower SYRK B := alpha op(A) op(A)^T + beta B (B is m x m, k inner). */
+int kb_launch_syrk(char prec, int trans, int m, int k, double alpha, kb_mat_t A,
+                   double beta, kb_mat_t B, const int* vm, const int* vk, int batch, hipStream_t s);
+/* Batched lower TRSM: op(A) X = alpha B (side_left) or X op(A) = alpha B. */
+int kb_launch_trsm(char prec, int side_left, int trans, int unit, int m, int n, double alpha,
+                   kb_mat_t A, kb_mat_t B, const int* vm, const int* vn, int batch, hipStream_t s);
+
+/* Non-uniform Cholesky family in one persistent launch over a size-sorted
+ * queue.  B[i] is vm[i] x vn[i]; the order of A[i] is vm[i] (side_left) or
+ * vn[i] (right side); mode POTRF passes vm = NULL and vn[i] = order.
+ * prec 'd', 's' or 'm' (mode SPOSV: FP32 factor + FP64 refinement, X receives
+ * the solution, iter as in dsposv).  ws: device workspace of
+ * kb_vchol_ws_bytes(batch) bytes. */
+enum { KB_MODE_SPOSV = 3 };
+size_t kb_vchol_ws_bytes(int batch);
+int kb_launch_vchol(char prec, int mode, const int* vm, const int* vn, int side_left,
+                    kb_mat_t A, kb_mat_t B, kb_mat_t X, int* info, int* iter, int batch,
+                    int max_n, void* ws, hipStream_t s);
+
+/* Fill count ints with v (a kernel, so it is safe inside captured graphs). */
+int kb_launch_fill_int(int* p, int v, long count, hipStream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,786 @@
+/*
+ * kblas_oracle.c -- CPU restatement of KBLAS's batched SPD path (checker only).
+ *
+ * TEST INFRASTRUCTURE.  Only tests/, the smoke entry and the benchmark's CPU
+ * baseline leg use this file.  The product library never links it.
+ *
+ * Compile with -ffp-contract=off: every fused multiply-add below is written as
+ * fma() on purpose, matching where the reference kernels use FMA(); all other
+ * products and sums must stay separately rounded so the generators are
+ * bit-reproducible against their HIP twins.
+ */
+#define _GNU_SOURCE
+#include "kblas_oracle.h"
+
+#include <dlfcn.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define IDX(A, lda, i, j) (A)[(long)(i) + (long)(j) * (long)(lda)]
+
+/* ======================================================================= */
+/* Counter-based generators                                                 */
+/* ======================================================================= */
+
+static inline uint64_t mix64(uint64_t x)
+{
+    /* SplitMix64 finaliser (Steele, Lea, Flood 2014). */
+    x ^= x >> 30;
+    x *= 0xBF58476D1CE4E5B9ULL;
+    x ^= x >> 27;
+    x *= 0x94D049BB133111EBULL;
+    x ^= x >> 31;
+    return x;
+}
+
+uint64_t orc_hash(uint64_t seed, uint64_t id, uint64_t k)
+{
+    uint64_t h = mix64(seed ^ 0x6A09E667F3BCC909ULL);
+    h = mix64(h + id * 0x9E3779B97F4A7C15ULL);
+    h = mix64(h + k * 0xD1B54A32D192ED03ULL + 1ULL);
+    return h;
+}
+
+/* 53-bit uniform in the open interval (0,1). */
+static inline double u01(uint64_t h)
+{
+    return ((double)(h >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+double orc_uniform(uint64_t seed, uint64_t id, uint64_t k)
+{
+    return 2.0 * u01(orc_hash(seed, id, k)) - 1.0;
+}
+
+/* Natural log for x > 0 from frexp + the atanh series; deterministic. */
+static double orc_log(double x)
+{
+    int e;
+    double m = frexp(x, &e);            /* x = m 2^e, m in [0.5,1) */
+    if (m < 0.70710678118654752) { m = m * 2.0; e = e - 1; }
+    double s = (m - 1.0) / (m + 1.0);   /* |s| <= 0.1716 */
+    double s2 = s * s;
+    double p = 1.0 / 25.0;
+    for (int t = 23; t >= 1; t -= 2) p = fma(p, s2, 1.0 / (double)t);
+    return (double)e * 0.69314718055994531 + 2.0 * s * p;
+}
+
+double orc_exp2(double x)
+{
+    double k = floor(x);
+    double y = (x - k) * 0.69314718055994531;   /* in [0, ln 2) */
+    double c[21];
+    c[0] = 1.0;
+    for (int i = 1; i <= 20; ++i) c[i] = c[i - 1] / (double)i;
+    double p = c[20];
+    for (int i = 19; i >= 0; --i) p = fma(p, y, c[i]);
+    return ldexp(p, (int)k);
+}
+
+/* Marsaglia polar method on the counter stream (k, attempt). */
+double orc_normal(uint64_t seed, uint64_t id, uint64_t k)
+{
+    for (uint64_t a = 0; a < 64; ++a) {
+        double u = 2.0 * u01(orc_hash(seed, id, (k << 7) | (2 * a))) - 1.0;
+        double v = 2.0 * u01(orc_hash(seed, id, (k << 7) | (2 * a + 1))) - 1.0;
+        double s = u * u;
+        s = fma(v, v, s);
+        if (s >= 1.0 || s == 0.0) continue;
+        return u * sqrt(-2.0 * orc_log(s) / s);
+    }
+    return 0.0;
+}
+
+void orc_gen_spd_shift(uint64_t seed, uint64_t id, int n, double shift, double* A, int lda)
+{
+    /* A_ij = sum_k B_ik B_jk (k ascending, fma), i >= j; A_ii += shift; mirror. */
+    double* Bm = (double*)malloc(sizeof(double) * (size_t)n * (size_t)n);
+    for (int k = 0; k < n; ++k)
+        for (int i = 0; i < n; ++i)
+            Bm[i + (long)k * n] = orc_uniform(seed, id, (uint64_t)i + (uint64_t)k * (uint64_t)n);
+    for (int j = 0; j < n; ++j) {
+        for (int i = j; i < n; ++i) {
+            double s = 0.0;
+            for (int k = 0; k < n; ++k) s = fma(Bm[i + (long)k * n], Bm[j + (long)k * n], s);
+            if (i == j) s = s + shift;
+            IDX(A, lda, i, j) = s;
+            IDX(A, lda, j, i) = s;
+        }
+    }
+    free(Bm);
+}
+
+double orc_gen_spd_cond(uint64_t seed, uint64_t id, int n, double lk_lo, double lk_hi,
+                        double* A, int lda)
+{
+    /* kappa and the spectrum */
+    double uk = 0.5 * (orc_uniform(seed ^ 0x5A5A5A5AULL, id, 0) + 1.0);
+    double lk = lk_lo + (lk_hi - lk_lo) * uk;            /* log10(kappa) */
+    double l2k = lk * 3.3219280948873622;                 /* log2(kappa)  */
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) IDX(A, lda, i, j) = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double t;
+        if (i == 0) t = 0.0;
+        else if (i == n - 1) t = 1.0;
+        else t = 0.5 * (orc_uniform(seed ^ 0x5A5A5A5AULL, id, (uint64_t)i + 1) + 1.0);
+        IDX(A, lda, i, i) = orc_exp2(-l2k * t);
+    }
+    /* A <- H_p A H_p for p = n-2 .. 0, H_p acting on indices p..n-1 */
+    double* v = (double*)malloc(sizeof(double) * (size_t)n);
+    double* w = (double*)malloc(sizeof(double) * (size_t)n);
+    for (int p = n - 2; p >= 0; --p) {
+        int m = n - p;
+        for (int i = 0; i < m; ++i)
+            v[i] = orc_normal(seed ^ 0xA5A5A5A5ULL, id, (uint64_t)p * (uint64_t)n + (uint64_t)i);
+        double vv = 0.0;
+        for (int i = 0; i < m; ++i) vv = fma(v[i], v[i], vv);
+        if (vv == 0.0) continue;
+        double tau = 2.0 / vv;
+        for (int i = 0; i < m; ++i) {
+            double s = 0.0;
+            for (int j = 0; j < m; ++j) s = fma(IDX(A, lda, p + i, p + j), v[j], s);
+            w[i] = tau * s;
+        }
+        double vw = 0.0;
+        for (int i = 0; i < m; ++i) vw = fma(v[i], w[i], vw);
+        double alpha = (tau * vw) * 0.5;
+        for (int i = 0; i < m; ++i) w[i] = fma(-alpha, v[i], w[i]);
+        for (int j = 0; j < m; ++j) {
+            for (int i = j; i < m; ++i) {
+                double a = IDX(A, lda, p + i, p + j);
+                a = fma(-v[i], w[j], a);
+                a = fma(-w[i], v[j], a);
+                IDX(A, lda, p + i, p + j) = a;
+                IDX(A, lda, p + j, p + i) = a;
+            }
+        }
+    }
+    free(v);
+    free(w);
+    return orc_exp2(l2k);
+}
+
+void orc_gen_rhs(uint64_t seed, uint64_t id, int rows, int cols, double* B, int ldb)
+{
+    for (int j = 0; j < cols; ++j)
+        for (int i = 0; i < rows; ++i)
+            IDX(B, ldb, i, j) = orc_normal(seed, id, (uint64_t)i + (uint64_t)j * (uint64_t)rows);
+}
+
+/* testing/testing_helper.cu:394-403 (matrix_make_hpd) */
+void orc_make_hpd(int n, double* A, int lda)
+{
+    for (int i = 0; i < n; ++i) {
+        IDX(A, lda, i, i) += (double)n;
+        for (int j = 0; j < i; ++j) IDX(A, lda, j, i) = IDX(A, lda, i, j);
+    }
+}
+
+/* ======================================================================= */
+/* Reference algorithm restatement                                           */
+/* ======================================================================= */
+
+/* src/kblas_common.cu:241-243 */
+int orc_reg_size(int n) { return (n > 0) && !(n & (n - 1)); }
+
+/* src/kblas_common.cu:244-255 */
+int orc_closest_reg_size(int n)
+{
+    if (n <= 0) return 0;
+    int r = 1;
+    while (r < n) r <<= 1;
+    return r >> 1;
+}
+
+static void split(int n, int* n1, int* n2)
+{
+    if (orc_reg_size(n)) { *n1 = *n2 = n / 2; }
+    else { *n1 = orc_closest_reg_size(n); *n2 = n - *n1; }
+}
+
+/* Register POTRF, src/batch_triangular/Xpotrf_batch_kernels.cuh:52-69:
+ * s = sqrt(a_jj); l_ij = a_ij / s (true division, i >= j);
+ * a_ik = fma(l_ij, -l_kj, a_ik) for j < k <= i.
+ * The non-positive-pivot test is the LAPACK rule; the reference comments it
+ * out (Xpotrf_batch_kernels.cuh:123-125). */
+static int potrf_reg(int n, double* A, int lda)
+{
+    for (int j = 0; j < n; ++j) {
+        double d = IDX(A, lda, j, j);
+        if (!(d > 0.0)) return j + 1;
+        double s = sqrt(d);
+        for (int i = j; i < n; ++i) IDX(A, lda, i, j) = IDX(A, lda, i, j) / s;
+        for (int k = j + 1; k < n; ++k) {
+            double lkj = IDX(A, lda, k, j);
+            for (int i = k; i < n; ++i)
+                IDX(A, lda, i, k) = fma(IDX(A, lda, i, j), -lkj, IDX(A, lda, i, k));
+        }
+    }
+    return 0;
+}
+
+/* Recursion, src/batch_triangular/Xpotrf_batch_drivers.cuh:42-133:
+ * n <= 16 -> register kernel; else POTRF(n1) ; TRSM(R,L,T,N, n2 x n1) ;
+ * SYRK(L,N, n2, n1, -1, A21, +1, A22) ; POTRF(n2). */
+int orc_potrf(int n, double* A, int lda)
+{
+    if (n <= 0) return 0;
+    if (n <= 16) return potrf_reg(n, A, lda);
+    int n1, n2;
+    split(n, &n1, &n2);
+    int info = orc_potrf(n1, A, lda);
+    if (info) return info;
+    orc_trsm('R', 'T', n2, n1, 1.0, A, lda, &IDX(A, lda, n1, 0), lda);
+    orc_syrk('N', n2, n1, -1.0, &IDX(A, lda, n1, 0), lda, 1.0, &IDX(A, lda, n1, n1), lda);
+    info = orc_potrf(n2, &IDX(A, lda, n1, n1), lda);
+    return info ? info + n1 : 0;
+}
+
+/* Right-side register TRSM, src/batch_triangular/Xtrsm_batch_kernels.cuh:59-101.
+ * Trans:  rows solved unscaled then multiplied by alpha on store.
+ * NoTrans: rows scaled by alpha on load, solved backwards. */
+static void trsm_reg_right(char trans, int m, int n, double alpha,
+                           const double* A, int lda, double* B, int ldb)
+{
+    double b[64];
+    for (int r = 0; r < m; ++r) {
+        for (int i = 0; i < n; ++i)
+            b[i] = (trans == 'T') ? IDX(B, ldb, r, i) : alpha * IDX(B, ldb, r, i);
+        if (trans == 'T') {
+            for (int j = 0; j < n; ++j) {
+                b[j] = b[j] / IDX(A, lda, j, j);
+                for (int i = j + 1; i < n; ++i) b[i] = fma(b[j], -IDX(A, lda, i, j), b[i]);
+            }
+            for (int i = 0; i < n; ++i) IDX(B, ldb, r, i) = alpha * b[i];
+        } else {
+            for (int j = n - 1; j >= 0; --j) {
+                for (int i = j + 1; i < n; ++i) b[j] = fma(b[i], -IDX(A, lda, i, j), b[j]);
+                b[j] = b[j] / IDX(A, lda, j, j);
+            }
+            for (int i = 0; i < n; ++i) IDX(B, ldb, r, i) = b[i];
+        }
+    }
+}
+
+/* Left-side register TRSM (op(L) X = alpha B), column by column. */
+static void trsm_reg_left(char trans, int m, int n, double alpha,
+                          const double* A, int lda, double* B, int ldb)
+{
+    double b[64];
+    for (int c = 0; c < n; ++c) {
+        for (int i = 0; i < m; ++i) b[i] = alpha * IDX(B, ldb, i, c);
+        if (trans == 'N') {
+            for (int j = 0; j < m; ++j) {
+                b[j] = b[j] / IDX(A, lda, j, j);
+                for (int i = j + 1; i < m; ++i) b[i] = fma(-IDX(A, lda, i, j), b[j], b[i]);
+            }
+        } else {
+            for (int j = m - 1; j >= 0; --j) {
+                for (int i = j + 1; i < m; ++i) b[j] = fma(-IDX(A, lda, i, j), b[i], b[j]);
+                b[j] = b[j] / IDX(A, lda, j, j);
+            }
+        }
+        for (int i = 0; i < m; ++i) IDX(B, ldb, i, c) = b[i];
+    }
+}
+
+/* Recursion, src/batch_triangular/Xtrsm_batch_drivers.cuh:126-266. */
+void orc_trsm(char side, char trans, int m, int n, double alpha,
+              const double* A, int lda, double* B, int ldb)
+{
+    if (m <= 0 || n <= 0) return;
+    if (side == 'R' && n <= 16) { trsm_reg_right(trans, m, n, alpha, A, lda, B, ldb); return; }
+    if (side == 'L' && m <= 16) { trsm_reg_left(trans, m, n, alpha, A, lda, B, ldb); return; }
+    if (side == 'R') {
+        int n1, n2;
+        split(n, &n1, &n2);
+        if (trans == 'T') {
+            orc_trsm(side, trans, m, n1, alpha, A, lda, B, ldb);
+            orc_gemm('N', 'T', m, n2, n1, -1.0 / alpha, B, ldb, &IDX(A, lda, n1, 0), lda,
+                     1.0, &IDX(B, ldb, 0, n1), ldb);
+            orc_trsm(side, trans, m, n2, alpha, &IDX(A, lda, n1, n1), lda, &IDX(B, ldb, 0, n1), ldb);
+        } else {
+            orc_trsm(side, trans, m, n2, alpha, &IDX(A, lda, n1, n1), lda, &IDX(B, ldb, 0, n1), ldb);
+            orc_gemm('N', 'N', m, n1, n2, -1.0, &IDX(B, ldb, 0, n1), ldb, &IDX(A, lda, n1, 0), lda,
+                     alpha, B, ldb);
+            orc_trsm(side, trans, m, n1, 1.0, A, lda, B, ldb);
+        }
+    } else {
+        int m1, m2;
+        split(m, &m1, &m2);
+        if (trans == 'T') {
+            orc_trsm(side, trans, m2, n, alpha, &IDX(A, lda, m1, m1), lda, &IDX(B, ldb, m1, 0), ldb);
+            orc_gemm('T', 'N', m1, n, m2, -1.0, &IDX(A, lda, m1, 0), lda, &IDX(B, ldb, m1, 0), ldb,
+                     alpha, B, ldb);
+            orc_trsm(side, trans, m1, n, 1.0, A, lda, B, ldb);
+        } else {
+            orc_trsm(side, trans, m1, n, alpha, A, lda, B, ldb);
+            orc_gemm('N', 'N', m2, n, m1, -1.0, &IDX(A, lda, m1, 0), lda, B, ldb,
+                     alpha, &IDX(B, ldb, m1, 0), ldb);
+            orc_trsm(side, trans, m2, n, 1.0, &IDX(A, lda, m1, m1), lda, &IDX(B, ldb, m1, 0), ldb);
+        }
+    }
+}
+
+/* C := alpha op(A) op(B) + beta C (BLAS semantics: beta == 0 does not read C).
+ * The reference delegates this to cuBLAS (src/batch_triangular/Xgemm_batch_core.cuh:260-267). */
+void orc_gemm(char ta, char tb, int m, int n, int k, double alpha,
+              const double* A, int lda, const double* B, int ldb,
+              double beta, double* C, int ldc)
+{
+    for (int j = 0; j < n; ++j) {
+        for (int i = 0; i < m; ++i) {
+            double s = 0.0;
+            for (int p = 0; p < k; ++p) {
+                double a = (ta == 'N') ? IDX(A, lda, i, p) : IDX(A, lda, p, i);
+                double b = (tb == 'N') ? IDX(B, ldb, p, j) : IDX(B, ldb, j, p);
+                s = fma(a, b, s);
+            }
+            IDX(C, ldc, i, j) = (beta == 0.0) ? alpha * s : alpha * s + beta * IDX(C, ldc, i, j);
+        }
+    }
+}
+
+/* Lower SYRK, src/batch_triangular/Xsyrk_batch_kernels.cuh:58-75:
+ * b = beta*B; for every chunk of 8 columns of A: b = fma(alpha, s_chunk, b). */
+void orc_syrk(char trans, int m, int k, double alpha, const double* A, int lda,
+              double beta, double* B, int ldb)
+{
+    for (int j = 0; j < m; ++j) {
+        for (int i = j; i < m; ++i) {
+            double b = IDX(B, ldb, i, j) * beta;
+            for (int p0 = 0; p0 < k; p0 += 8) {
+                double s = 0.0;
+                int pe = p0 + 8 < k ? p0 + 8 : k;
+                for (int p = p0; p < pe; ++p) {
+                    double ai = (trans == 'N') ? IDX(A, lda, i, p) : IDX(A, lda, p, i);
+                    double aj = (trans == 'N') ? IDX(A, lda, j, p) : IDX(A, lda, p, j);
+                    s = fma(ai, aj, s);
+                }
+                b = fma(alpha, s, b);
+            }
+            IDX(B, ldb, i, j) = b;
+        }
+    }
+}
+
+/* POTRS, src/batch_triangular/Xpotrs_batch_drivers.cuh:79-171 (Right/Lower):
+ * TRSM(R,L,T,n1) ; GEMM(N,T,-1) ; TRSM(R,L,T,n2) ; TRSM(R,L,N,n2) ; GEMM(N,N,-1) ; TRSM(R,L,N,n1).
+ * n == 1 (where the reference split gives n1 = n2 = 0) is solved directly.
+ * Left side (LAPACK dpotrs layout) is an extension: TRSM(L,L,N) then TRSM(L,L,T). */
+void orc_potrs(char side, int m, int n, const double* A, int lda, double* B, int ldb)
+{
+    if (m <= 0 || n <= 0) return;
+    if (side == 'L') {
+        orc_trsm('L', 'N', m, n, 1.0, A, lda, B, ldb);
+        orc_trsm('L', 'T', m, n, 1.0, A, lda, B, ldb);
+        return;
+    }
+    if (n == 1) {
+        orc_trsm('R', 'T', m, 1, 1.0, A, lda, B, ldb);
+        orc_trsm('R', 'N', m, 1, 1.0, A, lda, B, ldb);
+        return;
+    }
+    int n1, n2;
+    split(n, &n1, &n2);
+    const double* A21 = &IDX(A, lda, n1, 0);
+    const double* A22 = &IDX(A, lda, n1, n1);
+    double* B2 = &IDX(B, ldb, 0, n1);
+    orc_trsm('R', 'T', m, n1, 1.0, A, lda, B, ldb);
+    orc_gemm('N', 'T', m, n2, n1, -1.0, B, ldb, A21, lda, 1.0, B2, ldb);
+    orc_trsm('R', 'T', m, n2, 1.0, A22, lda, B2, ldb);
+    orc_trsm('R', 'N', m, n2, 1.0, A22, lda, B2, ldb);
+    orc_gemm('N', 'N', m, n1, n2, -1.0, B2, ldb, A21, lda, 1.0, B, ldb);
+    orc_trsm('R', 'N', m, n1, 1.0, A, lda, B, ldb);
+}
+
+/* POSV, src/batch_triangular/Xposv_batch_drivers.cuh:83-114: POTRF then POTRS.
+ * (The reference runs POTRS even on a failed factor; here the solve is skipped
+ *  for that matrix, as LAPACK dposv does.)  side 'R': A is n x n; side 'L': m x m. */
+int orc_posv(char side, int m, int n, double* A, int lda, double* B, int ldb)
+{
+    int order = (side == 'R') ? n : m;
+    int info = orc_potrf(order, A, lda);
+    if (info) return info;
+    orc_potrs(side, m, n, A, lda, B, ldb);
+    return 0;
+}
+
+/* ---------------- single precision (S API checks) ---------------- */
+int orc_spotrf(int n, float* A, int lda)
+{
+    /* right-looking column Cholesky with the register-kernel arithmetic */
+    for (int j = 0; j < n; ++j) {
+        float d = IDX(A, lda, j, j);
+        if (!(d > 0.0f)) return j + 1;
+        float s = sqrtf(d);
+        for (int i = j; i < n; ++i) IDX(A, lda, i, j) = IDX(A, lda, i, j) / s;
+        for (int k = j + 1; k < n; ++k) {
+            float lkj = IDX(A, lda, k, j);
+            for (int i = k; i < n; ++i)
+                IDX(A, lda, i, k) = fmaf(IDX(A, lda, i, j), -lkj, IDX(A, lda, i, k));
+        }
+    }
+    return 0;
+}
+
+/* ---------------- mixed precision: LAPACK dsposv restated ---------------- */
+static double* rhs_ptr(char side, double* B, int ldb, int r, long* inc)
+{
+    if (side == 'L') { *inc = 1; return B + (long)r * ldb; }
+    *inc = ldb;
+    return B + r;
+}
+
+static void spotrs_vec(int n, const float* L, float* x)
+{
+    for (int j = 0; j < n; ++j) {
+        x[j] = x[j] / L[j + (long)j * n];
+        for (int i = j + 1; i < n; ++i) x[i] = fmaf(-L[i + (long)j * n], x[j], x[i]);
+    }
+    for (int j = n - 1; j >= 0; --j) {
+        for (int i = j + 1; i < n; ++i) x[j] = fmaf(-L[i + (long)j * n], x[i], x[j]);
+        x[j] = x[j] / L[j + (long)j * n];
+    }
+}
+
+int orc_sposv(char side, int m, int n, const double* A, int lda,
+              const double* B, int ldb, double* X, int ldx, int* iter)
+{
+    int order = (side == 'R') ? n : m;
+    int nrhs = (side == 'R') ? m : n;
+    const int itermax = 30;
+    if (order <= 0 || nrhs <= 0) { *iter = 0; return 0; }
+    /* full symmetric copy of A from its lower triangle */
+    double* Af = (double*)malloc(sizeof(double) * (size_t)order * order);
+    for (int j = 0; j < order; ++j)
+        for (int i = j; i < order; ++i) {
+            Af[i + (long)j * order] = IDX(A, lda, i, j);
+            Af[j + (long)i * order] = IDX(A, lda, i, j);
+        }
+    double anrm = 0.0;  /* infinity norm */
+    for (int i = 0; i < order; ++i) {
+        double s = 0.0;
+        for (int j = 0; j < order; ++j) s += fabs(Af[i + (long)j * order]);
+        if (s > anrm) anrm = s;
+    }
+    double cte = anrm * 1.1102230246251565e-16 * sqrt((double)order);
+    float* L = (float*)malloc(sizeof(float) * (size_t)order * order);
+    for (long t = 0; t < (long)order * order; ++t) L[t] = (float)Af[t];
+    float* xs = (float*)malloc(sizeof(float) * (size_t)order);
+    double* r = (double*)malloc(sizeof(double) * (size_t)order);
+    int sinfo = orc_spotrf(order, L, order);
+    int result_iter = 0, ok = 0;
+    if (sinfo == 0) {
+        /* x0 = solve(b) */
+        for (int c = 0; c < nrhs; ++c) {
+            long ib, ix;
+            const double* b = rhs_ptr(side, (double*)B, ldb, c, &ib);
+            double* x = rhs_ptr(side, X, ldx, c, &ix);
+            for (int i = 0; i < order; ++i) xs[i] = (float)b[i * ib];
+            spotrs_vec(order, L, xs);
+            for (int i = 0; i < order; ++i) x[i * ix] = (double)xs[i];
+        }
+        for (int it = 0; it <= itermax; ++it) {
+            /* residual and convergence test over all right-hand sides */
+            int conv = 1;
+            for (int c = 0; c < nrhs; ++c) {
+                long ib, ix;
+                const double* b = rhs_ptr(side, (double*)B, ldb, c, &ib);
+                double* x = rhs_ptr(side, X, ldx, c, &ix);
+                double xn = 0.0, rn = 0.0;
+                for (int i = 0; i < order; ++i) {
+                    double s = b[i * ib];
+                    for (int k = 0; k < order; ++k) s = fma(-Af[i + (long)k * order], x[k * ix], s);
+                    r[i] = s;
+                    if (fabs(s) > rn) rn = fabs(s);
+                    if (fabs(x[i * ix]) > xn) xn = fabs(x[i * ix]);
+                }
+                if (!(rn <= xn * cte)) conv = 0;
+            }
+            if (conv) { result_iter = it; ok = 1; break; }
+            if (it == itermax) break;
+            for (int c = 0; c < nrhs; ++c) {
+                long ib, ix;
+                const double* b = rhs_ptr(side, (double*)B, ldb, c, &ib);
+                double* x = rhs_ptr(side, X, ldx, c, &ix);
+                for (int i = 0; i < order; ++i) {
+                    double s = b[i * ib];
+                    for (int k = 0; k < order; ++k) s = fma(-Af[i + (long)k * order], x[k * ix], s);
+                    xs[i] = (float)s;
+                }
+                spotrs_vec(order, L, xs);
+                for (int i = 0; i < order; ++i) x[i * ix] = x[i * ix] + (double)xs[i];
+            }
+        }
+        if (!ok) result_iter = -(itermax + 1);
+    } else {
+        result_iter = -3;
+    }
+    int info = 0;
+    if (!ok) {
+        /* FP64 fallback: dpotrf + dpotrs */
+        double* Lf = (double*)malloc(sizeof(double) * (size_t)order * order);
+        memcpy(Lf, Af, sizeof(double) * (size_t)order * order);
+        info = orc_potrf(order, Lf, order);
+        if (info == 0) {
+            for (int c = 0; c < nrhs; ++c) {
+                long ib, ix;
+                const double* b = rhs_ptr(side, (double*)B, ldb, c, &ib);
+                double* x = rhs_ptr(side, X, ldx, c, &ix);
+                for (int i = 0; i < order; ++i) x[i * ix] = b[i * ib];
+            }
+            if (side == 'L') orc_potrs('L', order, nrhs, Lf, order, X, ldx);
+            else orc_potrs('R', nrhs, order, Lf, order, X, ldx);
+        }
+        free(Lf);
+    }
+    *iter = result_iter;
+    free(Af); free(L); free(xs); free(r);
+    return info;
+}
+
+/* ======================================================================= */
+/* Metrics (long double accumulation)                                        */
+/* ======================================================================= */
+double orc_potrf_berr(int n, const double* A, int lda, const double* L, int ldl)
+{
+    long double num = 0.0L, den = 0.0L;
+    for (int j = 0; j < n; ++j) {
+        for (int i = j; i < n; ++i) {
+            long double s = 0.0L;
+            for (int k = 0; k <= j; ++k) s += (long double)IDX(L, ldl, i, k) * IDX(L, ldl, j, k);
+            long double d = (long double)IDX(A, lda, i, j) - s;
+            long double a = IDX(A, lda, i, j);
+            long double w = (i == j) ? 1.0L : 2.0L;
+            num += w * d * d;
+            den += w * a * a;
+        }
+    }
+    if (den == 0.0L) return (double)sqrtl(num);
+    return (double)sqrtl(num / den);
+}
+
+double orc_solve_berr(char side, int m, int n, const double* A, int lda,
+                      const double* X, int ldx, const double* B, int ldb)
+{
+    int order = (side == 'R') ? n : m;
+    int nrhs = (side == 'R') ? m : n;
+    long double anrm = 0.0L;
+    for (int i = 0; i < order; ++i) {
+        long double s = 0.0L;
+        for (int k = 0; k < order; ++k)
+            s += fabsl((long double)(i >= k ? IDX(A, lda, i, k) : IDX(A, lda, k, i)));
+        if (s > anrm) anrm = s;
+    }
+    double worst = 0.0;
+    for (int c = 0; c < nrhs; ++c) {
+        long double rn = 0.0L, xn = 0.0L, bn = 0.0L;
+        for (int i = 0; i < order; ++i) {
+            long double b = (side == 'L') ? IDX(B, ldb, i, c) : IDX(B, ldb, c, i);
+            long double s = b;
+            for (int k = 0; k < order; ++k) {
+                long double a = (i >= k) ? IDX(A, lda, i, k) : IDX(A, lda, k, i);
+                long double x = (side == 'L') ? IDX(X, ldx, k, c) : IDX(X, ldx, c, k);
+                s -= a * x;
+            }
+            long double x = (side == 'L') ? IDX(X, ldx, i, c) : IDX(X, ldx, c, i);
+            if (fabsl(s) > rn) rn = fabsl(s);
+            if (fabsl(x) > xn) xn = fabsl(x);
+            if (fabsl(b) > bn) bn = fabsl(b);
+        }
+        long double den = anrm * xn + bn;
+        double e = (den > 0.0L) ? (double)(rn / den) : (double)rn;
+        if (!(e <= worst)) worst = e;   /* NaN propagates */
+    }
+    return worst;
+}
+
+/* ======================================================================= */
+/* FLOP model, testing/flops.h:74-150 (real arithmetic)                      */
+/* ======================================================================= */
+double orc_flops_potrf(int n)
+{
+    double x = n;
+    double fmuls = x * (((1.0 / 6.0) * x + 0.5) * x + (1.0 / 3.0));
+    double fadds = x * (((1.0 / 6.0) * x) * x - (1.0 / 6.0));
+    return fmuls + fadds;
+}
+double orc_flops_trsm(char side, int m, int n)
+{
+    double a = (side == 'L') ? m : n, b = (side == 'L') ? n : m;
+    return 0.5 * b * a * (a + 1.0) + 0.5 * b * a * (a - 1.0);
+}
+double orc_flops_potrs(char side, int m, int n) { return 2.0 * orc_flops_trsm(side, m, n); }
+double orc_flops_posv(char side, int m, int n)
+{
+    int order = (side == 'R') ? n : m;
+    return orc_flops_potrf(order) + orc_flops_potrs(side, m, n);
+}
+double orc_flops_syrk(int k, int n) { return (double)k * n * (n + 1.0); }
+double orc_flops_gemm(int m, int n, int k) { return 2.0 * m * (double)n * k; }
+
+/* ======================================================================= */
+/* Batch loops                                                                */
+/* ======================================================================= */
+static int g_threads = 0;
+int orc_threads(void)
+{
+#ifdef _OPENMP
+    return g_threads > 0 ? g_threads : omp_get_num_procs();
+#else
+    return 1;
+#endif
+}
+void orc_set_threads(int t) { g_threads = t; }
+
+void orc_potrf_batch(int n, double* A, int lda, long strideA, int batch, int* info)
+{
+#pragma omp parallel for schedule(static) num_threads(orc_threads())
+    for (int b = 0; b < batch; ++b) {
+        int r = orc_potrf(n, A + (long)b * strideA, lda);
+        if (info) info[b] = r;
+    }
+}
+
+void orc_posv_batch(char side, int m, int n, double* A, int lda, long strideA,
+                    double* B, int ldb, long strideB, int batch, int* info)
+{
+#pragma omp parallel for schedule(static) num_threads(orc_threads())
+    for (int b = 0; b < batch; ++b) {
+        int r = orc_posv(side, m, n, A + (long)b * strideA, lda, B + (long)b * strideB, ldb);
+        if (info) info[b] = r;
+    }
+}
+
+void orc_sposv_batch(char side, int m, int n, const double* A, int lda, long strideA,
+                     const double* B, int ldb, long strideB, double* X, int ldx, long strideX,
+                     int batch, int* info, int* iter)
+{
+#pragma omp parallel for schedule(static) num_threads(orc_threads())
+    for (int b = 0; b < batch; ++b) {
+        int it = 0;
+        int r = orc_sposv(side, m, n, A + (long)b * strideA, lda, B + (long)b * strideB, ldb,
+                          X + (long)b * strideX, ldx, &it);
+        if (info) info[b] = r;
+        if (iter) iter[b] = it;
+    }
+}
+
+void orc_gen_spd_shift_batch(uint64_t seed, int n, double shift, double* A, int lda,
+                             long strideA, long first, int batch)
+{
+#pragma omp parallel for schedule(static) num_threads(orc_threads())
+    for (int b = 0; b < batch; ++b)
+        orc_gen_spd_shift(seed, (uint64_t)(first + b), n, shift, A + (long)b * strideA, lda);
+}
+
+void orc_gen_spd_cond_batch(uint64_t seed, int n, double lk_lo, double lk_hi, double* A,
+                            int lda, long strideA, long first, int batch, double* kappa)
+{
+#pragma omp parallel for schedule(static) num_threads(orc_threads())
+    for (int b = 0; b < batch; ++b) {
+        double k = orc_gen_spd_cond(seed, (uint64_t)(first + b), n, lk_lo, lk_hi,
+                                    A + (long)b * strideA, lda);
+        if (kappa) kappa[b] = k;
+    }
+}
+
+void orc_gen_rhs_batch(uint64_t seed, int rows, int cols, double* B, int ldb, long strideB,
+                       long first, int batch)
+{
+#pragma omp parallel for schedule(static) num_threads(orc_threads())
+    for (int b = 0; b < batch; ++b)
+        orc_gen_rhs(seed, (uint64_t)(first + b), rows, cols, B + (long)b * strideB, ldb);
+}
+
+/* ======================================================================= */
+/* LAPACK loaded at run time (the reference tests' CPU comparator)          */
+/* ======================================================================= */
+typedef void (*potrf_t)(const char*, const int*, double*, const int*, int*);
+typedef void (*potrs_t)(const char*, const int*, const int*, const double*, const int*,
+                        double*, const int*, int*);
+typedef void (*trsm_t)(const char*, const char*, const char*, const char*, const int*,
+                       const int*, const double*, const double*, const int*, double*, const int*);
+
+static void* g_lapack = NULL;
+static const char* g_lapack_name = "none";
+static potrf_t p_potrf = NULL;
+static potrs_t p_potrs = NULL;
+static trsm_t p_trsm = NULL;
+static int g_lapack_tried = 0;
+
+static void lapack_load(void)
+{
+    if (g_lapack_tried) return;
+    g_lapack_tried = 1;
+    const char* names[] = {"libmkl_rt.so", "libmkl_rt.so.1", "/opt/conda/lib/libmkl_rt.so",
+                           "libopenblas.so.0", "libopenblas.so", "liblapack.so.3", NULL};
+    for (int i = 0; names[i]; ++i) {
+        void* h = dlopen(names[i], RTLD_NOW | RTLD_LOCAL);
+        if (!h) continue;
+        potrf_t f = (potrf_t)dlsym(h, "dpotrf_");
+        potrs_t s = (potrs_t)dlsym(h, "dpotrs_");
+        trsm_t t = (trsm_t)dlsym(h, "dtrsm_");
+        if (f && s && t) {
+            /* one LAPACK thread per OpenMP thread, as the reference's -lmkl_sequential */
+            int (*mkl_layer)(int) = (int (*)(int))dlsym(h, "MKL_Set_Threading_Layer");
+            if (mkl_layer) mkl_layer(1 /* MKL_THREADING_SEQUENTIAL */);
+            void (*ob_threads)(int) = (void (*)(int))dlsym(h, "openblas_set_num_threads");
+            if (ob_threads) ob_threads(1);
+            g_lapack = h; p_potrf = f; p_potrs = s; p_trsm = t; g_lapack_name = names[i];
+            return;
+        }
+        dlclose(h);
+    }
+}
+
+int orc_lapack_available(void) { lapack_load(); return g_lapack != NULL; }
+const char* orc_lapack_name(void) { lapack_load(); return g_lapack_name; }
+
+int orc_lapack_potrf_batch(int n, double* A, int lda, long strideA, int batch, int* info)
+{
+    lapack_load();
+    if (!g_lapack) return 0;
+#pragma omp parallel for schedule(static) num_threads(orc_threads())
+    for (int b = 0; b < batch; ++b) {
+        int r = 0;
+        p_potrf("L", &n, A + (long)b * strideA, &lda, &r);
+        if (info) info[b] = r;
+    }
+    return 1;
+}
+
+/* testing/batch_triangular/test_Xposv_batch.cpp:313-326: dpotrf then
+ * dtrsm(R,L,T,N) and dtrsm(R,L,N,N) for side 'R'; dpotrs for side 'L'. */
+int orc_lapack_posv_batch(char side, int m, int n, double* A, int lda, long strideA,
+                          double* B, int ldb, long strideB, int batch, int* info)
+{
+    lapack_load();
+    if (!g_lapack) return 0;
+    const double one = 1.0;
+#pragma omp parallel for schedule(static) num_threads(orc_threads())
+    for (int b = 0; b < batch; ++b) {
+        int r = 0;
+        int order = (side == 'R') ? n : m;
+        double* Ab = A + (long)b * strideA;
+        double* Bb = B + (long)b * strideB;
+        p_potrf("L", &order, Ab, &lda, &r);
+        if (r == 0) {
+            if (side == 'R') {
+                p_trsm("R", "L", "T", "N", &m, &n, &one, Ab, &lda, Bb, &ldb);
+                p_trsm("R", "L", "N", "N", &m, &n, &one, Ab, &lda, Bb, &ldb);
+            } else {
+                int rr = 0;
+                p_potrs("L", &m, &n, Ab, &lda, Bb, &ldb, &rr);
+            }
+        }
+        if (info) info[b] = r;
+    }
+    return 1;
+}

@@ -1,0 +1,217 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE: used only by tests/, the smoke entry and the benchmark's
+CPU-baseline leg, always as the checker.  The product library never loads it.
+All arrays are numpy float64/int32, column-major (Fortran order).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+_d = ctypes.c_double
+_i = ctypes.c_int
+_l = ctypes.c_long
+_u64 = ctypes.c_uint64
+_c = ctypes.c_char
+_p = ctypes.c_void_p
+
+_SIG = {
+    "orc_hash": (_u64, [_u64, _u64, _u64]),
+    "orc_uniform": (_d, [_u64, _u64, _u64]),
+    "orc_normal": (_d, [_u64, _u64, _u64]),
+    "orc_exp2": (_d, [_d]),
+    "orc_gen_spd_shift": (None, [_u64, _u64, _i, _d, _p, _i]),
+    "orc_gen_spd_cond": (_d, [_u64, _u64, _i, _d, _d, _p, _i]),
+    "orc_gen_rhs": (None, [_u64, _u64, _i, _i, _p, _i]),
+    "orc_make_hpd": (None, [_i, _p, _i]),
+    "orc_reg_size": (_i, [_i]),
+    "orc_closest_reg_size": (_i, [_i]),
+    "orc_potrf": (_i, [_i, _p, _i]),
+    "orc_spotrf": (_i, [_i, _p, _i]),
+    "orc_trsm": (None, [_c, _c, _i, _i, _d, _p, _i, _p, _i]),
+    "orc_syrk": (None, [_c, _i, _i, _d, _p, _i, _d, _p, _i]),
+    "orc_gemm": (None, [_c, _c, _i, _i, _i, _d, _p, _i, _p, _i, _d, _p, _i]),
+    "orc_potrs": (None, [_c, _i, _i, _p, _i, _p, _i]),
+    "orc_posv": (_i, [_c, _i, _i, _p, _i, _p, _i]),
+    "orc_sposv": (_i, [_c, _i, _i, _p, _i, _p, _i, _p, _i, _p]),
+    "orc_potrf_berr": (_d, [_i, _p, _i, _p, _i]),
+    "orc_solve_berr": (_d, [_c, _i, _i, _p, _i, _p, _i, _p, _i]),
+    "orc_flops_potrf": (_d, [_i]),
+    "orc_flops_trsm": (_d, [_c, _i, _i]),
+    "orc_flops_potrs": (_d, [_c, _i, _i]),
+    "orc_flops_posv": (_d, [_c, _i, _i]),
+    "orc_flops_syrk": (_d, [_i, _i]),
+    "orc_flops_gemm": (_d, [_i, _i, _i]),
+    "orc_threads": (_i, []),
+    "orc_set_threads": (None, [_i]),
+    "orc_potrf_batch": (None, [_i, _p, _i, _l, _i, _p]),
+    "orc_posv_batch": (None, [_c, _i, _i, _p, _i, _l, _p, _i, _l, _i, _p]),
+    "orc_sposv_batch": (None, [_c, _i, _i, _p, _i, _l, _p, _i, _l, _p, _i, _l, _i, _p, _p]),
+    "orc_lapack_available": (_i, []),
+    "orc_lapack_name": (ctypes.c_char_p, []),
+    "orc_lapack_potrf_batch": (_i, [_i, _p, _i, _l, _i, _p]),
+    "orc_lapack_posv_batch": (_i, [_c, _i, _i, _p, _i, _l, _p, _i, _l, _i, _p]),
+    "orc_gen_spd_shift_batch": (None, [_u64, _i, _d, _p, _i, _l, _l, _i]),
+    "orc_gen_spd_cond_batch": (None, [_u64, _i, _d, _d, _p, _i, _l, _l, _i, _p]),
+    "orc_gen_rhs_batch": (None, [_u64, _i, _i, _p, _i, _l, _l, _i]),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError("oracle not built: %s" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for k, (r, a) in _SIG.items():
+            f = getattr(L, k)
+            f.restype = r
+            f.argtypes = a
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def ch(c):
+    return c.encode() if isinstance(c, str) else c
+
+
+def fmat(rows, cols, dtype=np.float64):
+    return np.zeros((rows, cols), dtype=dtype, order="F")
+
+
+# ---------------- convenience wrappers (single matrix) ----------------
+def potrf(A):
+    L = np.array(A, dtype=np.float64, order="F", copy=True)
+    info = lib().orc_potrf(L.shape[0], ptr(L), L.shape[0])
+    return L, info
+
+
+def spotrf(A):
+    L = np.array(A, dtype=np.float32, order="F", copy=True)
+    info = lib().orc_spotrf(L.shape[0], ptr(L), L.shape[0])
+    return L, info
+
+
+def trsm(side, trans, alpha, A, B):
+    X = np.array(B, dtype=np.float64, order="F", copy=True)
+    A = np.asfortranarray(A, dtype=np.float64)
+    m, n = X.shape
+    lib().orc_trsm(ch(side), ch(trans), m, n, alpha, ptr(A), A.shape[0], ptr(X), m)
+    return X
+
+
+def syrk(trans, alpha, A, beta, B):
+    C = np.array(B, dtype=np.float64, order="F", copy=True)
+    A = np.asfortranarray(A, dtype=np.float64)
+    m = C.shape[0]
+    k = A.shape[1] if trans == "N" else A.shape[0]
+    lib().orc_syrk(ch(trans), m, k, alpha, ptr(A), A.shape[0], beta, ptr(C), m)
+    return C
+
+
+def gemm(ta, tb, alpha, A, B, beta, C):
+    R = np.array(C, dtype=np.float64, order="F", copy=True)
+    A = np.asfortranarray(A, dtype=np.float64)
+    B = np.asfortranarray(B, dtype=np.float64)
+    m, n = R.shape
+    k = A.shape[1] if ta == "N" else A.shape[0]
+    lib().orc_gemm(ch(ta), ch(tb), m, n, k, alpha, ptr(A), A.shape[0], ptr(B), B.shape[0], beta, ptr(R), m)
+    return R
+
+
+def potrs(side, L, B):
+    X = np.array(B, dtype=np.float64, order="F", copy=True)
+    L = np.asfortranarray(L, dtype=np.float64)
+    m, n = X.shape
+    lib().orc_potrs(ch(side), m, n, ptr(L), L.shape[0], ptr(X), m)
+    return X
+
+
+def posv(side, A, B):
+    L = np.array(A, dtype=np.float64, order="F", copy=True)
+    X = np.array(B, dtype=np.float64, order="F", copy=True)
+    m, n = X.shape
+    info = lib().orc_posv(ch(side), m, n, ptr(L), L.shape[0], ptr(X), m)
+    return L, X, info
+
+
+def sposv(side, A, B):
+    A = np.array(A, dtype=np.float64, order="F", copy=True)
+    B = np.asfortranarray(B, dtype=np.float64)
+    X = np.zeros_like(B, order="F")
+    m, n = B.shape
+    it = ctypes.c_int(0)
+    info = lib().orc_sposv(ch(side), m, n, ptr(A), A.shape[0], ptr(B), m, ptr(X), m, ctypes.byref(it))
+    return X, info, it.value
+
+
+def potrf_berr(A, L):
+    A = np.asfortranarray(A, dtype=np.float64)
+    L = np.asfortranarray(L, dtype=np.float64)
+    return lib().orc_potrf_berr(A.shape[0], ptr(A), A.shape[0], ptr(L), L.shape[0])
+
+
+def solve_berr(side, A, X, B):
+    A = np.asfortranarray(A, dtype=np.float64)
+    X = np.asfortranarray(X, dtype=np.float64)
+    B = np.asfortranarray(B, dtype=np.float64)
+    m, n = B.shape
+    return lib().orc_solve_berr(ch(side), m, n, ptr(A), A.shape[0], ptr(X), m, ptr(B), m)
+
+
+def gen_spd_shift(seed, mid, n, shift=None):
+    A = fmat(n, n)
+    lib().orc_gen_spd_shift(seed, mid, n, float(n if shift is None else shift), ptr(A), n)
+    return A
+
+
+def gen_spd_cond(seed, mid, n, lk_lo, lk_hi):
+    A = fmat(n, n)
+    k = lib().orc_gen_spd_cond(seed, mid, n, lk_lo, lk_hi, ptr(A), n)
+    return A, k
+
+
+def gen_rhs(seed, mid, rows, cols):
+    B = fmat(rows, cols)
+    lib().orc_gen_rhs(seed, mid, rows, cols, ptr(B), rows)
+    return B
+
+
+# ---------------- batches: arrays of shape (batch, rows, cols), each matrix column-major ----
+def batch_array(batch, rows, cols, dtype=np.float64):
+    """(batch, cols, rows) C-order storage == batch of column-major rows x cols matrices."""
+    return np.zeros((batch, cols, rows), dtype=dtype)
+
+
+def gen_spd_shift_batch(seed, n, batch, first=0, shift=None):
+    A = batch_array(batch, n, n)
+    lib().orc_gen_spd_shift_batch(seed, n, float(n if shift is None else shift), ptr(A), n, n * n, first, batch)
+    return A
+
+
+def gen_spd_cond_batch(seed, n, batch, lk_lo, lk_hi, first=0):
+    A = batch_array(batch, n, n)
+    kap = np.zeros(batch)
+    lib().orc_gen_spd_cond_batch(seed, n, lk_lo, lk_hi, ptr(A), n, n * n, first, batch, ptr(kap))
+    return A, kap
+
+
+def gen_rhs_batch(seed, rows, cols, batch, first=0):
+    B = batch_array(batch, rows, cols)
+    lib().orc_gen_rhs_batch(seed, rows, cols, ptr(B), rows, rows * cols, first, batch)
+    return B
+
+
+def col(Ab, i):
+    """matrix i of a batch array as a (rows, cols) Fortran-ordered view"""
+    return Ab[i].T

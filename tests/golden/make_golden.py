@@ -1,0 +1,173 @@
+"""Writes the golden fixtures in this folder from LAPACK/BLAS (Intel MKL, the
+vendor the reference's tests link: testing/Makefile:44, make.inc:30-32).
+
+The reference's own tests hold no golden vectors (they compare against LAPACK
+at run time with unseeded rand(), testing/testing_helper.cu:361-367), so the
+fixtures are produced the same way the reference's tests produce their
+expected values, but seeded and committed:
+  - SPD inputs built like testing/testing_helper.cu:361-403 (U(0,1) entries,
+    A_ii += n, lower mirrored to upper), from numpy's seeded generator;
+  - expected outputs from MKL dpotrf / dtrsm / dsyrk / dgemm / dpotrs, with
+    POSV side 'R' computed exactly as testing/batch_triangular/test_Xposv_batch.cpp:313-326
+    (dpotrf, then dtrsm(R,L,T,N) and dtrsm(R,L,N,N)).
+Run in the build container only (needs MKL): python tests/golden/make_golden.py
+"""
+import ctypes
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+MKL_PATHS = ["/opt/conda/lib/libmkl_rt.so", "libmkl_rt.so", "libmkl_rt.so.1"]
+
+
+def load_mkl():
+    for p in MKL_PATHS:
+        try:
+            lib = ctypes.CDLL(p)
+            lib.MKL_Get_Version_String.argtypes = [ctypes.c_char_p, ctypes.c_int]
+            buf = ctypes.create_string_buffer(256)
+            lib.MKL_Get_Version_String(buf, 256)
+            return lib, buf.value.decode()
+        except OSError:
+            continue
+    sys.exit("MKL not found")
+
+
+mkl, MKL_VERSION = load_mkl()
+ci = ctypes.c_int
+cd = ctypes.c_double
+cc = ctypes.c_char
+
+
+def P(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def byr(x):
+    return ctypes.byref(x)
+
+
+def dpotrf(A):
+    L = np.array(A, order="F", copy=True)
+    n = ci(L.shape[0]); info = ci(0)
+    mkl.dpotrf_(byr(cc(b"L")), byr(n), P(L), byr(n), byr(info))
+    return L, info.value
+
+
+def dtrsm(side, trans, alpha, A, B):
+    X = np.array(B, order="F", copy=True)
+    m, n = X.shape
+    lda = A.shape[0]
+    mkl.dtrsm_(byr(cc(side)), byr(cc(b"L")), byr(cc(trans)), byr(cc(b"N")), byr(ci(m)), byr(ci(n)),
+               byr(cd(alpha)), P(np.asfortranarray(A)), byr(ci(lda)), P(X), byr(ci(m)))
+    return X
+
+
+def dsyrk(trans, alpha, A, beta, C):
+    R = np.array(C, order="F", copy=True)
+    n = R.shape[0]
+    k = A.shape[1] if trans == b"N" else A.shape[0]
+    mkl.dsyrk_(byr(cc(b"L")), byr(cc(trans)), byr(ci(n)), byr(ci(k)), byr(cd(alpha)),
+               P(np.asfortranarray(A)), byr(ci(A.shape[0])), byr(cd(beta)), P(R), byr(ci(n)))
+    return R
+
+
+def dgemm(ta, tb, alpha, A, B, beta, C):
+    R = np.array(C, order="F", copy=True)
+    m, n = R.shape
+    k = A.shape[1] if ta == b"N" else A.shape[0]
+    mkl.dgemm_(byr(cc(ta)), byr(cc(tb)), byr(ci(m)), byr(ci(n)), byr(ci(k)), byr(cd(alpha)),
+               P(np.asfortranarray(A)), byr(ci(A.shape[0])), P(np.asfortranarray(B)), byr(ci(B.shape[0])),
+               byr(cd(beta)), P(R), byr(ci(m)))
+    return R
+
+
+def dpotrs(L, B):
+    X = np.array(B, order="F", copy=True)
+    n, nrhs = X.shape
+    info = ci(0)
+    mkl.dpotrs_(byr(cc(b"L")), byr(ci(n)), byr(ci(nrhs)), P(np.asfortranarray(L)), byr(ci(n)), P(X),
+                byr(ci(n)), byr(info))
+    return X
+
+
+def make_hpd(rng, n):
+    """testing/testing_helper.cu:361-367 + 394-403"""
+    A = np.asfortranarray(rng.random((n, n)))
+    for i in range(n):
+        A[i, i] += n
+        for j in range(i):
+            A[j, i] = A[i, j]
+    return A
+
+
+def main():
+    rng = np.random.default_rng(20181114)
+    out = {"mkl_version": np.array(MKL_VERSION)}
+    sizes = [1, 2, 3, 4, 7, 8, 9, 15, 16, 17, 31, 32, 33, 48, 64, 65, 100]
+    for n in sizes:
+        A = make_hpd(rng, n)
+        L, info = dpotrf(A)
+        assert info == 0
+        out["potrf_A_%d" % n] = A
+        out["potrf_L_%d" % n] = np.tril(L)
+        # POSV side R (reference layout): B is m x n, m right-hand sides as rows
+        for m in (1, 4):
+            B = np.asfortranarray(rng.standard_normal((m, n)))
+            X = dtrsm(b"R", b"T", 1.0, L, B)
+            X = dtrsm(b"R", b"N", 1.0, L, X)
+            out["posvR_B_%d_%d" % (n, m)] = B
+            out["posvR_X_%d_%d" % (n, m)] = X
+        # POSV side L (LAPACK layout): B is n x nrhs
+        B = np.asfortranarray(rng.standard_normal((n, 3)))
+        out["posvL_B_%d" % n] = B
+        out["posvL_X_%d" % n] = dpotrs(L, B)
+    # TRSM: all four lower/non-unit variants with alpha != 1
+    for n in (5, 16, 24, 40):
+        A = make_hpd(rng, n)
+        L, _ = dpotrf(A)
+        L = np.tril(L)
+        for side in (b"L", b"R"):
+            for trans in (b"N", b"T"):
+                shape = (n, 6) if side == b"L" else (6, n)
+                B = np.asfortranarray(rng.standard_normal(shape))
+                key = "trsm_%s%s_%d" % (side.decode(), trans.decode(), n)
+                out[key + "_A"] = L
+                out[key + "_B"] = B
+                out[key + "_X"] = dtrsm(side, trans, 0.75, L, B)
+    # SYRK lower, both transposes
+    for (m, k) in ((8, 8), (16, 32), (33, 17)):
+        for trans in (b"N", b"T"):
+            A = np.asfortranarray(rng.standard_normal((m, k) if trans == b"N" else (k, m)))
+            C = np.asfortranarray(rng.standard_normal((m, m)))
+            key = "syrk_%s_%d_%d" % (trans.decode(), m, k)
+            out[key + "_A"] = A
+            out[key + "_C"] = C
+            out[key + "_R"] = dsyrk(trans, -1.0, A, 0.5, C)
+    # GEMM, all transposes
+    for (m, n, k) in ((4, 32, 32), (32, 16, 16), (7, 9, 11)):
+        for ta in (b"N", b"T"):
+            for tb in (b"N", b"T"):
+                A = np.asfortranarray(rng.standard_normal((m, k) if ta == b"N" else (k, m)))
+                B = np.asfortranarray(rng.standard_normal((k, n) if tb == b"N" else (n, k)))
+                C = np.asfortranarray(rng.standard_normal((m, n)))
+                key = "gemm_%s%s_%d_%d_%d" % (ta.decode(), tb.decode(), m, n, k)
+                out[key + "_A"] = A
+                out[key + "_B"] = B
+                out[key + "_C"] = C
+                out[key + "_R"] = dgemm(ta, tb, -1.0, A, B, 1.0, C)
+    # planted non-SPD: first non-positive leading minor at position p (1-based)
+    for n, p in ((8, 5), (20, 1), (40, 33)):
+        A = make_hpd(rng, n)
+        A[p - 1, p - 1] = -float(n)
+        L, info = dpotrf(A)
+        out["nonspd_A_%d" % n] = A
+        out["nonspd_info_%d" % n] = np.array(info)
+    np.savez_compressed(os.path.join(HERE, "lapack_golden.npz"), **out)
+    print("wrote", len(out), "arrays with", MKL_VERSION)
+
+
+if __name__ == "__main__":
+    main()
