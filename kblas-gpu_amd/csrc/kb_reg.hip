@@ -31,3 +31,36 @@ extern "C" int kb_launch_reg_sposv(int n, int nrhs, int side_left, kb_mat_t A, k
     if (n <= 32) return kb_reg_sposv_32(n, nrhs, side_left, A, B, X, info, iter, batch, s);
     return kb_reg_sposv_64(n, nrhs, side_left, A, B, X, info, iter, batch, s);
 }
+
+extern "C" int kb_blk_chol_d(int, int, int, int, kb_mat_t, kb_mat_t, int*, int, hipStream_t);
+extern "C" int kb_blk_chol_s(int, int, int, int, kb_mat_t, kb_mat_t, int*, int, hipStream_t);
+
+extern "C" int kb_launch_blk_chol(char prec, int mode, int n, int nrhs, int side_left,
+                                  kb_mat_t A, kb_mat_t B, int* info, int batch, hipStream_t s)
+{
+    if (n < 1 || batch < 1) return (int)hipErrorInvalidValue;
+    if (prec == 'd') return kb_blk_chol_d(mode, n, nrhs, side_left, A, B, info, batch, s);
+    return kb_blk_chol_s(mode, n, nrhs, side_left, A, B, info, batch, s);
+}
+
+extern "C" int kb_vchol_d(int, const int*, const int*, int, kb_mat_t, kb_mat_t, kb_mat_t, int*, int*, int, void*, hipStream_t);
+extern "C" int kb_vchol_s(int, const int*, const int*, int, kb_mat_t, kb_mat_t, kb_mat_t, int*, int*, int, void*, hipStream_t);
+extern "C" int kb_vchol_m(int, const int*, const int*, int, kb_mat_t, kb_mat_t, kb_mat_t, int*, int*, int, void*, hipStream_t);
+
+// hist + offsets + counters + the order array (see kb_vchol_impl.h)
+extern "C" size_t kb_vchol_ws_bytes(int batch)
+{
+    return sizeof(int) * ((size_t)(2 * 4096 + 64) + (size_t)(batch > 0 ? batch : 0));
+}
+
+extern "C" int kb_launch_vchol(char prec, int mode, const int* vm, const int* vn, int side_left,
+                               kb_mat_t A, kb_mat_t B, kb_mat_t X, int* info, int* iter, int batch,
+                               int max_n, void* ws, hipStream_t s)
+{
+    (void)max_n;
+    if (batch < 1) return 0;
+    if (!ws) return (int)hipErrorInvalidValue;
+    if (prec == 'm') return kb_vchol_m(mode, vm, vn, side_left, A, B, X, info, iter, batch, ws, s);
+    if (prec == 'd') return kb_vchol_d(mode, vm, vn, side_left, A, B, X, info, iter, batch, ws, s);
+    return kb_vchol_s(mode, vm, vn, side_left, A, B, X, info, iter, batch, ws, s);
+}

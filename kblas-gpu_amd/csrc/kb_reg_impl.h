@@ -129,6 +129,83 @@ KB_DEV T* rhs_elem(T* Bp, int ldb, int side_left, int row, int col)
     return side_left ? Bp + row + (long)col * ldb : Bp + col + (long)row * ldb;
 }
 
+// Backward sweep reading L(k,r) straight from the factor in global memory
+// (used where no LDS copy is kept: lanes read a row of L, which the same
+// wavefront has just written and fenced).
+template <typename T, int NB, int G, int RC>
+KB_DEV void reg_backward_global(const T* Ap, int lda, T (&x)[RC], int n, int r, T my_inv)
+{
+#pragma unroll
+    for (int k = NB - 1; k >= 0; --k) {
+        if (k < n) {
+            T l = (r < k) ? Ap[k + (long)r * lda] : T(0);
+            T ik = bcastk<G>(my_inv, k);
+#pragma unroll
+            for (int c = 0; c < RC; ++c) {
+                T xk = bcastk<G>(x[c], k) * ik;
+                x[c] = kb_fma<T>(-l, xk, x[c]);
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < RC; ++c) x[c] *= my_inv;
+}
+
+// One matrix on one G-lane group (lane r = row r).  Lt: the group's LDS area
+// of NB*(NB+1) elements, or nullptr to read the transposed factor from A.
+template <typename T, int NB, int G, int MODE, int RC>
+KB_DEV void reg_chol_one(int n, int nrhs, int side_left, T* Ap, int lda, T* Bp, int ldb,
+                         int* info_out, T* Lt, int r)
+{
+    T a[NB];
+    T my_diag = T(1), my_inv = T(1);
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        T v = T(0);
+        if (k <= r && r < n) v = Ap[r + (long)k * lda];
+        a[k] = v;
+        if constexpr (MODE == KB_MODE_POTRS) {
+            if (k == r) my_diag = v;
+        }
+    }
+    int inf = 0;
+    if constexpr (MODE != KB_MODE_POTRS) {
+        inf = reg_potrf<T, NB, G>(a, n, r, my_diag, my_inv);
+        // write L (lower triangle only; the strict upper part is never touched).
+        // After a failed pivot the unfinished columns keep their partially
+        // updated values, as dpotrf leaves them.
+#pragma unroll
+        for (int k = 0; k < NB; ++k)
+            if (k <= r && r < n) Ap[r + (long)k * lda] = a[k];
+        if (r == 0 && info_out) *info_out = inf;
+    } else {
+        my_inv = T(1) / my_diag;
+    }
+    if constexpr (MODE != KB_MODE_POTRF) {
+        if (inf != 0) return;
+        // strictly lower row for the sweeps
+#pragma unroll
+        for (int k = 0; k < NB; ++k) a[k] = (k < r) ? a[k] : T(0);
+        if (Lt) lds_store_rows<T, NB, G>(Lt, a, r);
+        else __threadfence();
+        for (int c0 = 0; c0 < nrhs; c0 += RC) {
+            T x[RC];
+#pragma unroll
+            for (int c = 0; c < RC; ++c) {
+                T v = T(0);
+                if (r < n && c0 + c < nrhs) v = *rhs_elem(Bp, ldb, side_left, r, c0 + c);
+                x[c] = v;
+            }
+            reg_forward<T, NB, G, RC>(a, x, n, my_inv);
+            if (Lt) reg_backward<T, NB, G, RC>(Lt, x, n, r, my_inv);
+            else reg_backward_global<T, NB, G, RC>(Ap, lda, x, n, r, my_inv);
+#pragma unroll
+            for (int c = 0; c < RC; ++c)
+                if (r < n && c0 + c < nrhs) *rhs_elem(Bp, ldb, side_left, r, c0 + c) = x[c];
+        }
+    }
+}
+
 template <typename T, int NB, int MODE, int RC>
 __global__ void __launch_bounds__((reg_mpb<T, NB, MODE>() * NB))
 k_reg_chol(int n, int nrhs, int side_left, BatchArg<T> A, BatchArg<T> B, int* info, int batch)
@@ -142,61 +219,11 @@ k_reg_chol(int n, int nrhs, int side_left, BatchArg<T> A, BatchArg<T> B, int* in
     const int g = threadIdx.x / G;
     const long mat = (long)blockIdx.x * MPB + g;
     if (mat >= batch) return;   // the whole group leaves together
-
-    T* Ap = A.get(mat);
-    const int lda = A.ldim(mat);
-    T a[NB];
-    T my_diag = T(1), my_inv = T(1);
-#pragma unroll
-    for (int k = 0; k < NB; ++k) {
-        T v = T(0);
-        if (k <= r && r < n) v = Ap[r + (long)k * lda];
-        a[k] = v;
-        if constexpr (MODE == KB_MODE_POTRS) {
-            if (k == r) my_diag = v;
-        }
-    }
-
-    int inf = 0;
-    if constexpr (MODE != KB_MODE_POTRS) {
-        inf = reg_potrf<T, NB, G>(a, n, r, my_diag, my_inv);
-        // write L (lower triangle only; the strict upper part is never touched).
-        // After a failed pivot the unfinished columns keep their partially
-        // updated values, as dpotrf leaves them.
-#pragma unroll
-        for (int k = 0; k < NB; ++k)
-            if (k <= r && r < n) Ap[r + (long)k * lda] = a[k];
-        if (r == 0 && info) info[mat] = inf;
-    } else {
-        my_inv = T(1) / my_diag;
-    }
-    if constexpr (MODE == KB_MODE_POTRF) {
-        return;
-    } else {
-        if (inf != 0) return;
-        // strictly lower row for the sweeps
-#pragma unroll
-        for (int k = 0; k < NB; ++k) a[k] = (k < r) ? a[k] : T(0);
-        T* Lt = lds + g * NB * (NB + 1);
-        lds_store_rows<T, NB, G>(Lt, a, r);
-
-        T* Bp = B.get(mat);
-        const int ldb = B.ldim(mat);
-        for (int c0 = 0; c0 < nrhs; c0 += RC) {
-            T x[RC];
-#pragma unroll
-            for (int c = 0; c < RC; ++c) {
-                T v = T(0);
-                if (r < n && c0 + c < nrhs) v = *rhs_elem(Bp, ldb, side_left, r, c0 + c);
-                x[c] = v;
-            }
-            reg_forward<T, NB, G, RC>(a, x, n, my_inv);
-            reg_backward<T, NB, G, RC>(Lt, x, n, r, my_inv);
-#pragma unroll
-            for (int c = 0; c < RC; ++c)
-                if (r < n && c0 + c < nrhs) *rhs_elem(Bp, ldb, side_left, r, c0 + c) = x[c];
-        }
-    }
+    T* Bp = (MODE == KB_MODE_POTRF) ? nullptr : B.get(mat);
+    const int ldb = (MODE == KB_MODE_POTRF) ? 1 : B.ldim(mat);
+    reg_chol_one<T, NB, G, MODE, RC>(n, nrhs, side_left, A.get(mat), A.ldim(mat), Bp, ldb,
+                                     info ? info + mat : nullptr,
+                                     (MODE == KB_MODE_POTRF) ? nullptr : lds + g * NB * (NB + 1), r);
 }
 
 // ---------------------------------------------------------------------------
@@ -217,22 +244,10 @@ constexpr int sposv_mpb()
     return m;
 }
 
-template <int NB, int RC>
-__global__ void __launch_bounds__((sposv_mpb<NB>() * NB))
-k_reg_sposv(int n, int nrhs, int side_left, BatchArg<double> A, BatchArg<double> B,
-            BatchArg<double> X, int* info, int* iter, int batch)
+template <int NB, int G, int RC>
+KB_DEV void reg_sposv_one(int n, int nrhs, int side_left, double* Ap, int lda, double* Bp, int ldb,
+                          double* Xp, int ldx, int* info_out, int* iter_out, float* Lt, int r)
 {
-    constexpr int G = NB;
-    constexpr int MPB = sposv_mpb<NB>();
-    __shared__ float lds[MPB * NB * (NB + 1)];
-
-    const int r = threadIdx.x & (G - 1);
-    const int g = threadIdx.x / G;
-    const long mat = (long)blockIdx.x * MPB + g;
-    if (mat >= batch) return;
-
-    double* Ap = A.get(mat);
-    const int lda = A.ldim(mat);
     // full row r of the symmetric matrix from its lower triangle
     double ad[NB];
 #pragma unroll
@@ -253,16 +268,11 @@ k_reg_sposv(int n, int nrhs, int side_left, BatchArg<double> A, BatchArg<double>
     float d32 = 1.0f, inv32 = 1.0f;
     const int finfo = reg_potrf<float, NB, G>(a, n, r, d32, inv32);
 
-    double* Bp = B.get(mat);
-    const int ldb = B.ldim(mat);
-    double* Xp = X.get(mat);
-    const int ldx = X.ldim(mat);
     int iters = 0;
     bool ok = (finfo == 0);
     if (ok) {
 #pragma unroll
         for (int k = 0; k < NB; ++k) a[k] = (k < r) ? a[k] : 0.0f;
-        float* Lt = lds + g * NB * (NB + 1);
         lds_store_rows<float, NB, G>(Lt, a, r);
         for (int c0 = 0; c0 < nrhs && ok; c0 += RC) {
             double b[RC], x[RC], res[RC];
@@ -357,9 +367,26 @@ k_reg_sposv(int n, int nrhs, int side_left, BatchArg<double> A, BatchArg<double>
         }
     }
     if (r == 0) {
-        if (info) info[mat] = inf;
-        if (iter) iter[mat] = iters;
+        if (info_out) *info_out = inf;
+        if (iter_out) *iter_out = iters;
     }
+}
+
+template <int NB, int RC>
+__global__ void __launch_bounds__((sposv_mpb<NB>() * NB))
+k_reg_sposv(int n, int nrhs, int side_left, BatchArg<double> A, BatchArg<double> B,
+            BatchArg<double> X, int* info, int* iter, int batch)
+{
+    constexpr int G = NB;
+    constexpr int MPB = sposv_mpb<NB>();
+    __shared__ float lds[MPB * NB * (NB + 1)];
+    const int r = threadIdx.x & (G - 1);
+    const int g = threadIdx.x / G;
+    const long mat = (long)blockIdx.x * MPB + g;
+    if (mat >= batch) return;
+    reg_sposv_one<NB, G, RC>(n, nrhs, side_left, A.get(mat), A.ldim(mat), B.get(mat), B.ldim(mat),
+                             X.get(mat), X.ldim(mat), info ? info + mat : nullptr,
+                             iter ? iter + mat : nullptr, lds + g * NB * (NB + 1), r);
 }
 
 // ---------------------------------------------------------------------------

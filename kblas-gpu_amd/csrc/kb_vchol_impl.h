@@ -1,0 +1,221 @@
+// kb_vchol_impl.h -- non-uniform batched Cholesky family in one persistent launch.
+//
+// The reference has no non-uniform POTRF/POSV; its non-uniform TRSM/SYRK go
+// through MAGMA with a host round trip for the maximum sizes
+// (src/batch_triangular/Xtrsm_batch_drivers.cuh:315-320).  Here:
+//   1. a device counting sort orders the matrix ids by order n, largest
+//      first (clear -> histogram -> scan -> scatter; no host synchronisation,
+//      no allocation, fills done by kernels so the sequence can be captured in
+//      a graph);
+//   2. one persistent kernel pulls tickets from that queue with an atomic
+//      counter.  Matrices above the register range are factored by the whole
+//      256-thread workgroup (blocked algorithm); once the queue reaches the
+//      register range every wavefront takes its own tickets and runs the
+//      register-resident routine on one matrix.
+#pragma once
+#include "kb_blk_impl.h"
+#include "kb_reg_impl.h"
+
+constexpr int KB_VQ_BINS = 4096;   // orders >= 4095 share the last bin
+constexpr int KB_VQ_HDR = 2 * KB_VQ_BINS + 64;   // ints: hist, offsets, counters
+
+struct VQueue {
+    int* hist;
+    int* offs;
+    int* counter;
+    int* order;
+};
+
+KB_DEV VQueue vq_view(void* ws, int batch)
+{
+    int* w = (int*)ws;
+    VQueue q;
+    q.hist = w;
+    q.offs = w + KB_VQ_BINS;
+    q.counter = w + 2 * KB_VQ_BINS;
+    q.order = w + KB_VQ_HDR;
+    (void)batch;
+    return q;
+}
+
+// order of A[i]: side_left -> m[i]; right side (and POTRF, which passes vm = null) -> n[i]
+KB_DEV int vq_order(const int* vm, const int* vn, int side_left, long i)
+{
+    return (side_left && vm) ? vm[i] : vn[i];
+}
+KB_DEV int vq_nrhs(const int* vm, const int* vn, int side_left, long i)
+{
+    if (!vm) return 0;
+    return side_left ? vn[i] : vm[i];
+}
+KB_DEV int vq_bin(int n) { return n < 0 ? 0 : (n >= KB_VQ_BINS ? KB_VQ_BINS - 1 : n); }
+
+namespace {
+
+__global__ void k_vq_clear(void* ws)
+{
+    int* w = (int*)ws;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < KB_VQ_HDR; i += gridDim.x * blockDim.x) w[i] = 0;
+}
+
+__global__ void k_vq_hist(void* ws, const int* vm, const int* vn, int side_left, int batch)
+{
+    VQueue q = vq_view(ws, batch);
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < batch; i += (long)gridDim.x * blockDim.x)
+        atomicAdd(&q.hist[vq_bin(vq_order(vm, vn, side_left, i))], 1);
+}
+
+// exclusive scan in descending order of n: offs[b] = sum_{b' > b} hist[b']
+__global__ void __launch_bounds__(1024) k_vq_scan(void* ws, int batch)
+{
+    VQueue q = vq_view(ws, batch);
+    __shared__ int part[1024];
+    constexpr int PER = KB_VQ_BINS / 1024;
+    const int t = threadIdx.x;
+    // thread t owns bins (descending) [B-1-t*PER, ..., B-1-(t*PER+PER-1)]
+    int s = 0;
+    for (int j = 0; j < PER; ++j) s += q.hist[KB_VQ_BINS - 1 - (t * PER + j)];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int v = (t >= off) ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - s;   // exclusive prefix of this thread's chunk
+    for (int j = 0; j < PER; ++j) {
+        const int b = KB_VQ_BINS - 1 - (t * PER + j);
+        q.offs[b] = run;
+        run += q.hist[b];
+    }
+}
+
+__global__ void k_vq_scatter(void* ws, const int* vm, const int* vn, int side_left, int batch)
+{
+    VQueue q = vq_view(ws, batch);
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < batch; i += (long)gridDim.x * blockDim.x) {
+        const int pos = atomicAdd(&q.offs[vq_bin(vq_order(vm, vn, side_left, i))], 1);
+        q.order[pos] = (int)i;
+    }
+}
+
+}  // namespace
+
+// Shared memory of the persistent kernel: the blocked workgroup routine or,
+// in the register phase, one float LDS area per wavefront.
+template <typename T>
+union VSmem {
+    BlkSmem<T> blk;
+    float lt[4][64 * 65];
+};
+
+template <typename T, int MODE>
+KB_DEV void v_small(long mat, int n, int nrhs, int side_left, BatchArg<T>& A, BatchArg<T>& B,
+                    BatchArg<T>& X, int* info, int* iter, float* lt, int lane)
+{
+    if (n <= 0) {
+        if (lane == 0 && info) info[mat] = 0;
+        if (lane == 0 && iter) iter[mat] = 0;
+        return;
+    }
+    if constexpr (MODE == KB_MODE_SPOSV) {
+        reg_sposv_one<64, 64, 4>(n, nrhs, side_left, (double*)A.get(mat), A.ldim(mat), (double*)B.get(mat),
+                                 B.ldim(mat), (double*)X.get(mat), X.ldim(mat), info ? info + mat : nullptr,
+                                 iter ? iter + mat : nullptr, lt, lane);
+    } else if constexpr (MODE == KB_MODE_POTRF) {
+        reg_chol_one<T, 64, 64, MODE, 1>(n, 0, 0, A.get(mat), A.ldim(mat), nullptr, 1,
+                                         info ? info + mat : nullptr, nullptr, lane);
+    } else {
+        // FP64 factor reads its transpose back from A; FP32 keeps it in LDS
+        T* Lt = (sizeof(T) == 4) ? (T*)lt : nullptr;
+        reg_chol_one<T, 64, 64, MODE, 4>(n, nrhs, side_left, A.get(mat), A.ldim(mat), B.get(mat), B.ldim(mat),
+                                         info ? info + mat : nullptr, Lt, lane);
+    }
+}
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(KB_BLK_WG)
+k_vchol(const int* vm, const int* vn, int side_left, BatchArg<T> A, BatchArg<T> B, BatchArg<T> X,
+        int* info, int* iter, int batch, void* ws)
+{
+    __shared__ VSmem<T> S;
+    __shared__ int s_ticket;
+    VQueue q = vq_view(ws, batch);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // ---- workgroup phase: matrices above the register range ----
+    int ticket;
+    for (;;) {
+        if (threadIdx.x == 0) s_ticket = atomicAdd(q.counter, 1);
+        __syncthreads();
+        ticket = s_ticket;
+        __syncthreads();
+        if (ticket >= batch) return;
+        const long mat = q.order[ticket];
+        const int n = vq_order(vm, vn, side_left, mat);
+        if (n <= KB_REG_MAX_N) break;     // sorted: every later ticket is small too
+        const int nrhs = vq_nrhs(vm, vn, side_left, mat);
+        if constexpr (MODE == KB_MODE_SPOSV) {
+            // Orders above the register range are solved in FP64 directly
+            // (LAPACK dsposv's ITER = -1: full precision for implementation
+            // reasons); A receives L as in the dsposv fallback.
+            T* Ap = A.get(mat);
+            const int lda = A.ldim(mat);
+            T* Bp = B.get(mat);
+            T* Xp = X.get(mat);
+            const int ldb = B.ldim(mat), ldx = X.ldim(mat);
+            for (int e = threadIdx.x; e < n * nrhs; e += KB_BLK_WG)
+                rhs_at(Xp, ldx, side_left, e % n, e / n) = rhs_at(Bp, ldb, side_left, e % n, e / n);
+            __syncthreads();
+            const int inf = blk_potrf<T>(Ap, lda, n, S.blk);
+            if (threadIdx.x == 0) { if (info) info[mat] = inf; if (iter) iter[mat] = -1; }
+            if (inf == 0) blk_solve<T>(Ap, lda, n, Xp, ldx, side_left, nrhs, S.blk);
+        } else {
+            T* Ap = A.get(mat);
+            const int lda = A.ldim(mat);
+            int inf = 0;
+            if constexpr (MODE != KB_MODE_POTRS) {
+                inf = blk_potrf<T>(Ap, lda, n, S.blk);
+                if (threadIdx.x == 0 && info) info[mat] = inf;
+            }
+            if constexpr (MODE != KB_MODE_POTRF) {
+                if (inf == 0) blk_solve<T>(Ap, lda, n, B.get(mat), B.ldim(mat), side_left, nrhs, S.blk);
+            }
+        }
+        __syncthreads();
+    }
+    // ---- wavefront phase: wave 0 keeps the ticket it holds, the others draw ----
+    float* lt = S.lt[wave];
+    if (wave != 0) ticket = -1;
+    for (;;) {
+        if (ticket < 0) {
+            int t = 0;
+            if (lane == 0) t = atomicAdd(q.counter, 1);
+            ticket = __builtin_amdgcn_readfirstlane(t);
+        }
+        if (ticket >= batch) return;
+        const long mat = q.order[ticket];
+        const int n = vq_order(vm, vn, side_left, mat);
+        const int nrhs = vq_nrhs(vm, vn, side_left, mat);
+        v_small<T, MODE>(mat, n, nrhs, side_left, A, B, X, info, iter, lt, lane);
+        ticket = -1;
+    }
+}
+
+template <typename T, int MODE>
+static int launch_vchol(const int* vm, const int* vn, int side_left, kb_mat_t A, kb_mat_t B, kb_mat_t X,
+                        int* info, int* iter, int batch, void* ws, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_vq_clear, dim3(8), dim3(1024), 0, s, ws);
+    const int gb = (batch + 255) / 256 < 1024 ? (batch + 255) / 256 : 1024;
+    hipLaunchKernelGGL(k_vq_hist, dim3(gb), dim3(256), 0, s, ws, vm, vn, side_left, batch);
+    hipLaunchKernelGGL(k_vq_scan, dim3(1), dim3(1024), 0, s, ws, batch);
+    hipLaunchKernelGGL(k_vq_scatter, dim3(gb), dim3(256), 0, s, ws, vm, vn, side_left, batch);
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    int grid = cus * 2;
+    if (grid > batch) grid = batch;
+    hipLaunchKernelGGL((k_vchol<T, MODE>), dim3(grid), dim3(KB_BLK_WG), 0, s, vm, vn, side_left,
+                       to_arg<T>(A), to_arg<T>(B), to_arg<T>(X), info, iter, batch, ws);
+    return (int)hipGetLastError();
+}

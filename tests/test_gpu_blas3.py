@@ -1,0 +1,190 @@
+"""GPU parity tests of the batched TRSM / SYRK / GEMM entry points (uniform
+strided, pointer-array and non-uniform) and of the blocked (n > 64) Cholesky
+family, against the CPU oracle.  FP64 bar: relative max-norm error within
+c * k * eps64 of the oracle (k = inner dimension / order)."""
+import numpy as np
+import pytest
+
+import kbtest as kt
+import pykblas
+import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+EPS = kt.EPS64
+RNG = np.random.default_rng(2024)
+
+
+@pytest.fixture(scope="module")
+def h():
+    hh = kt.handle()
+    yield hh
+    hh.close()
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def lower_factor_batch(n, batch, seed):
+    A = po.gen_spd_shift_batch(seed, n, batch)
+    L = np.zeros_like(A)
+    for i in range(batch):
+        L[i] = np.tril(po.potrf(po.col(A, i))[0]).T
+    return L
+
+
+@pytest.mark.parametrize("side", ["L", "R"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("mn", [(16, 16), (4, 64), (70, 33), (5, 100)])
+def test_dtrsm_strided(h, side, trans, mn):
+    m, n = mn
+    batch = 9
+    order = m if side == "L" else n
+    L = lower_factor_batch(order, batch, 3)
+    B = RNG.standard_normal((batch, n, m))
+    dL, dB = kt.to_dev(L), kt.to_dev(B)
+    st = h.lib.kblasDtrsm_batch_strided(h.h, side.encode(), b"L", trans.encode(), b"N", m, n, 0.5,
+                                        kt.P(dL), order, order * order, kt.P(dB), m, m * n, batch)
+    assert st == pykblas.KBLAS_Success
+    X = kt.to_host(dB)
+    for i in range(batch):
+        Xo = po.trsm(side, trans, 0.5, po.col(L, i), po.col(B, i))
+        assert rel(po.col(X, i), Xo) <= 64 * order * EPS
+
+
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("mk", [(16, 16), (32, 32), (70, 20), (8, 130)])
+def test_dsyrk_pointer_array(h, trans, mk):
+    m, k = mk
+    batch = 7
+    shapeA = (m, k) if trans == "N" else (k, m)
+    A = RNG.standard_normal((batch, shapeA[1], shapeA[0]))
+    C = RNG.standard_normal((batch, m, m))
+    dA, dC = kt.to_dev(A), kt.to_dev(C)
+    pA, pC = kt.ptr_array(dA, batch, m * k), kt.ptr_array(dC, batch, m * m)
+    st = h.lib.kblasDsyrk_batch(h.h, b"L", trans.encode(), m, k, -1.0, kt.P(pA), shapeA[0], 0.5, kt.P(pC), m, batch)
+    assert st == pykblas.KBLAS_Success
+    R = kt.to_host(dC)
+    il, iu = np.tril_indices(m), np.triu_indices(m, 1)
+    for i in range(batch):
+        Ro = po.syrk(trans, -1.0, po.col(A, i), 0.5, po.col(C, i))
+        assert rel(po.col(R, i)[il], Ro[il]) <= 16 * k * EPS
+        assert np.array_equal(po.col(R, i)[iu], po.col(C, i)[iu])
+
+
+@pytest.mark.parametrize("ta", ["N", "T"])
+@pytest.mark.parametrize("tb", ["N", "T"])
+@pytest.mark.parametrize("mnk", [(4, 32, 32), (32, 16, 16), (100, 70, 45), (1, 1, 1)])
+def test_dgemm_strided(h, ta, tb, mnk):
+    m, n, k = mnk
+    batch = 6
+    sa = (m, k) if ta == "N" else (k, m)
+    sb = (k, n) if tb == "N" else (n, k)
+    A = RNG.standard_normal((batch, sa[1], sa[0]))
+    B = RNG.standard_normal((batch, sb[1], sb[0]))
+    C = RNG.standard_normal((batch, n, m))
+    dA, dB, dC = kt.to_dev(A), kt.to_dev(B), kt.to_dev(C)
+    st = h.lib.kblasDgemm_batch_strided(h.h, ta.encode(), tb.encode(), m, n, k, -1.0, kt.P(dA), sa[0], sa[0] * sa[1],
+                                        kt.P(dB), sb[0], sb[0] * sb[1], 1.0, kt.P(dC), m, m * n, batch)
+    assert st == pykblas.KBLAS_Success
+    R = kt.to_host(dC)
+    for i in range(batch):
+        Ro = po.gemm(ta, tb, -1.0, po.col(A, i), po.col(B, i), 1.0, po.col(C, i))
+        assert rel(po.col(R, i), Ro) <= 16 * k * EPS
+
+
+def test_nonuniform_trsm_syrk_gemm(h):
+    batch = 12
+    ms = RNG.integers(1, 60, batch).astype(np.int32)
+    ns = RNG.integers(1, 60, batch).astype(np.int32)
+    # TRSM side R, trans T: B_i is m_i x n_i, L_i of order n_i
+    Ls = [np.tril(po.potrf(po.gen_spd_shift(5, i, int(ns[i])))[0]) for i in range(batch)]
+    Bs = [RNG.standard_normal((int(ms[i]), int(ns[i]))) for i in range(batch)]
+    dLs = [kt.to_dev(np.asfortranarray(L).T.copy()) for L in Ls]   # column-major storage
+    dBs = [kt.to_dev(np.asfortranarray(B).T.copy()) for B in Bs]
+    torch = kt.torch_dev()
+    pL = torch.tensor([t.data_ptr() for t in dLs], dtype=torch.int64, device="cuda")
+    pB = torch.tensor([t.data_ptr() for t in dBs], dtype=torch.int64, device="cuda")
+    dm, dn = kt.to_dev(ms), kt.to_dev(ns)
+    st = h.lib.kblasDtrsm_vbatch(h.h, b"R", b"L", b"T", b"N", kt.P(dm), kt.P(dn), 0, 0, 1.0, kt.P(pL), kt.P(dn),
+                                 kt.P(pB), kt.P(dm), batch)
+    assert st == pykblas.KBLAS_Success
+    for i in range(batch):
+        X = kt.to_host(dBs[i]).T
+        Xo = po.trsm("R", "T", 1.0, Ls[i], Bs[i])
+        assert rel(X, Xo) <= 64 * int(ns[i]) * EPS
+    # SYRK non-uniform: C_i (m_i x m_i) -= A_i A_i^T, A_i m_i x n_i
+    As = [RNG.standard_normal((int(ms[i]), int(ns[i]))) for i in range(batch)]
+    Cs = [RNG.standard_normal((int(ms[i]), int(ms[i]))) for i in range(batch)]
+    dAs = [kt.to_dev(A.T.copy()) for A in As]
+    dCs = [kt.to_dev(C.T.copy()) for C in Cs]
+    pA = torch.tensor([t.data_ptr() for t in dAs], dtype=torch.int64, device="cuda")
+    pC = torch.tensor([t.data_ptr() for t in dCs], dtype=torch.int64, device="cuda")
+    st = h.lib.kblasDsyrk_vbatch(h.h, b"L", b"N", kt.P(dm), kt.P(dn), 0, 0, -1.0, kt.P(pA), kt.P(dm), 1.0,
+                                 kt.P(pC), kt.P(dm), batch)
+    assert st == pykblas.KBLAS_Success
+    for i in range(batch):
+        R = kt.to_host(dCs[i]).T
+        Ro = po.syrk("N", -1.0, As[i], 1.0, Cs[i])
+        il = np.tril_indices(int(ms[i]))
+        assert rel(R[il], Ro[il]) <= 16 * int(ns[i]) * EPS
+    # GEMM non-uniform: C_i (m_i x n_i) = A_i (m_i x k_i) B_i (k_i x n_i)
+    ks = RNG.integers(1, 40, batch).astype(np.int32)
+    As = [RNG.standard_normal((int(ms[i]), int(ks[i]))) for i in range(batch)]
+    Bs = [RNG.standard_normal((int(ks[i]), int(ns[i]))) for i in range(batch)]
+    Cs = [RNG.standard_normal((int(ms[i]), int(ns[i]))) for i in range(batch)]
+    dAs = [kt.to_dev(A.T.copy()) for A in As]
+    dBs = [kt.to_dev(B.T.copy()) for B in Bs]
+    dCs = [kt.to_dev(C.T.copy()) for C in Cs]
+    pA = torch.tensor([t.data_ptr() for t in dAs], dtype=torch.int64, device="cuda")
+    pB = torch.tensor([t.data_ptr() for t in dBs], dtype=torch.int64, device="cuda")
+    pC = torch.tensor([t.data_ptr() for t in dCs], dtype=torch.int64, device="cuda")
+    dk = kt.to_dev(ks)
+    st = h.lib.kblasDgemm_vbatch(h.h, b"N", b"N", kt.P(dm), kt.P(dn), kt.P(dk), 0, 0, 0, 2.0, kt.P(pA), kt.P(dm),
+                                 kt.P(pB), kt.P(dk), 0.0, kt.P(pC), kt.P(dm), batch)
+    assert st == pykblas.KBLAS_Success
+    for i in range(batch):
+        R = kt.to_host(dCs[i]).T
+        assert rel(R, 2.0 * As[i] @ Bs[i]) <= 16 * int(ks[i]) * EPS
+
+
+@pytest.mark.parametrize("n", [65, 100, 128, 200, 257])
+def test_dposv_blocked_orders(h, n):
+    batch = 5
+    A = po.gen_spd_shift_batch(12, n, batch)
+    B = po.gen_rhs_batch(13, n, 4, batch)
+    Ain = A.copy()
+    for c in range(n):
+        Ain[:, c, :c] = np.nan
+    dA, dB = kt.to_dev(Ain), kt.to_dev(B)
+    info = kt.to_dev(np.full(batch, -7, np.int32))
+    st = h.lib.kblasDposv_batch_strided(h.h, b"L", b"L", n, 4, kt.P(dA), n, n * n, kt.P(dB), n, 4 * n, batch, kt.P(info))
+    assert st == pykblas.KBLAS_Success
+    L, X = kt.to_host(dA), kt.to_host(dB)
+    assert np.all(kt.to_host(info) == 0)
+    for i in range(batch):
+        Ai = po.col(A, i)
+        assert po.potrf_berr(Ai, po.col(L, i)) <= 4 * n * EPS
+        _, Xo, _ = po.posv("L", Ai, po.col(B, i))
+        assert rel(po.col(X, i), Xo) <= 64 * n * EPS
+        assert np.all(np.isnan(po.col(L, i)[np.triu_indices(n, 1)]))
+
+
+@pytest.mark.parametrize("n", [80, 150])
+def test_dpotrf_blocked_info_and_right_side_solve(h, n):
+    batch = 4
+    A = po.gen_spd_shift_batch(14, n, batch)
+    A[2, 70, 70] = -1.0   # leading minor 71 not positive definite
+    B = po.gen_rhs_batch(15, 3, n, batch)
+    dA, dB = kt.to_dev(A), kt.to_dev(B)
+    info = kt.to_dev(np.full(batch, -7, np.int32))
+    st = h.lib.kblasDposv_batch_strided(h.h, b"R", b"L", 3, n, kt.P(dA), n, n * n, kt.P(dB), 3, 3 * n, batch, kt.P(info))
+    assert st == pykblas.KBLAS_Success
+    inf = kt.to_host(info)
+    _, info_o = po.potrf(po.col(A, 2))
+    assert inf[2] == info_o and inf[2] > 0
+    X = kt.to_host(dB)
+    for i in (0, 1, 3):
+        assert inf[i] == 0
+        _, Xo, _ = po.posv("R", po.col(A, i), po.col(B, i))
+        assert rel(po.col(X, i), Xo) <= 64 * n * EPS
