@@ -2,7 +2,8 @@
 """Benchmark of the batched SPD path on one MI355X.
 
 Default workload (the headline metric): 50,000 SPD 64x64 systems,
-A = Q diag(s) Q^T with Q Haar-orthogonal (QR of a Gaussian) and s log-spaced
+A = Q diag(s) Q^T with Q Haar-orthogonal (Stewart: product of Householder
+reflectors from Gaussian vectors) and s log-spaced
 so cond(A) = kappa, log10(kappa) ~ U[2, 6]; 4 right-hand sides ~ N(0,1) in the
 reference's Right-side layout (B is 4 x 64 per system).  One step = one
 mixed-precision batched POSV (FP32 factor + FP64 refinement, FP64 fallback)
@@ -53,17 +54,26 @@ def gen_inputs(torch, cfg, seed):
         A += n * torch.eye(n, device=dev, dtype=torch.float64)
         kappa = None
     else:
-        Gm = torch.randn((bc, n, n), generator=g, device=dev, dtype=torch.float64)
-        Q, _ = torch.linalg.qr(Gm)
+        # A = Q diag(s) Q^T, Q = H_0 ... H_{n-2} with H_p a Householder reflector
+        # from a Gaussian vector supported on entries p..n-1 (Stewart's
+        # construction of a Haar orthogonal matrix), applied two-sidedly to
+        # diag(s) in a few hundred batched kernels.
         lk = 2.0 + 4.0 * torch.rand((bc, 1), generator=g, device=dev, dtype=torch.float64)
         t = torch.rand((bc, n), generator=g, device=dev, dtype=torch.float64)
         t[:, 0] = 0.0
         t[:, -1] = 1.0
         s = torch.pow(10.0, -lk * t)
-        A = torch.bmm(Q * s[:, None, :], Q.transpose(1, 2).contiguous())
+        A = torch.diag_embed(s)
+        for p in range(n - 2, -1, -1):
+            v = torch.randn((bc, n, 1), generator=g, device=dev, dtype=torch.float64)
+            v[:, :p, :] = 0.0
+            tau = 2.0 / (v * v).sum(dim=1, keepdim=True)          # (bc,1,1)
+            w = tau * torch.bmm(A, v)                              # (bc,n,1)
+            alpha = 0.5 * tau * (v * w).sum(dim=1, keepdim=True)
+            w = w - alpha * v
+            A = A - torch.bmm(v, w.transpose(1, 2).contiguous()) - torch.bmm(w, v.transpose(1, 2).contiguous())
         A = 0.5 * (A + A.transpose(1, 2))
         kappa = torch.pow(10.0, lk[:, 0])
-        del Gm, Q
     A = A.contiguous()                       # symmetric: row- and column-major agree
     B = None
     if nrhs:

@@ -9,6 +9,11 @@ extern "C" int kb_reg_sposv_16(int, int, int, kb_mat_t, kb_mat_t, kb_mat_t, int*
 extern "C" int kb_reg_sposv_32(int, int, int, kb_mat_t, kb_mat_t, kb_mat_t, int*, int*, int, hipStream_t);
 extern "C" int kb_reg_sposv_64(int, int, int, kb_mat_t, kb_mat_t, kb_mat_t, int*, int*, int, hipStream_t);
 
+extern "C" int kb_tile_chol_d(int, int, int, int, kb_mat_t, kb_mat_t, int*, int, hipStream_t);
+extern "C" int kb_tile_chol_s(int, int, int, int, kb_mat_t, kb_mat_t, int*, int, hipStream_t);
+
+// n <= 16 and n <= 32: one matrix per 16/32-lane group, all in VGPRs;
+// 32 < n <= 64: MFMA-tiled kernel, one matrix per wavefront.
 extern "C" int kb_launch_reg_chol(char prec, int mode, int n, int nrhs, int side_left,
                                   kb_mat_t A, kb_mat_t B, int* info, int batch, hipStream_t s)
 {
@@ -16,10 +21,20 @@ extern "C" int kb_launch_reg_chol(char prec, int mode, int n, int nrhs, int side
     if (prec == 'd') {
         if (n <= 16) return kb_reg_chol_d16(mode, n, nrhs, side_left, A, B, info, batch, s);
         if (n <= 32) return kb_reg_chol_d32(mode, n, nrhs, side_left, A, B, info, batch, s);
-        return kb_reg_chol_d64(mode, n, nrhs, side_left, A, B, info, batch, s);
+        return kb_tile_chol_d(mode, n, nrhs, side_left, A, B, info, batch, s);
     }
     if (n <= 16) return kb_reg_chol_s16(mode, n, nrhs, side_left, A, B, info, batch, s);
     if (n <= 32) return kb_reg_chol_s32(mode, n, nrhs, side_left, A, B, info, batch, s);
+    return kb_tile_chol_s(mode, n, nrhs, side_left, A, B, info, batch, s);
+}
+
+// The row-per-lane n <= 64 kernels stay reachable for comparison (and serve
+// the non-uniform kernel's wavefront phase).
+extern "C" int kb_launch_rowlane_chol64(char prec, int mode, int n, int nrhs, int side_left,
+                                        kb_mat_t A, kb_mat_t B, int* info, int batch, hipStream_t s)
+{
+    if (n < 1 || n > KB_REG_MAX_N || batch < 1) return (int)hipErrorInvalidValue;
+    if (prec == 'd') return kb_reg_chol_d64(mode, n, nrhs, side_left, A, B, info, batch, s);
     return kb_reg_chol_s64(mode, n, nrhs, side_left, A, B, info, batch, s);
 }
 

@@ -37,8 +37,8 @@ KB_DEV int reg_potrf(T (&a)[NB], int n, int r, T& my_diag, T& my_inv)
             if (!(d > T(0))) {
                 info = j + 1;
             } else {
-                T s = kb_sqrt<T>(d);
-                T inv = T(1) / s;
+                const T inv = kb_rsqrt<T>(d);
+                const T s = d * inv;
                 if (r == j) { my_diag = s; my_inv = inv; }
                 a[j] = (r == j) ? s : a[j] * inv;
 #pragma unroll

@@ -1,0 +1,324 @@
+// kb_tile_impl.h -- MFMA-tiled batched Cholesky family for 32 < n <= 64, one
+// matrix per wavefront (gfx950).
+//
+// Storage: the lower triangle as ten 16x16 tiles T(I,J), I >= J.  Lane l holds
+// row (l & 15) and columns 4*(l >> 4) .. 4*(l >> 4) + 3 of every tile, i.e. the
+// accumulator layout of v_mfma_f64_16x16x4f64 for the tile's transpose:
+//     D[i][j] lives at lane j + 16*(i/4), register i%4   =>   D = T^T.
+// Consequences:
+//   * a 4-column panel (global columns 16J+4g .. 16J+4g+3) sits entirely in
+//     DPP row g (lanes 16g..16g+15) of the tiles of tile-column J, so its
+//     unblocked factorization needs only `row_newbcast` broadcasts;
+//   * the rank-4 trailing update T(I',J') -= P(I') P(J')^T of that panel is
+//     ONE 16x16x4 MFMA per tile: A operand = -P(J') and B operand = P(I'),
+//     both in the "lane x + 16k holds P[x][k]" layout obtained from the
+//     panel's DPP row through a 2 KB LDS transpose.
+// The O(n^3) work therefore runs on the matrix cores (80 MFMAs at n = 64)
+// while the VALU only does the 4-wide panels, instead of the reference's
+// recursion of separate launches with cuBLAS GEMMs
+// (src/batch_triangular/Xpotrf_batch_drivers.cuh:91-131).
+//
+// Solves (POSV/POTRS) use a second layout: right-hand side c of a group of
+// four sits in DPP row c, lane 16c + i holding row 16K + i of block K, so the
+// substitution broadcasts are again `row_newbcast`; each 16x16 factor tile is
+// replicated per DPP row through a small LDS staging buffer.
+#pragma once
+#include "kb_device.h"
+#include "kb_launch.h"
+#include "kb_reg_impl.h"
+
+typedef double kb_d4 __attribute__((ext_vector_type(4)));
+typedef float kb_f4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct TileV;
+template <> struct TileV<double> { typedef kb_d4 v4; };
+template <> struct TileV<float> { typedef kb_f4 v4; };
+
+KB_DEV kb_d4 mfma16x16x4(double a, double b, kb_d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+KB_DEV kb_f4 mfma16x16x4(float a, float b, kb_f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+
+constexpr KB_DEV int tix(int I, int J) { return I * (I + 1) / 2 + J; }
+
+// Per-wave LDS: a 64 x 4 panel transpose buffer and a 16 x 17 tile staging buffer.
+template <typename T>
+struct TileLds {
+    T pan[4][16][4];
+    T tile[16][17];
+    T inv[16];
+    T y[4][64];      // right-hand-side blocks during the solve
+    T invd[4][64];   // 1/L(16K+i,16K+i) per lane
+};
+
+KB_DEV void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---------------------------------------------------------------------------
+// factorization
+// ---------------------------------------------------------------------------
+template <typename T>
+KB_DEV void tile_load(typename TileV<T>::v4 (&t)[10], const T* A, int lda, int n, int lane)
+{
+    const int i = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int I = 0; I < 4; ++I) {
+#pragma unroll
+        for (int J = 0; J <= I; ++J) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * I + i, col = 16 * J + 4 * g + r;
+                const bool in = row < n && col <= row;
+                const int rr = in ? row : 0, cc = in ? col : 0;
+                const T ld = A[rr + (long)cc * lda];
+                // identity padding beyond n; the strict upper triangle is never read
+                t[tix(I, J)][r] = in ? ld : ((row == col) ? T(1) : T(0));
+            }
+        }
+    }
+}
+
+template <typename T>
+KB_DEV void tile_store(const typename TileV<T>::v4 (&t)[10], T* A, int lda, int n, int lane)
+{
+    const int i = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int I = 0; I < 4; ++I) {
+#pragma unroll
+        for (int J = 0; J <= I; ++J) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * I + i, col = 16 * J + 4 * g + r;
+                if (row < n && col <= row) A[row + (long)col * lda] = t[tix(I, J)][r];
+            }
+        }
+    }
+}
+
+// In-register Cholesky of the (padded) 64 x 64 matrix.  invd[K] receives
+// 1/L(16K+i, 16K+i) in every lane with (lane & 15) == i.  Returns LAPACK info.
+template <typename T>
+KB_DEV int tile_potrf(typename TileV<T>::v4 (&t)[10], T (&invd)[4], int n, TileLds<T>& S, int lane)
+{
+    const int i = lane & 15, g = lane >> 4;
+    int info = 0;
+#pragma unroll
+    for (int J = 0; J < 4; ++J) {
+#pragma unroll
+        for (int gp = 0; gp < 4; ++gp) {
+            if (info) break;
+            // ---- panel: global columns 16J + 4gp + c, held by DPP row gp ----
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int pl = 16 * gp + 4 * gp + c;          // lane holding the pivot
+                const T d = bcastk<64>(t[tix(J, J)][c], pl);  // uniform
+                if (!(d > T(0))) {
+                    info = 16 * J + 4 * gp + c + 1;
+                    break;
+                }
+                const T rs = kb_rsqrt<T>(d);
+                const T s = d * rs;
+                if (i == 4 * gp + c) invd[J] = rs;
+                if (g == gp) {
+#pragma unroll
+                    for (int I = J; I < 4; ++I) {
+                        T v = t[tix(I, J)][c] * rs;
+                        if (I == J && i == 4 * gp + c) v = s;
+                        t[tix(I, J)][c] = v;
+                    }
+#pragma unroll
+                    for (int cp = c + 1; cp < 4; ++cp) {
+                        const T l = bcastk<16>(t[tix(J, J)][c], 4 * gp + cp);
+#pragma unroll
+                        for (int I = J; I < 4; ++I)
+                            t[tix(I, J)][cp] = kb_fma<T>(-t[tix(I, J)][c], l, t[tix(I, J)][cp]);
+                    }
+                }
+            }
+            if (info) break;
+            // ---- transpose the panel: lane x + 16k receives P(X)[x][k] ----
+            if (g == gp) {
+#pragma unroll
+                for (int X = J; X < 4; ++X)
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) S.pan[X][i][k] = t[tix(X, J)][k];
+            }
+            wave_lds_sync();
+            T ts[4];
+#pragma unroll
+            for (int X = 0; X < 4; ++X) ts[X] = (X >= J) ? S.pan[X][i][g] : T(0);
+            wave_lds_sync();
+            // ---- rank-4 trailing update on the matrix cores ----
+#pragma unroll
+            for (int Jp = J; Jp < 4; ++Jp) {
+                if (Jp == J && gp == 3) continue;            // tile column finished
+                T a = -ts[Jp];
+                if (Jp == J && i < 4 * (gp + 1)) a = T(0);   // columns already factored
+#pragma unroll
+                for (int Ip = Jp; Ip < 4; ++Ip) t[tix(Ip, Jp)] = mfma16x16x4(a, ts[Ip], t[tix(Ip, Jp)]);
+            }
+        }
+        if (info) break;
+    }
+    return info;
+}
+
+// ---------------------------------------------------------------------------
+// solve: right-hand side c of the group of four in DPP row c
+// ---------------------------------------------------------------------------
+// Stage factor tile (I, J) from L in global memory (padding rows/columns at or
+// beyond n read as the identity) into LDS and read row i (ROW) or column i of
+// it into v[16].  STRICT (diagonal tiles) keeps strictly-lower entries only and
+// scales entry j by 1/L(j,j) (invk = this lane's 1/L(i,i)), i.e. hands out the
+// unit-triangular factor so the sweeps need no per-step division.
+template <typename T, bool ROW, bool STRICT>
+KB_DEV void tile_stage(const T* L, int lda, int n, int I, int J, T (&v)[16], TileLds<T>& S, int lane,
+                       T invk = T(1))
+{
+    const int i = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int gr = 16 * I + i, gc = 16 * J + 4 * g + r;
+        const bool in = gr < n && gc < n && gc <= gr;
+        // branch-free: always load a valid lower element, then select
+        const int rr = in ? gr : 0, cc = in ? gc : 0;
+        const T ld = L[rr + (long)cc * lda];
+        S.tile[i][4 * g + r] = in ? ld : ((gr == gc) ? T(1) : T(0));
+    }
+    if (STRICT && g == 0) S.inv[i] = invk;
+    wave_lds_sync();
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        T x = ROW ? S.tile[i][j] : S.tile[j][i];
+        if (STRICT) x = (ROW ? (j < i) : (j > i)) ? x * S.inv[j] : T(0);
+        v[j] = x;
+    }
+    wave_lds_sync();
+}
+
+// (L L^T) X = B for the group's four right-hand sides.  Right-hand side c of
+// the group sits in DPP row c: S.y[K][16c + i] = row 16K + i.  The tile loops
+// are runtime loops over LDS-resident blocks so that register use stays that
+// of a single 16 x 16 step (occupancy 4).
+template <typename T>
+KB_DEV void tile_solve(const T* L, int lda, int n, TileLds<T>& S, int lane)
+{
+    T v[16];
+    // forward: L y = b, L = Lu D with Lu unit lower: z = Lu^-1 b, y = D^-1 z
+#pragma nounroll
+    for (int K = 0; K < 4; ++K) {
+        const T ik = S.invd[K][lane];
+        T yk = S.y[K][lane];
+        tile_stage<T, true, true>(L, lda, n, K, K, v, S, lane, ik);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) yk = kb_fma<T>(-v[j], bcastk<16>(yk, j), yk);
+        yk *= ik;
+        S.y[K][lane] = yk;
+#pragma nounroll
+        for (int I = K + 1; I < 4; ++I) {
+            T yi = S.y[I][lane];
+            tile_stage<T, true, false>(L, lda, n, I, K, v, S, lane);
+#pragma unroll
+            for (int j = 0; j < 16; ++j) yi = kb_fma<T>(-v[j], bcastk<16>(yk, j), yi);
+            S.y[I][lane] = yi;
+        }
+    }
+    // backward: L^T x = y, v[j] = L(j,i)/L(j,j) on the diagonal tiles
+#pragma nounroll
+    for (int K = 3; K >= 0; --K) {
+        const T ik = S.invd[K][lane];
+        T yk = S.y[K][lane];
+#pragma nounroll
+        for (int I = K + 1; I < 4; ++I) {
+            const T xi = S.y[I][lane];
+            tile_stage<T, false, false>(L, lda, n, I, K, v, S, lane);
+#pragma unroll
+            for (int j = 0; j < 16; ++j) yk = kb_fma<T>(-v[j], bcastk<16>(xi, j), yk);
+        }
+        tile_stage<T, false, true>(L, lda, n, K, K, v, S, lane, ik);
+#pragma unroll
+        for (int j = 15; j >= 0; --j) yk = kb_fma<T>(-v[j], bcastk<16>(yk, j), yk);
+        yk *= ik;
+        S.y[K][lane] = yk;
+    }
+}
+
+template <typename T, int MODE>
+KB_DEV void tile_chol_one(int n, int nrhs, int side_left, T* Ap, int lda, T* Bp, int ldb, int* info_out,
+                          TileLds<T>& S, int lane)
+{
+    T invd[4] = {T(1), T(1), T(1), T(1)};
+    int inf = 0;
+    const int i = lane & 15;
+    if constexpr (MODE != KB_MODE_POTRS) {
+        typename TileV<T>::v4 t[10];
+        tile_load<T>(t, Ap, lda, n, lane);
+        inf = tile_potrf<T>(t, invd, n, S, lane);
+        tile_store<T>(t, Ap, lda, n, lane);
+        if (lane == 0 && info_out) *info_out = inf;
+        // the solve re-reads L from memory: make this wavefront's stores visible
+        if (MODE != KB_MODE_POTRF && inf == 0) __threadfence();
+    } else {
+#pragma unroll
+        for (int K = 0; K < 4; ++K) {
+            const int row = 16 * K + i;
+            invd[K] = (row < n) ? T(1) / Ap[row + (long)row * lda] : T(1);
+        }
+    }
+    if constexpr (MODE != KB_MODE_POTRF) {
+        if (inf) return;
+#pragma unroll
+        for (int K = 0; K < 4; ++K) S.invd[K][lane] = invd[K];
+        const int c = lane >> 4;
+        for (int c0 = 0; c0 < nrhs; c0 += 4) {
+            const bool act = c0 + c < nrhs;
+#pragma unroll
+            for (int K = 0; K < 4; ++K) {
+                const int row = 16 * K + i;
+                const bool in = act && row < n;
+                const T ld = *rhs_elem(Bp, ldb, side_left, in ? row : 0, in ? c0 + c : 0);
+                S.y[K][lane] = in ? ld : T(0);
+            }
+            wave_lds_sync();
+            tile_solve<T>(Ap, lda, n, S, lane);
+            wave_lds_sync();
+#pragma unroll
+            for (int K = 0; K < 4; ++K) {
+                const int row = 16 * K + i;
+                if (act && row < n) *rhs_elem(Bp, ldb, side_left, row, c0 + c) = S.y[K][lane];
+            }
+        }
+    }
+}
+
+// 4 wavefronts (4 matrices) per 256-thread workgroup
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_tile_chol(int n, int nrhs, int side_left, BatchArg<T> A, BatchArg<T> B, int* info, int batch)
+{
+    __shared__ TileLds<T> S[4];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const long mat = (long)blockIdx.x * 4 + wave;
+    if (mat >= batch) return;
+    T* Bp = (MODE == KB_MODE_POTRF) ? nullptr : B.get(mat);
+    const int ldb = (MODE == KB_MODE_POTRF) ? 1 : B.ldim(mat);
+    tile_chol_one<T, MODE>(n, nrhs, side_left, A.get(mat), A.ldim(mat), Bp, ldb, info ? info + mat : nullptr,
+                           S[wave], lane);
+}
+
+template <typename T>
+static int launch_tile(int mode, int n, int nrhs, int side_left, kb_mat_t A, kb_mat_t B, int* info, int batch,
+                       hipStream_t s)
+{
+    dim3 grid((unsigned)((batch + 3) / 4));
+    BatchArg<T> a = to_arg<T>(A), b = to_arg<T>(B);
+    if (mode == KB_MODE_POTRF)
+        hipLaunchKernelGGL((k_tile_chol<T, KB_MODE_POTRF>), grid, dim3(256), 0, s, n, nrhs, side_left, a, b, info, batch);
+    else if (mode == KB_MODE_POSV)
+        hipLaunchKernelGGL((k_tile_chol<T, KB_MODE_POSV>), grid, dim3(256), 0, s, n, nrhs, side_left, a, b, info, batch);
+    else
+        hipLaunchKernelGGL((k_tile_chol<T, KB_MODE_POTRS>), grid, dim3(256), 0, s, n, nrhs, side_left, a, b, info, batch);
+    return (int)hipGetLastError();
+}
