@@ -37,6 +37,9 @@ CONFIGS = {
                workload="dpotrf_batch_strided n=32 x 10000, A=B*B^T+32I"),
     "c2": dict(op="posv", n=16, batch=100000, nrhs=1, gen="shift",
                workload="dposv_batch_strided n=16 x 100000, 1 RHS, A=B*B^T+16I"),
+    "c3": dict(op="vposv", n=256, nmin=8, batch=20000, nrhs=4, gen="shift",
+               workload="dposv_vbatch n~U{8..256} x 20000, 4 RHS, pointer arrays, one persistent launch "
+                        "over a device size-sorted queue, each step a captured HIP graph"),
     "c4": dict(op="sposv", n=64, batch=50000, nrhs=4, gen="cond",
                workload="dsposv_batch_strided n=64 x 50000, 4 RHS, cond in [1e2,1e6], mixed FP32+IR vs FP64"),
 }
@@ -80,6 +83,31 @@ def gen_inputs(torch, cfg, seed):
         # Right side (reference layout): B is nrhs x n, ldb = nrhs, column-major
         B = torch.randn((bc, n, nrhs), generator=g, device=dev, dtype=torch.float64).contiguous()
     return A, B, kappa
+
+
+def gen_vbatch(torch, cfg, seed):
+    """Non-uniform batch: sizes n_i ~ U{nmin..n}, A_i = B_i B_i^T + n_i I packed one
+    after another (column-major, lda = n_i), B_i (right side: nrhs x n_i, ldb = nrhs)."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    bc, nmax, nmin, nrhs = cfg["batch"], cfg["n"], cfg["nmin"], cfg["nrhs"]
+    ns = torch.randint(nmin, nmax + 1, (bc,), generator=g, device="cuda", dtype=torch.int32)
+    ns_h = ns.cpu().numpy()
+    offA = np.zeros(bc + 1, dtype=np.int64)
+    offA[1:] = np.cumsum(ns_h.astype(np.int64) ** 2)
+    offB = np.zeros(bc + 1, dtype=np.int64)
+    offB[1:] = np.cumsum(ns_h.astype(np.int64) * nrhs)
+    Abuf = torch.empty(int(offA[-1]), device="cuda", dtype=torch.float64)
+    Bbuf = torch.randn(int(offB[-1]), generator=g, device="cuda", dtype=torch.float64)
+    for n in np.unique(ns_h):
+        idx = np.nonzero(ns_h == n)[0]
+        k = len(idx)
+        Bm = torch.rand((k, n, n), generator=g, device="cuda", dtype=torch.float64) * 2 - 1
+        A = torch.bmm(Bm, Bm.transpose(1, 2).contiguous())
+        A = 0.5 * (A + A.transpose(1, 2)) + n * torch.eye(int(n), device="cuda", dtype=torch.float64)
+        pos = torch.from_numpy((offA[idx][:, None] + np.arange(n * n)[None, :]).reshape(-1)).cuda()
+        Abuf.index_copy_(0, pos, A.reshape(-1))
+    return ns, ns_h, offA, offB, Abuf, Bbuf
 
 
 def time_steps(torch, call, K, W):
@@ -143,6 +171,35 @@ def cpu_baseline(cfg, A_s, B_s, X_gpu, cores):
     return res
 
 
+def cpu_baseline_vbatch(cfg, vb, K, unit):
+    """Oracle FP64 POSV (OpenMP over matrices) on the first matrices of the same batch."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    import ctypes
+    L = po.lib()
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    L.orc_set_threads(cores)
+    ns, ns_h, offA, offB, Abuf, Bbuf = vb
+    nrhs = cfg["nrhs"]
+    bs = min(2000, cfg["batch"])
+    A = Abuf[: int(offA[bs])].cpu().numpy()
+    B = Bbuf[: int(offB[bs])].cpu().numpy()
+    best = None
+    for rep in range(3):
+        Aw, Bw = A.copy(), B.copy()
+        t0 = time.perf_counter()
+        # one OpenMP batch per distinct order keeps the loop inside C
+        for n in np.unique(ns_h[:bs]):
+            idx = np.nonzero(ns_h[:bs] == n)[0]
+            for i in idx:
+                L.orc_posv(b"R", nrhs, int(n), Aw[offA[i]:].ctypes.data_as(ctypes.c_void_p), int(n),
+                           Bw[offB[i]:].ctypes.data_as(ctypes.c_void_p), nrhs)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    return {"value": bs / best, "unit": unit, "cores": 1, "kind": "port",
+            "sample": "first %d systems of the non-uniform batch, oracle FP64 POSV, one thread" % bs}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,7 +221,12 @@ def main():
     n, bc, nrhs = cfg["n"], cfg["batch"], cfg["nrhs"]
     K, W = args.steps, args.warmup
     print("bench: config %s (%s), generating inputs" % (args.config, cfg["workload"]), file=sys.stderr, flush=True)
-    A0, B0, kappa = gen_inputs(torch, cfg, 1234 + int(args.config[1]))
+    vb = None
+    if cfg["op"] == "vposv":
+        vb = gen_vbatch(torch, cfg, 1234 + int(args.config[1]))
+        A0, B0, kappa = None, None, None
+    else:
+        A0, B0, kappa = gen_inputs(torch, cfg, 1234 + int(args.config[1]))
     torch.cuda.synchronize()
     h = pykblas.Handle(torch.cuda.current_stream().cuda_stream)
     lib = h.lib
@@ -173,7 +235,51 @@ def main():
     NA = n * n
     out = {}
 
-    if cfg["op"] == "potrf":
+    if cfg["op"] == "vposv":
+        ns, ns_h, offA, offB, Abuf0, Bbuf0 = vb
+        nrhs_arr = torch.full((bc,), nrhs, dtype=torch.int32, device="cuda")
+        lib.kblas_posv_vbatch_wsquery(h.h, n, bc)
+        assert lib.kblasAllocateWorkspace(h.h) == pykblas.KBLAS_Success
+        offA_d = torch.from_numpy(offA[:-1] * 8).cuda()
+        offB_d = torch.from_numpy(offB[:-1] * 8).cuda()
+        As = [Abuf0.clone() for _ in range(K + W)]
+        Bs = [Bbuf0.clone() for _ in range(K + W)]
+        pA = [offA_d + a.data_ptr() for a in As]
+        pB = [offB_d + b.data_ptr() for b in Bs]
+        torch.cuda.synchronize()
+
+        def issue(i):
+            st = lib.kblasDposv_vbatch(h.h, b"R", b"L", nrhs_arr.data_ptr(), ns.data_ptr(), nrhs, n,
+                                       pA[i].data_ptr(), ns.data_ptr(), pB[i].data_ptr(), nrhs_arr.data_ptr(),
+                                       bc, info.data_ptr())
+            assert st == pykblas.KBLAS_Success, st
+        # one HIP graph per step (sort kernels + persistent kernel), captured on torch's stream
+        graphs = []
+        cap = torch.cuda.Stream()
+        for i in range(K + W):
+            gr = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(gr, stream=cap):
+                h.set_stream(torch.cuda.current_stream().cuda_stream)
+                issue(i)
+            graphs.append(gr)
+        h.set_stream(torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+
+        def call(i):
+            graphs[i].replay()
+        t, wall = time_steps(torch, call, K, W)
+        nn = ns_h.astype(np.float64)
+        per_unit_bytes = float(np.mean(8 * nn * (nn + 1) + 16 * nn * nrhs))
+        flops = float(np.sum(nn ** 3 / 3 + nn ** 2 / 2 + nn / 6 + 2 * nrhs * nn ** 2))
+        value = bc * K / t
+        metric, unit = "solves/s (non-uniform batched DPOSV n~U{8..256}, 4 RHS)", "solves/s"
+        dtype = "f64"
+        X_sample = None
+        out["gflops"] = flops * K / t / 1e9
+        out["fp64_peak_frac"] = out["gflops"] / 78600.0
+        out["info_nonzero"] = int((info != 0).sum().item())
+        del As
+    elif cfg["op"] == "potrf":
         As = [A0.clone() for _ in range(K + W)]
 
         def call(i):
@@ -255,7 +361,9 @@ def main():
     }
     line.update(out)
     torch.cuda.synchronize()
-    if not args.no_cpu:
+    if not args.no_cpu and cfg["op"] == "vposv":
+        line["cpu_baseline"] = cpu_baseline_vbatch(cfg, vb, K, unit)
+    elif not args.no_cpu:
         cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
         bs = args.cpu_sample or {"c1": 20000, "c2": 50000, "c4": 2000}[args.config]
         bs = min(bs, bc)

@@ -136,17 +136,21 @@ KB_DEV int group_all(int pred)
     return pred;
 }
 
-// 1/sqrt(x): hardware estimate refined by one Newton-Raphson step (FP64:
-// v_rsq_f64 then r += r/2 (1 - x r^2), ~1 ulp); FP32 v_rsq_f32 is already ~1 ulp.
+// 1/sqrt(x): hardware estimate refined by Newton-Raphson (FP64: v_rsq_f64 then
+// twice r += r/2 (1 - x r^2), ~1 ulp); FP32 v_rsq_f32 is already ~1 ulp.
 // Cholesky only needs the pivot's square root to a few ulp for backward
 // stability; LAPACK's dpotf2 likewise scales the column by the reciprocal.
 template <typename T> KB_DEV T kb_rsqrt(T x);
 template <> KB_DEV double kb_rsqrt<double>(double x)
 {
     double r = __builtin_amdgcn_rsq(x);
-    double t = x * r;
-    double e = __builtin_fma(-t, r, 1.0);
-    return __builtin_fma(0.5 * r, e, r);
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        double t = x * r;
+        double e = __builtin_fma(-t, r, 1.0);
+        r = __builtin_fma(0.5 * r, e, r);
+    }
+    return r;
 }
 template <> KB_DEV float kb_rsqrt<float>(float x) { return __builtin_amdgcn_rsqf(x); }
 

@@ -38,12 +38,22 @@ extern "C" int kb_launch_rowlane_chol64(char prec, int mode, int n, int nrhs, in
     return kb_reg_chol_s64(mode, n, nrhs, side_left, A, B, info, batch, s);
 }
 
+extern "C" int kb_tile_sposv(int, int, int, kb_mat_t, kb_mat_t, kb_mat_t, int*, int*, int, hipStream_t);
+
 extern "C" int kb_launch_reg_sposv(int n, int nrhs, int side_left, kb_mat_t A, kb_mat_t B,
                                    kb_mat_t X, int* info, int* iter, int batch, hipStream_t s)
 {
     if (n < 1 || n > KB_REG_MAX_N || batch < 1) return (int)hipErrorInvalidValue;
     if (n <= 16) return kb_reg_sposv_16(n, nrhs, side_left, A, B, X, info, iter, batch, s);
     if (n <= 32) return kb_reg_sposv_32(n, nrhs, side_left, A, B, X, info, iter, batch, s);
+    return kb_tile_sposv(n, nrhs, side_left, A, B, X, info, iter, batch, s);
+}
+
+// row-per-lane mixed kernel for 32 < n <= 64, kept reachable for comparison
+extern "C" int kb_launch_rowlane_sposv64(int n, int nrhs, int side_left, kb_mat_t A, kb_mat_t B,
+                                         kb_mat_t X, int* info, int* iter, int batch, hipStream_t s)
+{
+    if (n < 1 || n > KB_REG_MAX_N || batch < 1) return (int)hipErrorInvalidValue;
     return kb_reg_sposv_64(n, nrhs, side_left, A, B, X, info, iter, batch, s);
 }
 

@@ -3,8 +3,10 @@
 //
 // Storage: the lower triangle as ten 16x16 tiles T(I,J), I >= J.  Lane l holds
 // row (l & 15) and columns 4*(l >> 4) .. 4*(l >> 4) + 3 of every tile, i.e. the
-// accumulator layout of v_mfma_f64_16x16x4f64 for the tile's transpose:
-//     D[i][j] lives at lane j + 16*(i/4), register i%4   =>   D = T^T.
+// accumulator layout of v_mfma_f32_16x16x4f32 for the tile's transpose:
+//     D[i][j] lives at lane j + 16*(i/4), register i%4   =>   D = T^T
+// (v_mfma_f64_16x16x4f64 interleaves the rows of D instead; the A operand is
+// fed with permuted rows so the FP64 product lands in the same storage).
 // Consequences:
 //   * a 4-column panel (global columns 16J+4g .. 16J+4g+3) sits entirely in
 //     DPP row g (lanes 16g..16g+15) of the tiles of tile-column J, so its
@@ -100,7 +102,7 @@ KB_DEV void tile_store(const typename TileV<T>::v4 (&t)[10], T* A, int lda, int 
 // In-register Cholesky of the (padded) 64 x 64 matrix.  invd[K] receives
 // 1/L(16K+i, 16K+i) in every lane with (lane & 15) == i.  Returns LAPACK info.
 template <typename T>
-KB_DEV int tile_potrf(typename TileV<T>::v4 (&t)[10], T (&invd)[4], int n, TileLds<T>& S, int lane)
+KB_DEV int tile_potrf(typename TileV<T>::v4 (&t)[10], T (&invd)[4], int n, T (&pan)[4][16][4], int lane)
 {
     const int i = lane & 15, g = lane >> 4;
     int info = 0;
@@ -143,19 +145,28 @@ KB_DEV int tile_potrf(typename TileV<T>::v4 (&t)[10], T (&invd)[4], int n, TileL
 #pragma unroll
                 for (int X = J; X < 4; ++X)
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) S.pan[X][i][k] = t[tix(X, J)][k];
+                    for (int k = 0; k < 4; ++k) pan[X][i][k] = t[tix(X, J)][k];
             }
             wave_lds_sync();
-            T ts[4];
+            // The accumulator row index i of the MFMA is the tile column.  For
+            // the FP32 instruction D[i][j] lives at lane j + 16(i/4), register
+            // i%4 (blocked, the tile storage as is); the FP64 instruction
+            // interleaves rows, D[i][j] at lane j + 16(i%4), register i/4, so
+            // its A operand row i must carry tile column sig(i) = 4(i%4) + i/4.
+            const int ci = (sizeof(T) == 8) ? 4 * (i & 3) + (i >> 2) : i;
+            T ts[4], ta[4];
 #pragma unroll
-            for (int X = 0; X < 4; ++X) ts[X] = (X >= J) ? S.pan[X][i][g] : T(0);
+            for (int X = 0; X < 4; ++X) {
+                ts[X] = (X >= J) ? pan[X][i][g] : T(0);
+                ta[X] = (X >= J) ? pan[X][ci][g] : T(0);
+            }
             wave_lds_sync();
             // ---- rank-4 trailing update on the matrix cores ----
 #pragma unroll
             for (int Jp = J; Jp < 4; ++Jp) {
-                if (Jp == J && gp == 3) continue;            // tile column finished
-                T a = -ts[Jp];
-                if (Jp == J && i < 4 * (gp + 1)) a = T(0);   // columns already factored
+                if (Jp == J && gp == 3) continue;             // tile column finished
+                T a = -ta[Jp];
+                if (Jp == J && ci < 4 * (gp + 1)) a = T(0);   // columns already factored
 #pragma unroll
                 for (int Ip = Jp; Ip < 4; ++Ip) t[tix(Ip, Jp)] = mfma16x16x4(a, ts[Ip], t[tix(Ip, Jp)]);
             }
@@ -198,50 +209,65 @@ KB_DEV void tile_stage(const T* L, int lda, int n, int I, int J, T (&v)[16], Til
     wave_lds_sync();
 }
 
-// (L L^T) X = B for the group's four right-hand sides.  Right-hand side c of
-// the group sits in DPP row c: S.y[K][16c + i] = row 16K + i.  The tile loops
-// are runtime loops over LDS-resident blocks so that register use stays that
-// of a single 16 x 16 step (occupancy 4).
+// Factor tiles read from L in global memory (staged through LDS).
 template <typename T>
-KB_DEV void tile_solve(const T* L, int lda, int n, TileLds<T>& S, int lane)
+struct GlobalTiles {
+    const T* L;
+    int lda, n;
+    TileLds<T>* S;
+    template <bool ROW, bool STRICT>
+    KB_DEV void get(int I, int J, T (&v)[16], int lane, T invk) const
+    {
+        tile_stage<T, ROW, STRICT>(L, lda, n, I, J, v, *S, lane, invk);
+    }
+};
+
+// (L L^T) X = B for the group's four right-hand sides.  Right-hand side c of
+// the group sits in DPP row c: y[K][16c + i] = row 16K + i.  The tile loops
+// are runtime loops over LDS-resident blocks so that register use stays that
+// of a single 16 x 16 step.  src.get<ROW,STRICT>(I, J, v, lane, 1/L(i,i))
+// returns row / column i of factor tile (I,J) (STRICT: unit-scaled strictly
+// lower part of a diagonal tile).
+template <typename T, typename Src>
+KB_DEV void tile_solve(const Src& src, T (&y)[4][64], const T (&invd)[4][64], int lane)
 {
     T v[16];
     // forward: L y = b, L = Lu D with Lu unit lower: z = Lu^-1 b, y = D^-1 z
 #pragma nounroll
     for (int K = 0; K < 4; ++K) {
-        const T ik = S.invd[K][lane];
-        T yk = S.y[K][lane];
-        tile_stage<T, true, true>(L, lda, n, K, K, v, S, lane, ik);
+        const T ik = invd[K][lane];
+        T yk = y[K][lane];
+        src.template get<true, true>(K, K, v, lane, ik);
 #pragma unroll
         for (int j = 0; j < 16; ++j) yk = kb_fma<T>(-v[j], bcastk<16>(yk, j), yk);
         yk *= ik;
-        S.y[K][lane] = yk;
+        y[K][lane] = yk;
 #pragma nounroll
         for (int I = K + 1; I < 4; ++I) {
-            T yi = S.y[I][lane];
-            tile_stage<T, true, false>(L, lda, n, I, K, v, S, lane);
+            T yi = y[I][lane];
+            src.template get<true, false>(I, K, v, lane, T(1));
 #pragma unroll
             for (int j = 0; j < 16; ++j) yi = kb_fma<T>(-v[j], bcastk<16>(yk, j), yi);
-            S.y[I][lane] = yi;
+            y[I][lane] = yi;
         }
     }
     // backward: L^T x = y, v[j] = L(j,i)/L(j,j) on the diagonal tiles
 #pragma nounroll
     for (int K = 3; K >= 0; --K) {
-        const T ik = S.invd[K][lane];
-        T yk = S.y[K][lane];
+        const T ik = invd[K][lane];
+        T yk = y[K][lane];
 #pragma nounroll
         for (int I = K + 1; I < 4; ++I) {
-            const T xi = S.y[I][lane];
-            tile_stage<T, false, false>(L, lda, n, I, K, v, S, lane);
+            const T xi = y[I][lane];
+            src.template get<false, false>(I, K, v, lane, T(1));
 #pragma unroll
             for (int j = 0; j < 16; ++j) yk = kb_fma<T>(-v[j], bcastk<16>(xi, j), yk);
         }
-        tile_stage<T, false, true>(L, lda, n, K, K, v, S, lane, ik);
+        src.template get<false, true>(K, K, v, lane, ik);
 #pragma unroll
         for (int j = 15; j >= 0; --j) yk = kb_fma<T>(-v[j], bcastk<16>(yk, j), yk);
         yk *= ik;
-        S.y[K][lane] = yk;
+        y[K][lane] = yk;
     }
 }
 
@@ -255,7 +281,7 @@ KB_DEV void tile_chol_one(int n, int nrhs, int side_left, T* Ap, int lda, T* Bp,
     if constexpr (MODE != KB_MODE_POTRS) {
         typename TileV<T>::v4 t[10];
         tile_load<T>(t, Ap, lda, n, lane);
-        inf = tile_potrf<T>(t, invd, n, S, lane);
+        inf = tile_potrf<T>(t, invd, n, S.pan, lane);
         tile_store<T>(t, Ap, lda, n, lane);
         if (lane == 0 && info_out) *info_out = inf;
         // the solve re-reads L from memory: make this wavefront's stores visible
@@ -282,7 +308,8 @@ KB_DEV void tile_chol_one(int n, int nrhs, int side_left, T* Ap, int lda, T* Bp,
                 S.y[K][lane] = in ? ld : T(0);
             }
             wave_lds_sync();
-            tile_solve<T>(Ap, lda, n, S, lane);
+            GlobalTiles<T> src{Ap, lda, n, &S};
+            tile_solve<T>(src, S.y, S.invd, lane);
             wave_lds_sync();
 #pragma unroll
             for (int K = 0; K < 4; ++K) {
@@ -322,3 +349,4 @@ static int launch_tile(int mode, int n, int nrhs, int side_left, kb_mat_t A, kb_
         hipLaunchKernelGGL((k_tile_chol<T, KB_MODE_POTRS>), grid, dim3(256), 0, s, n, nrhs, side_left, a, b, info, batch);
     return (int)hipGetLastError();
 }
+

@@ -59,8 +59,9 @@ def test_sposv_conditioned_batch(h, n, side):
             assert np.array_equal(Aout[i], A[i])     # A untouched when refinement served
 
 
-def test_sposv_fallback_on_fp32_failure(h):
-    n, batch = 32, 6
+@pytest.mark.parametrize("n", [32, 48])
+def test_sposv_fallback_on_fp32_failure(h, n):
+    batch = 6
     A, kap = po.gen_spd_cond_batch(9, n, batch, 12.0, 12.0)   # kappa 1e12: FP32 factor fails
     easy, _ = po.gen_spd_cond_batch(9, n, 1, 2.0, 2.0, first=1)
     A[1] = easy[0]                                             # one easy matrix
