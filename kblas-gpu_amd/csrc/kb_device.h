@@ -26,9 +26,40 @@ struct BatchArg {
     T* const* ptrs;      // pointer-array mode (device array of device pointers)
     const int* lds;      // non-uniform leading dimensions (device), or nullptr
     int ld;              // uniform leading dimension
-    KB_DEV T* get(long b) const { return ptrs ? ptrs[b] : base + b * stride; }
+    KB_DEV T* get(long b) const
+    {
+        return ptrs ? ptrs[b] : base + b * stride;
+    }
     KB_DEV int ldim(long b) const { return lds ? lds[b] : ld; }
 };
+
+// ---------------------------------------------------------------------------
+// Global-memory accessors.  Batch operands always live in device (global)
+// memory; going through an address_space(1) pointer lets the compiler emit
+// global_load/global_store instead of flat accesses, which would also count
+// against the LDS wait counter.
+// ---------------------------------------------------------------------------
+template <typename T>
+KB_DEV T gld(const T* p)
+{
+    return *(const __attribute__((address_space(1))) T*)p;
+}
+template <typename T>
+KB_DEV void gst(T* p, T v)
+{
+    *(__attribute__((address_space(1))) T*)p = v;
+}
+
+// Make this wavefront's global stores visible to its own later global loads
+// (by other lanes of the same wavefront).  Workgroup scope suffices: the
+// stores and loads go through the same CU's vector L1, so this is a wait for
+// the stores to complete, not a device-scope fence, which on gfx950 writes
+// back and invalidates the XCD's whole L2.
+KB_DEV void wave_global_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
 
 // ---------------------------------------------------------------------------
 // bit casts

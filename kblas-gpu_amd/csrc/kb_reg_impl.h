@@ -26,27 +26,27 @@ template <> KB_DEV float kb_fma<float>(float a, float b, float c) { return __bui
 // Returns LAPACK info (0, or j+1 for the first non-positive pivot); after a
 // failure the remaining columns are left as they were, like dpotrf.
 // ---------------------------------------------------------------------------
-template <typename T, int NB, int G>
+template <typename T, int NB, int G, bool EX = false>
 KB_DEV int reg_potrf(T (&a)[NB], int n, int r, T& my_diag, T& my_inv)
 {
+    // EX: n == NB, every guard below folds away.  A non-positive pivot only
+    // records info (first one wins); the later columns then carry NaNs, the
+    // columns before it are final -- the part of the factor LAPACK defines.
     int info = 0;
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
-        if (j < n && info == 0) {
-            T d = bcastk<G>(a[j], j);
-            if (!(d > T(0))) {
-                info = j + 1;
-            } else {
-                const T inv = kb_rsqrt<T>(d);
-                const T s = d * inv;
-                if (r == j) { my_diag = s; my_inv = inv; }
-                a[j] = (r == j) ? s : a[j] * inv;
+        if (EX || j < n) {
+            const T d = bcastk<G>(a[j], j);
+            info = (info == 0 && !(d > T(0))) ? j + 1 : info;
+            const T inv = kb_rsqrt<T>(d);
+            const T s = d * inv;
+            if (r == j) { my_diag = s; my_inv = inv; }
+            a[j] = (r == j) ? s : a[j] * inv;
 #pragma unroll
-                for (int k = 0; k < NB; ++k) {
-                    if (k > j && k < n) {
-                        T lk = bcastk<G>(a[j], k);
-                        a[k] = kb_fma<T>(-a[j], lk, a[k]);
-                    }
+            for (int k = 0; k < NB; ++k) {
+                if (k > j && (EX || k < n)) {
+                    const T lk = bcastk<G>(a[j], k);
+                    a[k] = kb_fma<T>(-a[j], lk, a[k]);
                 }
             }
         }
@@ -56,12 +56,12 @@ KB_DEV int reg_potrf(T (&a)[NB], int n, int r, T& my_diag, T& my_inv)
 
 // Forward sweep L y = b on RC vectors; a[] holds the strictly lower row
 // (zeros at and above the diagonal), my_inv = 1/L(r,r).
-template <typename T, int NB, int G, int RC>
+template <typename T, int NB, int G, int RC, bool EX = false>
 KB_DEV void reg_forward(const T (&a)[NB], T (&x)[RC], int n, T my_inv)
 {
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
-        if (k < n) {
+        if (EX || k < n) {
             T ik = bcastk<G>(my_inv, k);
 #pragma unroll
             for (int c = 0; c < RC; ++c) {
@@ -77,12 +77,12 @@ KB_DEV void reg_forward(const T (&a)[NB], T (&x)[RC], int n, T my_inv)
 // Backward sweep L^T x = y; Lt points at the group's LDS copy of L stored by
 // rows with stride NB+1 and zeros on/above the diagonal, so lane r reads
 // L(k,r) at Lt[k*(NB+1) + r] (consecutive lanes -> consecutive banks).
-template <typename T, int NB, int G, int RC>
+template <typename T, int NB, int G, int RC, bool EX = false>
 KB_DEV void reg_backward(const T* Lt, T (&x)[RC], int n, int r, T my_inv)
 {
 #pragma unroll
     for (int k = NB - 1; k >= 0; --k) {
-        if (k < n) {
+        if (EX || k < n) {
             T l = Lt[k * (NB + 1) + r];
             T ik = bcastk<G>(my_inv, k);
 #pragma unroll
@@ -138,7 +138,7 @@ KB_DEV void reg_backward_global(const T* Ap, int lda, T (&x)[RC], int n, int r, 
 #pragma unroll
     for (int k = NB - 1; k >= 0; --k) {
         if (k < n) {
-            T l = (r < k) ? Ap[k + (long)r * lda] : T(0);
+            T l = (r < k) ? gld(Ap + k + (long)r * lda) : T(0);
             T ik = bcastk<G>(my_inv, k);
 #pragma unroll
             for (int c = 0; c < RC; ++c) {
@@ -153,30 +153,35 @@ KB_DEV void reg_backward_global(const T* Ap, int lda, T (&x)[RC], int n, int r, 
 
 // One matrix on one G-lane group (lane r = row r).  Lt: the group's LDS area
 // of NB*(NB+1) elements, or nullptr to read the transposed factor from A.
-template <typename T, int NB, int G, int MODE, int RC>
+template <typename T, int NB, int G, int MODE, int RC, bool EX = false>
 KB_DEV void reg_chol_one(int n, int nrhs, int side_left, T* Ap, int lda, T* Bp, int ldb,
                          int* info_out, T* Lt, int r)
 {
     T a[NB];
     T my_diag = T(1), my_inv = T(1);
+    const bool rowin = EX || r < n;
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
-        T v = T(0);
-        if (k <= r && r < n) v = Ap[r + (long)k * lda];
-        a[k] = v;
+        // branch-free: lanes above the diagonal (and padding lanes) load the
+        // column's diagonal element, a line the lower part fetches anyway
+        const bool in = k <= r && rowin && (EX || k < n);
+        const int rr = in ? r : ((EX || k < n) ? k : 0);
+        const int kk = (EX || k < n) ? k : 0;
+        const T v = gld(Ap + rr + (long)kk * lda);
+        a[k] = in ? v : T(0);
         if constexpr (MODE == KB_MODE_POTRS) {
-            if (k == r) my_diag = v;
+            if (k == r) my_diag = in ? v : T(1);
         }
     }
     int inf = 0;
     if constexpr (MODE != KB_MODE_POTRS) {
-        inf = reg_potrf<T, NB, G>(a, n, r, my_diag, my_inv);
+        inf = reg_potrf<T, NB, G, EX>(a, n, r, my_diag, my_inv);
         // write L (lower triangle only; the strict upper part is never touched).
         // After a failed pivot the unfinished columns keep their partially
         // updated values, as dpotrf leaves them.
 #pragma unroll
         for (int k = 0; k < NB; ++k)
-            if (k <= r && r < n) Ap[r + (long)k * lda] = a[k];
+            if (k <= r && r < n) gst(Ap + r + (long)k * lda, a[k]);
         if (r == 0 && info_out) *info_out = inf;
     } else {
         my_inv = T(1) / my_diag;
@@ -187,26 +192,26 @@ KB_DEV void reg_chol_one(int n, int nrhs, int side_left, T* Ap, int lda, T* Bp, 
 #pragma unroll
         for (int k = 0; k < NB; ++k) a[k] = (k < r) ? a[k] : T(0);
         if (Lt) lds_store_rows<T, NB, G>(Lt, a, r);
-        else __threadfence();
+        else wave_global_sync();
         for (int c0 = 0; c0 < nrhs; c0 += RC) {
             T x[RC];
 #pragma unroll
             for (int c = 0; c < RC; ++c) {
                 T v = T(0);
-                if (r < n && c0 + c < nrhs) v = *rhs_elem(Bp, ldb, side_left, r, c0 + c);
+                if (r < n && c0 + c < nrhs) v = gld(rhs_elem(Bp, ldb, side_left, r, c0 + c));
                 x[c] = v;
             }
-            reg_forward<T, NB, G, RC>(a, x, n, my_inv);
-            if (Lt) reg_backward<T, NB, G, RC>(Lt, x, n, r, my_inv);
+            reg_forward<T, NB, G, RC, EX>(a, x, n, my_inv);
+            if (Lt) reg_backward<T, NB, G, RC, EX>(Lt, x, n, r, my_inv);
             else reg_backward_global<T, NB, G, RC>(Ap, lda, x, n, r, my_inv);
 #pragma unroll
             for (int c = 0; c < RC; ++c)
-                if (r < n && c0 + c < nrhs) *rhs_elem(Bp, ldb, side_left, r, c0 + c) = x[c];
+                if (r < n && c0 + c < nrhs) gst(rhs_elem(Bp, ldb, side_left, r, c0 + c), x[c]);
         }
     }
 }
 
-template <typename T, int NB, int MODE, int RC>
+template <typename T, int NB, int MODE, int RC, bool EX>
 __global__ void __launch_bounds__((reg_mpb<T, NB, MODE>() * NB))
 k_reg_chol(int n, int nrhs, int side_left, BatchArg<T> A, BatchArg<T> B, int* info, int batch)
 {
@@ -221,7 +226,7 @@ k_reg_chol(int n, int nrhs, int side_left, BatchArg<T> A, BatchArg<T> B, int* in
     if (mat >= batch) return;   // the whole group leaves together
     T* Bp = (MODE == KB_MODE_POTRF) ? nullptr : B.get(mat);
     const int ldb = (MODE == KB_MODE_POTRF) ? 1 : B.ldim(mat);
-    reg_chol_one<T, NB, G, MODE, RC>(n, nrhs, side_left, A.get(mat), A.ldim(mat), Bp, ldb,
+    reg_chol_one<T, NB, G, MODE, RC, EX>(n, nrhs, side_left, A.get(mat), A.ldim(mat), Bp, ldb,
                                      info ? info + mat : nullptr,
                                      (MODE == KB_MODE_POTRF) ? nullptr : lds + g * NB * (NB + 1), r);
 }
@@ -253,7 +258,7 @@ KB_DEV void reg_sposv_one(int n, int nrhs, int side_left, double* Ap, int lda, d
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
         double v = 0.0;
-        if (r < n && k < n) v = (k <= r) ? Ap[r + (long)k * lda] : Ap[k + (long)r * lda];
+        if (r < n && k < n) v = gld(Ap + ((k <= r) ? r + (long)k * lda : k + (long)r * lda));
         ad[k] = v;
     }
     double rowsum = 0.0;
@@ -279,7 +284,7 @@ KB_DEV void reg_sposv_one(int n, int nrhs, int side_left, double* Ap, int lda, d
 #pragma unroll
             for (int c = 0; c < RC; ++c) {
                 double v = 0.0;
-                if (r < n && c0 + c < nrhs) v = *rhs_elem(Bp, ldb, side_left, r, c0 + c);
+                if (r < n && c0 + c < nrhs) v = gld(rhs_elem(Bp, ldb, side_left, r, c0 + c));
                 b[c] = v;
                 x[c] = 0.0;
                 res[c] = v;
@@ -320,7 +325,7 @@ KB_DEV void reg_sposv_one(int n, int nrhs, int side_left, double* Ap, int lda, d
             if (it > iters) iters = it;
 #pragma unroll
             for (int c = 0; c < RC; ++c)
-                if (r < n && c0 + c < nrhs) *rhs_elem(Xp, ldx, side_left, r, c0 + c) = x[c];
+                if (r < n && c0 + c < nrhs) gst(rhs_elem(Xp, ldx, side_left, r, c0 + c), x[c]);
         }
     }
     int inf = 0;
@@ -331,17 +336,17 @@ KB_DEV void reg_sposv_one(int n, int nrhs, int side_left, double* Ap, int lda, d
         inf = reg_potrf<double, NB, G>(ad, n, r, dd, dinv);
 #pragma unroll
         for (int k = 0; k < NB; ++k)
-            if (k <= r && r < n && k < n) Ap[r + (long)k * lda] = ad[k];
+            if (k <= r && r < n && k < n) gst(Ap + r + (long)k * lda, ad[k]);
         if (inf == 0) {
 #pragma unroll
             for (int k = 0; k < NB; ++k) ad[k] = (k < r) ? ad[k] : 0.0;
-            __threadfence();
+            wave_global_sync();
             for (int c0 = 0; c0 < nrhs; c0 += RC) {
                 double x[RC];
 #pragma unroll
                 for (int c = 0; c < RC; ++c) {
                     double v = 0.0;
-                    if (r < n && c0 + c < nrhs) v = *rhs_elem(Bp, ldb, side_left, r, c0 + c);
+                    if (r < n && c0 + c < nrhs) v = gld(rhs_elem(Bp, ldb, side_left, r, c0 + c));
                     x[c] = v;
                 }
                 reg_forward<double, NB, G, RC>(ad, x, n, dinv);
@@ -349,7 +354,7 @@ KB_DEV void reg_sposv_one(int n, int nrhs, int side_left, double* Ap, int lda, d
 #pragma unroll
                 for (int k = NB - 1; k >= 0; --k) {
                     if (k < n) {
-                        double l = (r < k) ? Ap[k + (long)r * lda] : 0.0;
+                        double l = (r < k) ? gld(Ap + k + (long)r * lda) : 0.0;
                         double ik = bcastk<G>(dinv, k);
 #pragma unroll
                         for (int c = 0; c < RC; ++c) {
@@ -361,7 +366,7 @@ KB_DEV void reg_sposv_one(int n, int nrhs, int side_left, double* Ap, int lda, d
 #pragma unroll
                 for (int c = 0; c < RC; ++c) {
                     x[c] *= dinv;
-                    if (r < n && c0 + c < nrhs) *rhs_elem(Xp, ldx, side_left, r, c0 + c) = x[c];
+                    if (r < n && c0 + c < nrhs) gst(rhs_elem(Xp, ldx, side_left, r, c0 + c), x[c]);
                 }
             }
         }
@@ -411,8 +416,12 @@ static int launch_chol(int n, int nrhs, int side_left, kb_mat_t A, kb_mat_t B, i
     constexpr int MPB = reg_mpb<T, NB, MODE>();
     dim3 grid((unsigned)((batch + MPB - 1) / MPB));
     dim3 block(MPB * NB);
-    hipLaunchKernelGGL((k_reg_chol<T, NB, MODE, RC>), grid, block, 0, s, n, nrhs, side_left,
-                       to_arg<T>(A), to_arg<T>(B), info, batch);
+    if (n == NB && NB <= 32)
+        hipLaunchKernelGGL((k_reg_chol<T, NB, MODE, RC, true>), grid, block, 0, s, n, nrhs, side_left,
+                           to_arg<T>(A), to_arg<T>(B), info, batch);
+    else
+        hipLaunchKernelGGL((k_reg_chol<T, NB, MODE, RC, false>), grid, block, 0, s, n, nrhs, side_left,
+                           to_arg<T>(A), to_arg<T>(B), info, batch);
     return (int)hipGetLastError();
 }
 

@@ -74,7 +74,7 @@ KB_DEV void tile_load(typename TileV<T>::v4 (&t)[10], const T* A, int lda, int n
                 const int row = 16 * I + i, col = 16 * J + 4 * g + r;
                 const bool in = row < n && col <= row;
                 const int rr = in ? row : 0, cc = in ? col : 0;
-                const T ld = A[rr + (long)cc * lda];
+                const T ld = gld(A + rr + (long)cc * lda);
                 // identity padding beyond n; the strict upper triangle is never read
                 t[tix(I, J)][r] = in ? ld : ((row == col) ? T(1) : T(0));
             }
@@ -93,7 +93,7 @@ KB_DEV void tile_store(const typename TileV<T>::v4 (&t)[10], T* A, int lda, int 
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int row = 16 * I + i, col = 16 * J + 4 * g + r;
-                if (row < n && col <= row) A[row + (long)col * lda] = t[tix(I, J)][r];
+                if (row < n && col <= row) gst(A + row + (long)col * lda, t[tix(I, J)][r]);
             }
         }
     }
@@ -195,7 +195,7 @@ KB_DEV void tile_stage(const T* L, int lda, int n, int I, int J, T (&v)[16], Til
         const bool in = gr < n && gc < n && gc <= gr;
         // branch-free: always load a valid lower element, then select
         const int rr = in ? gr : 0, cc = in ? gc : 0;
-        const T ld = L[rr + (long)cc * lda];
+        const T ld = gld(L + rr + (long)cc * lda);
         S.tile[i][4 * g + r] = in ? ld : ((gr == gc) ? T(1) : T(0));
     }
     if (STRICT && g == 0) S.inv[i] = invk;
@@ -285,12 +285,12 @@ KB_DEV void tile_chol_one(int n, int nrhs, int side_left, T* Ap, int lda, T* Bp,
         tile_store<T>(t, Ap, lda, n, lane);
         if (lane == 0 && info_out) *info_out = inf;
         // the solve re-reads L from memory: make this wavefront's stores visible
-        if (MODE != KB_MODE_POTRF && inf == 0) __threadfence();
+        if (MODE != KB_MODE_POTRF && inf == 0) wave_global_sync();
     } else {
 #pragma unroll
         for (int K = 0; K < 4; ++K) {
             const int row = 16 * K + i;
-            invd[K] = (row < n) ? T(1) / Ap[row + (long)row * lda] : T(1);
+            invd[K] = (row < n) ? T(1) / gld(Ap + row + (long)row * lda) : T(1);
         }
     }
     if constexpr (MODE != KB_MODE_POTRF) {
@@ -304,7 +304,7 @@ KB_DEV void tile_chol_one(int n, int nrhs, int side_left, T* Ap, int lda, T* Bp,
             for (int K = 0; K < 4; ++K) {
                 const int row = 16 * K + i;
                 const bool in = act && row < n;
-                const T ld = *rhs_elem(Bp, ldb, side_left, in ? row : 0, in ? c0 + c : 0);
+                const T ld = gld(rhs_elem(Bp, ldb, side_left, in ? row : 0, in ? c0 + c : 0));
                 S.y[K][lane] = in ? ld : T(0);
             }
             wave_lds_sync();
@@ -314,7 +314,7 @@ KB_DEV void tile_chol_one(int n, int nrhs, int side_left, T* Ap, int lda, T* Bp,
 #pragma unroll
             for (int K = 0; K < 4; ++K) {
                 const int row = 16 * K + i;
-                if (act && row < n) *rhs_elem(Bp, ldb, side_left, row, c0 + c) = S.y[K][lane];
+                if (act && row < n) gst(rhs_elem(Bp, ldb, side_left, row, c0 + c), S.y[K][lane]);
             }
         }
     }

@@ -134,7 +134,7 @@ k_tile_sposv(int n, int nrhs, int side_left, BatchArg<double> A, BatchArg<double
         for (int K = 0; K < 4; ++K) {
             const int row = 16 * K + i;
             const bool in = act && row < n;
-            const double ld = *rhs_elem(Bp, ldb, side_left, in ? row : 0, in ? c0 + c : 0);
+            const double ld = gld(rhs_elem(Bp, ldb, side_left, in ? row : 0, in ? c0 + c : 0));
             S.m.x[K][lane] = 0.0;
             S.m.r[K][lane] = in ? ld : 0.0;
         }
@@ -154,7 +154,7 @@ k_tile_sposv(int n, int nrhs, int side_left, BatchArg<double> A, BatchArg<double
             for (int K = 0; K < 4; ++K) {
                 const int row = 16 * K + i;
                 const bool in = act && row < n;
-                const double ld = *rhs_elem(Bp, ldb, side_left, in ? row : 0, in ? c0 + c : 0);
+                const double ld = gld(rhs_elem(Bp, ldb, side_left, in ? row : 0, in ? c0 + c : 0));
                 S.m.x[K][lane] += (double)S.m.z[K][lane];
                 S.m.r[K][lane] = in ? ld : 0.0;
             }
@@ -208,7 +208,7 @@ k_tile_sposv(int n, int nrhs, int side_left, BatchArg<double> A, BatchArg<double
 #pragma unroll
         for (int K = 0; K < 4; ++K) {
             const int row = 16 * K + i;
-            if (act && row < n) *rhs_elem(Xp, ldx, side_left, row, c0 + c) = S.m.x[K][lane];
+            if (act && row < n) gst(rhs_elem(Xp, ldx, side_left, row, c0 + c), S.m.x[K][lane]);
         }
     }
 
@@ -220,7 +220,7 @@ k_tile_sposv(int n, int nrhs, int side_left, BatchArg<double> A, BatchArg<double
         inf = tile_potrf<double>(ta, invd, n, S.d.pan, lane);
         tile_store<double>(ta, Ap, lda, n, lane);
         if (inf == 0) {
-            __threadfence();
+            wave_global_sync();
 #pragma unroll
             for (int K = 0; K < 4; ++K) S.d.invd[K][lane] = invd[K];
             for (int c0 = 0; c0 < nrhs; c0 += 4) {
@@ -229,7 +229,7 @@ k_tile_sposv(int n, int nrhs, int side_left, BatchArg<double> A, BatchArg<double
                 for (int K = 0; K < 4; ++K) {
                     const int row = 16 * K + i;
                     const bool in = act && row < n;
-                    const double ld = *rhs_elem(Bp, ldb, side_left, in ? row : 0, in ? c0 + c : 0);
+                    const double ld = gld(rhs_elem(Bp, ldb, side_left, in ? row : 0, in ? c0 + c : 0));
                     S.d.y[K][lane] = in ? ld : 0.0;
                 }
                 wave_lds_sync();
@@ -239,7 +239,7 @@ k_tile_sposv(int n, int nrhs, int side_left, BatchArg<double> A, BatchArg<double
 #pragma unroll
                 for (int K = 0; K < 4; ++K) {
                     const int row = 16 * K + i;
-                    if (act && row < n) *rhs_elem(Xp, ldx, side_left, row, c0 + c) = S.d.y[K][lane];
+                    if (act && row < n) gst(rhs_elem(Xp, ldx, side_left, row, c0 + c), S.d.y[K][lane]);
                 }
             }
         }

@@ -212,7 +212,9 @@ static int launch_vchol(const int* vm, const int* vn, int side_left, kb_mat_t A,
     hipLaunchKernelGGL(k_vq_scan, dim3(1), dim3(1024), 0, s, ws, batch);
     hipLaunchKernelGGL(k_vq_scatter, dim3(gb), dim3(256), 0, s, ws, vm, vn, side_left, batch);
     int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+        cus = 256;
     int grid = cus * 2;
     if (grid > batch) grid = batch;
     hipLaunchKernelGGL((k_vchol<T, MODE>), dim3(grid), dim3(KB_BLK_WG), 0, s, vm, vn, side_left,
