@@ -11,6 +11,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include "kb_launch.h"
 
 #define KB_DEV __device__ __forceinline__
 
@@ -32,6 +33,26 @@ struct BatchArg {
     }
     KB_DEV int ldim(long b) const { return lds ? lds[b] : ld; }
 };
+
+template <typename T>
+static BatchArg<T> to_arg(kb_mat_t m)
+{
+    BatchArg<T> a;
+    a.base = (T*)m.base;
+    a.stride = m.stride;
+    a.ptrs = (T* const*)m.ptrs;
+    a.lds = m.lds;
+    a.ld = m.ld;
+    return a;
+}
+
+// Element (row, col) of right-hand side col: side_left, B is n x nrhs and the
+// vector is column col; right side, B is nrhs x n and the vector is row col.
+template <typename T>
+KB_DEV T* rhs_elem(T* Bp, int ldb, int side_left, int row, int col)
+{
+    return side_left ? Bp + row + (long)col * ldb : Bp + col + (long)row * ldb;
+}
 
 // ---------------------------------------------------------------------------
 // Global-memory accessors.  Batch operands always live in device (global)
@@ -59,6 +80,14 @@ KB_DEV void wave_global_sync()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Make this wavefront's LDS writes visible to its other lanes.
+KB_DEV void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // ---------------------------------------------------------------------------

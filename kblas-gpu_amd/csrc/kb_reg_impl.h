@@ -121,13 +121,6 @@ constexpr int reg_mpb()
     return m;
 }
 
-template <typename T>
-KB_DEV T* rhs_elem(T* Bp, int ldb, int side_left, int row, int col)
-{
-    // side_left: B is (n x nrhs), vector col is column col.
-    // right side: B is (nrhs x n), vector col is row col.
-    return side_left ? Bp + row + (long)col * ldb : Bp + col + (long)row * ldb;
-}
 
 // Backward sweep reading L(k,r) straight from the factor in global memory
 // (used where no LDS copy is kept: lanes read a row of L, which the same
@@ -397,17 +390,6 @@ k_reg_sposv(int n, int nrhs, int side_left, BatchArg<double> A, BatchArg<double>
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <typename T>
-static BatchArg<T> to_arg(kb_mat_t m)
-{
-    BatchArg<T> a;
-    a.base = (T*)m.base;
-    a.stride = m.stride;
-    a.ptrs = (T* const*)m.ptrs;
-    a.lds = m.lds;
-    a.ld = m.ld;
-    return a;
-}
 
 template <typename T, int NB, int MODE, int RC>
 static int launch_chol(int n, int nrhs, int side_left, kb_mat_t A, kb_mat_t B, int* info,

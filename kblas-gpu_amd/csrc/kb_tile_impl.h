@@ -51,12 +51,6 @@ struct TileLds {
     T invd[4][64];   // 1/L(16K+i,16K+i) per lane
 };
 
-KB_DEV void wave_lds_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // ---------------------------------------------------------------------------
 // factorization
