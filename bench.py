@@ -110,6 +110,21 @@ def gen_vbatch(torch, cfg, seed):
     return ns, ns_h, offA, offB, Abuf, Bbuf
 
 
+def measured_traffic(config):
+    """HBM bytes per launch of the workload's library kernel, from the committed
+    rocprofv3 counter summary profiles/traffic_<config>.md (FETCH_SIZE +
+    WRITE_SIZE, scaled by the same counters read for a plain copy of known
+    size).  None when no summary exists for this workload."""
+    path = os.path.join(ROOT, "profiles", "traffic_%s.md" % config)
+    if not os.path.exists(path):
+        return None
+    for ln in open(path):
+        if ln.startswith("traffic_bytes_per_launch:"):
+            return {"bytes_per_launch": float(ln.split(":", 1)[1].split()[0]),
+                    "source": "profiles/traffic_%s.md" % config}
+    return None
+
+
 def time_steps(torch, call, K, W):
     for i in range(W):
         call(i)
@@ -208,6 +223,9 @@ def main():
     ap.add_argument("--config", default="c4", choices=sorted(CONFIGS))
     ap.add_argument("--cpu-sample", type=int, default=0, help="matrices in the CPU-baseline sample (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="after the timed region, run a plain device copy of MIB MiB three times "
+                         "(the reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the batched SPD path shards by matrix with "
@@ -360,7 +378,19 @@ def main():
                      "bytes_per_unit": per_unit_bytes},
     }
     line.update(out)
+    tr = measured_traffic(args.config)
+    if tr is not None:
+        line["roofline"]["traffic"] = tr["bytes_per_launch"]
+        line["roofline"]["traffic_source"] = tr["source"]
     torch.cuda.synchronize()
+    if args.copy_calibration > 0:
+        nb = args.copy_calibration << 20
+        src = torch.ones(nb // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     if not args.no_cpu and cfg["op"] == "vposv":
         line["cpu_baseline"] = cpu_baseline_vbatch(cfg, vb, K, unit)
     elif not args.no_cpu:

@@ -57,6 +57,18 @@ KB_DEV void blk_syrk_update(const T* L21, T* C, int lda, int m2, int kb, BlkSmem
                 S.sA[r * (NBK + 1) + p] = (gi < m2 && p < kb) ? L21[gi + (long)p * lda] : T(0);
                 S.sB[r * (NBK + 1) + p] = (gj < m2 && p < kb) ? L21[gj + (long)p * lda] : T(0);
             }
+            // the tile of C, fetched before the rank-kb product so that its
+            // latency overlaps the FMAs
+            T cv[4][4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const int gi = I * TL + ty + 16 * a, gj = J * TL + tx + 16 * b;
+                    const bool in = gi < m2 && gj <= gi;
+                    const T v = gld(C + (in ? gi + (long)gj * lda : 0));
+                    cv[a][b] = in ? v : T(0);
+                }
             __syncthreads();
             T acc[4][4];
 #pragma unroll
@@ -79,10 +91,7 @@ KB_DEV void blk_syrk_update(const T* L21, T* C, int lda, int m2, int kb, BlkSmem
 #pragma unroll
                 for (int b = 0; b < 4; ++b) {
                     const int gi = I * TL + ty + 16 * a, gj = J * TL + tx + 16 * b;
-                    if (gi < m2 && gj <= gi) {
-                        T* c = C + gi + (long)gj * lda;
-                        *c = *c - acc[a][b];
-                    }
+                    if (gi < m2 && gj <= gi) gst(C + gi + (long)gj * lda, cv[a][b] - acc[a][b]);
                 }
             }
             __syncthreads();

@@ -200,6 +200,68 @@ KB_DEV int group_all(int pred)
 // twice r += r/2 (1 - x r^2), ~1 ulp); FP32 v_rsq_f32 is already ~1 ulp.
 // Cholesky only needs the pivot's square root to a few ulp for backward
 // stability; LAPACK's dpotf2 likewise scales the column by the reciprocal.
+// ---------------------------------------------------------------------------
+// fused row reads
+// ---------------------------------------------------------------------------
+// acc += src(lane K of this 16-lane row) * mul.  The s_nop supplies the two
+// wait states a DPP read needs after a VALU write of its source register.
+template <int K>
+KB_DEV void fmac_row(double& acc, double src, double mul)
+{
+    asm("s_nop 1\n\tv_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+        : "+v"(acc)
+        : "v"(src), "v"(mul), "n"(K));
+}
+template <int K>
+KB_DEV void fmac_row(float& acc, float src, float mul)
+{
+    asm("s_nop 1\n\tv_fmac_f32_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+        : "+v"(acc)
+        : "v"(src), "v"(mul), "n"(K));
+}
+
+#define KB_FR(i) case i: fmac_row<i>(acc, src, mul); return;
+// k becomes a constant once the caller's loops are unrolled; the switch folds.
+template <typename T>
+KB_DEV void fmac_rowk(T& acc, T src, T mul, int k)
+{
+    switch (k & 15) { KB_REP16(KB_FR, 0) }
+}
+#undef KB_FR
+
+// value of lane K of this 16-lane row (compiler-visible DPP move, undefined old value)
+#define KB_MR(i) case i: return __builtin_amdgcn_mov_dpp(x, 0x150 + i, 0xF, 0xF, false);
+template <typename T>
+KB_DEV T row_bcast(T x, int k)
+{
+    switch (k & 15) { KB_REP16(KB_MR, 0) default: return x; }
+}
+#undef KB_MR
+
+// acc -= src(lane K of this 16-lane row) * mul
+template <int K>
+KB_DEV void fnmac_row(double& acc, double src, double mul)
+{
+    asm("s_nop 1\n\tv_fmac_f64_dpp %0, -%1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+        : "+v"(acc)
+        : "v"(src), "v"(mul), "n"(K));
+}
+template <int K>
+KB_DEV void fnmac_row(float& acc, float src, float mul)
+{
+    asm("s_nop 1\n\tv_fmac_f32_dpp %0, -%1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+        : "+v"(acc)
+        : "v"(src), "v"(mul), "n"(K));
+}
+
+#define KB_FNR(i) case i: fnmac_row<i>(acc, src, mul); return;
+template <typename T>
+KB_DEV void fnmac_rowk(T& acc, T src, T mul, int k)
+{
+    switch (k & 15) { KB_REP16(KB_FNR, 0) }
+}
+#undef KB_FNR
+
 template <typename T> KB_DEV T kb_rsqrt(T x);
 template <> KB_DEV double kb_rsqrt<double>(double x)
 {

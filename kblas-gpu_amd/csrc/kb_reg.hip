@@ -14,10 +14,10 @@ extern "C" int kb_tile_chol_s(int, int, int, int, kb_mat_t, kb_mat_t, int*, int,
 extern "C" int kb_tile_sposv(int, int, int, kb_mat_t, kb_mat_t, kb_mat_t, int*, int*, int, hipStream_t);
 
 // Four matrices per wavefront, one per 16-lane DPP row: n <= 16 and n <= 32
-// whole in VGPRs, 32 < n <= 48 in 16-column blocks; 48 < n <= 64: the
-// MFMA-tiled kernel, one matrix per wavefront (measured crossover: the
-// blocked DPP-row kernel re-reads its finished columns, which costs more
-// than the tiled kernel's idle panel lanes once n passes 48).
+// whole in VGPRs; POTRF with 32 < n <= 48 in 16-column blocks.  Otherwise
+// (32 < n <= 64) the MFMA-tiled kernel, one matrix per wavefront, which keeps
+// the factor in registers for the solves (measured: the blocked DPP-row kernel
+// re-reads its finished columns and loses once a solve follows or n > 48).
 extern "C" int kb_launch_reg_chol(char prec, int mode, int n, int nrhs, int side_left,
                                   kb_mat_t A, kb_mat_t B, int* info, int batch, hipStream_t s)
 {
@@ -25,12 +25,12 @@ extern "C" int kb_launch_reg_chol(char prec, int mode, int n, int nrhs, int side
     if (prec == 'd') {
         if (n <= 16) return kb_rg_chol_d16(mode, n, nrhs, side_left, A, B, info, batch, s);
         if (n <= 32) return kb_rg_chol_d32(mode, n, nrhs, side_left, A, B, info, batch, s);
-        if (n <= 48) return kb_rg_chol_d64(mode, n, nrhs, side_left, A, B, info, batch, s);
+        if (n <= 48 && mode == KB_MODE_POTRF) return kb_rg_chol_d64(mode, n, nrhs, side_left, A, B, info, batch, s);
         return kb_tile_chol_d(mode, n, nrhs, side_left, A, B, info, batch, s);
     }
     if (n <= 16) return kb_rg_chol_s16(mode, n, nrhs, side_left, A, B, info, batch, s);
     if (n <= 32) return kb_rg_chol_s32(mode, n, nrhs, side_left, A, B, info, batch, s);
-    if (n <= 48) return kb_rg_chol_s64(mode, n, nrhs, side_left, A, B, info, batch, s);
+    if (n <= 48 && mode == KB_MODE_POTRF) return kb_rg_chol_s64(mode, n, nrhs, side_left, A, B, info, batch, s);
     return kb_tile_chol_s(mode, n, nrhs, side_left, A, B, info, batch, s);
 }
 

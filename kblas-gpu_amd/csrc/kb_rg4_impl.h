@@ -11,9 +11,11 @@
 //                          (left-looking; L(:, P) re-read from the output,
 //                           which this wavefront has just written)
 //             factor block J right-looking, column by column, and write it.
-// The re-reads cost memory traffic at n = 64 (the per-XCD L2 cannot hold the
-// batch in flight), so this kernel serves 32 < n <= 48 and the MFMA-tiled
-// kernel (kb_tile_impl.h) the rest; the dispatch table is measured.
+// The re-reads cost memory traffic (the per-XCD L2 cannot hold the batch in
+// flight), so this kernel serves POTRF with 32 < n <= 48 and the MFMA-tiled
+// kernel (kb_tile_impl.h), which keeps the factor in registers, the rest; the
+// dispatch table is measured.  The solve routines below are kept with the
+// kernel for its POSV/POTRS modes.
 // That is the reference's recursive split POTRF -> TRSM -> SYRK -> POTRF
 // (src/batch_triangular/Xpotrf_batch_drivers.cuh:91-131) carried out as one
 // pass over 16-column blocks inside one kernel, with the SYRK/GEMM block
@@ -25,30 +27,6 @@
 // row k comes from lane k by DPP.
 #pragma once
 #include "kb_rg_impl.h"
-
-// acc -= src(lane K of this 16-lane row) * mul
-template <int K>
-KB_DEV void fnmac_row(double& acc, double src, double mul)
-{
-    asm("s_nop 1\n\tv_fmac_f64_dpp %0, -%1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
-        : "+v"(acc)
-        : "v"(src), "v"(mul), "n"(K));
-}
-template <int K>
-KB_DEV void fnmac_row(float& acc, float src, float mul)
-{
-    asm("s_nop 1\n\tv_fmac_f32_dpp %0, -%1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
-        : "+v"(acc)
-        : "v"(src), "v"(mul), "n"(K));
-}
-
-#define KB_FNR(i) case i: fnmac_row<i>(acc, src, mul); return;
-template <typename T>
-KB_DEV void fnmac_rowk(T& acc, T src, T mul, int k)
-{
-    switch (k & 15) { KB_REP16(KB_FNR, 0) }
-}
-#undef KB_FNR
 
 // Element (i, k) of the lower triangle of a column-major matrix, branch-free:
 // other positions (upper part, or beyond n) read A(0,0) and return `other`.
@@ -313,15 +291,11 @@ static int rg4_launch(int n, int nrhs, int side_left, kb_mat_t A, kb_mat_t B, in
     return (int)hipGetLastError();
 }
 
+// POTRF only: the solves run in the MFMA-tiled kernel (see kb_reg.hip)
 template <typename T>
 static int rg4_dispatch(int mode, int n, int nrhs, int side_left, kb_mat_t A, kb_mat_t B, int* info, int batch,
                         hipStream_t s)
 {
-    if (mode == KB_MODE_POTRF) return rg4_launch<T, KB_MODE_POTRF, 1>(n, nrhs, side_left, A, B, info, batch, s);
-    if (mode == KB_MODE_POSV) {
-        if (nrhs <= 1) return rg4_launch<T, KB_MODE_POSV, 1>(n, nrhs, side_left, A, B, info, batch, s);
-        return rg4_launch<T, KB_MODE_POSV, 4>(n, nrhs, side_left, A, B, info, batch, s);
-    }
-    if (nrhs <= 1) return rg4_launch<T, KB_MODE_POTRS, 1>(n, nrhs, side_left, A, B, info, batch, s);
-    return rg4_launch<T, KB_MODE_POTRS, 4>(n, nrhs, side_left, A, B, info, batch, s);
+    if (mode != KB_MODE_POTRF) return (int)hipErrorInvalidValue;
+    return rg4_launch<T, KB_MODE_POTRF, 1>(n, nrhs, side_left, A, B, info, batch, s);
 }

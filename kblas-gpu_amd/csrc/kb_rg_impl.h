@@ -29,44 +29,6 @@
 #include "kb_device.h"
 #include "kb_launch.h"
 
-// ---------------------------------------------------------------------------
-// fused row reads
-// ---------------------------------------------------------------------------
-// acc += src(lane K of this 16-lane row) * mul.  The s_nop supplies the two
-// wait states a DPP read needs after a VALU write of its source register.
-template <int K>
-KB_DEV void fmac_row(double& acc, double src, double mul)
-{
-    asm("s_nop 1\n\tv_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
-        : "+v"(acc)
-        : "v"(src), "v"(mul), "n"(K));
-}
-template <int K>
-KB_DEV void fmac_row(float& acc, float src, float mul)
-{
-    asm("s_nop 1\n\tv_fmac_f32_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
-        : "+v"(acc)
-        : "v"(src), "v"(mul), "n"(K));
-}
-
-#define KB_FR(i) case i: fmac_row<i>(acc, src, mul); return;
-// k becomes a constant once the caller's loops are unrolled; the switch folds.
-template <typename T>
-KB_DEV void fmac_rowk(T& acc, T src, T mul, int k)
-{
-    switch (k & 15) { KB_REP16(KB_FR, 0) }
-}
-#undef KB_FR
-
-// value of lane K of this 16-lane row (compiler-visible DPP move, undefined old value)
-#define KB_MR(i) case i: return __builtin_amdgcn_mov_dpp(x, 0x150 + i, 0xF, 0xF, false);
-template <typename T>
-KB_DEV T row_bcast(T x, int k)
-{
-    switch (k & 15) { KB_REP16(KB_MR, 0) default: return x; }
-}
-#undef KB_MR
-
 // a(s, k) is kept only for columns up to the slot's last row
 constexpr KB_DEV bool rg_live(int s, int k) { return k < 16 * (s + 1); }
 // LDS copy of -Lu: row k holds the columns of its slot, 16 (k / 16 + 1), at an

@@ -126,10 +126,13 @@ KB_DEV int tile_potrf(typename TileV<T>::v4 (&t)[10], T (&invd)[4], int n, T (&p
                     }
 #pragma unroll
                     for (int cp = c + 1; cp < 4; ++cp) {
-                        const T l = bcastk<16>(t[tix(J, J)][c], 4 * gp + cp);
 #pragma unroll
-                        for (int I = J; I < 4; ++I)
-                            t[tix(I, J)][cp] = kb_fma<T>(-t[tix(I, J)][c], l, t[tix(I, J)][cp]);
+                        for (int I = J; I < 4; ++I) {
+                            // t(I,J)[cp] -= t(I,J)[c] * t(J,J)[c] of lane 4 gp + cp
+                            T acc = t[tix(I, J)][cp];
+                            fnmac_rowk<T>(acc, t[tix(J, J)][c], t[tix(I, J)][c], 4 * gp + cp);
+                            t[tix(I, J)][cp] = acc;
+                        }
                     }
                 }
             }
@@ -216,6 +219,46 @@ struct GlobalTiles {
     }
 };
 
+// Factor tiles taken from the wavefront's registers (the factorization's
+// output, or a tile_load of L), staged through LDS one tile at a time.
+template <typename T>
+struct RegTiles {
+    const typename TileV<T>::v4* t;
+    TileLds<T>* S;
+    template <bool ROW, bool STRICT>
+    KB_DEV void get(int I, int J, T (&v)[16], int lane, T invk) const
+    {
+        const int i = lane & 15, g = lane >> 4;
+        typename TileV<T>::v4 x;
+        // uniform switch: the register tile is picked by a scalar branch (the
+        // empty asm keeps the compiler from turning it into an indexed load
+        // from a stack copy of t)
+        switch (tix(I, J)) {
+        case 0: x = t[0]; asm volatile("" : "+v"(x)); break;
+        case 1: x = t[1]; asm volatile("" : "+v"(x)); break;
+        case 2: x = t[2]; asm volatile("" : "+v"(x)); break;
+        case 3: x = t[3]; asm volatile("" : "+v"(x)); break;
+        case 4: x = t[4]; asm volatile("" : "+v"(x)); break;
+        case 5: x = t[5]; asm volatile("" : "+v"(x)); break;
+        case 6: x = t[6]; asm volatile("" : "+v"(x)); break;
+        case 7: x = t[7]; asm volatile("" : "+v"(x)); break;
+        case 8: x = t[8]; asm volatile("" : "+v"(x)); break;
+        default: x = t[9]; asm volatile("" : "+v"(x)); break;
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) S->tile[i][4 * g + r] = x[r];
+        if (STRICT && g == 0) S->inv[i] = invk;
+        wave_lds_sync();
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            T y = ROW ? S->tile[i][j] : S->tile[j][i];
+            if (STRICT) y = (ROW ? (j < i) : (j > i)) ? y * S->inv[j] : T(0);
+            v[j] = y;
+        }
+        wave_lds_sync();
+    }
+};
+
 // (L L^T) X = B for the group's four right-hand sides.  Right-hand side c of
 // the group sits in DPP row c: y[K][16c + i] = row 16K + i.  The tile loops
 // are runtime loops over LDS-resident blocks so that register use stays that
@@ -233,7 +276,7 @@ KB_DEV void tile_solve(const Src& src, T (&y)[4][64], const T (&invd)[4][64], in
         T yk = y[K][lane];
         src.template get<true, true>(K, K, v, lane, ik);
 #pragma unroll
-        for (int j = 0; j < 16; ++j) yk = kb_fma<T>(-v[j], bcastk<16>(yk, j), yk);
+        for (int j = 0; j < 16; ++j) fnmac_rowk<T>(yk, yk, v[j], j);
         yk *= ik;
         y[K][lane] = yk;
 #pragma nounroll
@@ -241,7 +284,7 @@ KB_DEV void tile_solve(const Src& src, T (&y)[4][64], const T (&invd)[4][64], in
             T yi = y[I][lane];
             src.template get<true, false>(I, K, v, lane, T(1));
 #pragma unroll
-            for (int j = 0; j < 16; ++j) yi = kb_fma<T>(-v[j], bcastk<16>(yk, j), yi);
+            for (int j = 0; j < 16; ++j) fnmac_rowk<T>(yi, yk, v[j], j);
             y[I][lane] = yi;
         }
     }
@@ -255,11 +298,11 @@ KB_DEV void tile_solve(const Src& src, T (&y)[4][64], const T (&invd)[4][64], in
             const T xi = y[I][lane];
             src.template get<false, false>(I, K, v, lane, T(1));
 #pragma unroll
-            for (int j = 0; j < 16; ++j) yk = kb_fma<T>(-v[j], bcastk<16>(xi, j), yk);
+            for (int j = 0; j < 16; ++j) fnmac_rowk<T>(yk, xi, v[j], j);
         }
         src.template get<false, true>(K, K, v, lane, ik);
 #pragma unroll
-        for (int j = 15; j >= 0; --j) yk = kb_fma<T>(-v[j], bcastk<16>(yk, j), yk);
+        for (int j = 15; j >= 0; --j) fnmac_rowk<T>(yk, yk, v[j], j);
         yk *= ik;
         y[K][lane] = yk;
     }
@@ -272,15 +315,15 @@ KB_DEV void tile_chol_one(int n, int nrhs, int side_left, T* Ap, int lda, T* Bp,
     T invd[4] = {T(1), T(1), T(1), T(1)};
     int inf = 0;
     const int i = lane & 15;
+    typename TileV<T>::v4 t[10];
     if constexpr (MODE != KB_MODE_POTRS) {
-        typename TileV<T>::v4 t[10];
         tile_load<T>(t, Ap, lda, n, lane);
         inf = tile_potrf<T>(t, invd, n, S.pan, lane);
         tile_store<T>(t, Ap, lda, n, lane);
         if (lane == 0 && info_out) *info_out = inf;
-        // the solve re-reads L from memory: make this wavefront's stores visible
-        if (MODE != KB_MODE_POTRF && inf == 0) wave_global_sync();
     } else {
+        // the factor, all ten tiles in one batch of loads
+        tile_load<T>(t, Ap, lda, n, lane);
 #pragma unroll
         for (int K = 0; K < 4; ++K) {
             const int row = 16 * K + i;
@@ -302,7 +345,7 @@ KB_DEV void tile_chol_one(int n, int nrhs, int side_left, T* Ap, int lda, T* Bp,
                 S.y[K][lane] = in ? ld : T(0);
             }
             wave_lds_sync();
-            GlobalTiles<T> src{Ap, lda, n, &S};
+            RegTiles<T> src{t, &S};
             tile_solve<T>(src, S.y, S.invd, lane);
             wave_lds_sync();
 #pragma unroll
@@ -316,7 +359,7 @@ KB_DEV void tile_chol_one(int n, int nrhs, int side_left, T* Ap, int lda, T* Bp,
 
 // 4 wavefronts (4 matrices) per 256-thread workgroup
 template <typename T, int MODE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8)))
 k_tile_chol(int n, int nrhs, int side_left, BatchArg<T> A, BatchArg<T> B, int* info, int batch)
 {
     __shared__ TileLds<T> S[4];

@@ -21,6 +21,7 @@ struct MixLds {
             double tile[16][17];      // FP64 A-tile staging for the residual
             double x[4][64];          // solution blocks
             double r[4][64];          // residual blocks
+            double b[4][64];          // right-hand-side blocks
             double rs[64];            // row sums of |A|
         } m;
         TileLds<double> d;            // FP64 fallback
@@ -137,6 +138,7 @@ k_tile_sposv(int n, int nrhs, int side_left, BatchArg<double> A, BatchArg<double
             const double ld = gld(rhs_elem(Bp, ldb, side_left, in ? row : 0, in ? c0 + c : 0));
             S.m.x[K][lane] = 0.0;
             S.m.r[K][lane] = in ? ld : 0.0;
+            S.m.b[K][lane] = in ? ld : 0.0;
         }
         wave_lds_sync();
         bool conv = false;
@@ -152,11 +154,8 @@ k_tile_sposv(int n, int nrhs, int side_left, BatchArg<double> A, BatchArg<double
             // x += z ; r = b - A x
 #pragma unroll
             for (int K = 0; K < 4; ++K) {
-                const int row = 16 * K + i;
-                const bool in = act && row < n;
-                const double ld = gld(rhs_elem(Bp, ldb, side_left, in ? row : 0, in ? c0 + c : 0));
                 S.m.x[K][lane] += (double)S.m.z[K][lane];
-                S.m.r[K][lane] = in ? ld : 0.0;
+                S.m.r[K][lane] = S.m.b[K][lane];
             }
             wave_lds_sync();
 #pragma unroll
@@ -175,7 +174,7 @@ k_tile_sposv(int n, int nrhs, int side_left, BatchArg<double> A, BatchArg<double
                         for (int j = 0; j < 16; ++j) v[j] = (I == J && j > i) ? S.m.tile[j][i] : S.m.tile[i][j];
                         double ri = S.m.r[I][ln];
 #pragma unroll
-                        for (int j = 0; j < 16; ++j) ri = __builtin_fma(-v[j], bcastk<16>(xj, j), ri);
+                        for (int j = 0; j < 16; ++j) fnmac_rowk<double>(ri, xj, v[j], j);
                         S.m.r[I][ln] = ri;
                     }
                     if (I != J) {   // r_J -= A_IJ^T x_I   (column i of the tile)
@@ -184,7 +183,7 @@ k_tile_sposv(int n, int nrhs, int side_left, BatchArg<double> A, BatchArg<double
                         for (int j = 0; j < 16; ++j) v[j] = S.m.tile[j][i];
                         double rj = S.m.r[J][ln];
 #pragma unroll
-                        for (int j = 0; j < 16; ++j) rj = __builtin_fma(-v[j], bcastk<16>(xi, j), rj);
+                        for (int j = 0; j < 16; ++j) fnmac_rowk<double>(rj, xi, v[j], j);
                         S.m.r[J][ln] = rj;
                     }
                     wave_lds_sync();
