@@ -75,3 +75,19 @@ def test_sposv_fallback_on_fp32_failure(h, n):
             assert it[i] < 0
             L = po.col(Aout, i)
             assert po.potrf_berr(po.col(A, i), L) <= 4 * n * kt.EPS64   # A holds L after fallback
+
+
+@pytest.mark.parametrize("n", [65, 130])
+@pytest.mark.parametrize("side", ["R", "L"])
+def test_sposv_large_order_full_precision(h, n, side):
+    # above the register range the mixed entry takes dsposv's full-precision
+    # path: ITER = -1, A := L, X := the FP64 solution
+    batch = 5
+    A, kap = po.gen_spd_cond_batch(12, n, batch, 2.0, 5.0)
+    B = po.gen_rhs_batch(13, 3, n, batch) if side == "R" else po.gen_rhs_batch(13, n, 3, batch)
+    X, info, it, Aout = run_sposv(h, side, A, B, 3, n)
+    assert np.all(info == 0)
+    assert np.all(it == -1)
+    for i in range(batch):
+        assert po.solve_berr(side, po.col(A, i), po.col(X, i), po.col(B, i)) <= 1e-13
+        assert po.potrf_berr(po.col(A, i), po.col(Aout, i)) <= 4 * n * kt.EPS64
