@@ -40,6 +40,8 @@ CONFIGS = {
     "c3": dict(op="vposv", n=256, nmin=8, batch=20000, nrhs=4, gen="shift",
                workload="dposv_vbatch n~U{8..256} x 20000, 4 RHS, pointer arrays, one persistent launch "
                         "over a device size-sorted queue, each step a captured HIP graph"),
+    "c5": dict(op="vsposv", n=512, nmin=128, batch=2000, nrhs=64, gen="cond", lkmin=2.0, lkmax=4.0,
+               workload="dsposv_vbatch n~U{128..512} x 2000, 64 RHS, cond in [1e2,1e4], non-uniform mixed POSV"),
     "c4": dict(op="sposv", n=64, batch=50000, nrhs=4, gen="cond",
                workload="dsposv_batch_strided n=64 x 50000, 4 RHS, cond in [1e2,1e6], mixed FP32+IR vs FP64"),
 }
@@ -102,9 +104,21 @@ def gen_vbatch(torch, cfg, seed):
     for n in np.unique(ns_h):
         idx = np.nonzero(ns_h == n)[0]
         k = len(idx)
-        Bm = torch.rand((k, n, n), generator=g, device="cuda", dtype=torch.float64) * 2 - 1
-        A = torch.bmm(Bm, Bm.transpose(1, 2).contiguous())
-        A = 0.5 * (A + A.transpose(1, 2)) + n * torch.eye(int(n), device="cuda", dtype=torch.float64)
+        if cfg["gen"] == "cond":
+            # A = Q diag(s) Q^T, Q from the QR factorization of a Gaussian matrix,
+            # s log-spaced with endpoints 1 and 1/kappa, log10(kappa) ~ U[lkmin, lkmax]
+            G = torch.randn((k, n, n), generator=g, device="cuda", dtype=torch.float64)
+            Q, _ = torch.linalg.qr(G)
+            lk = cfg["lkmin"] + (cfg["lkmax"] - cfg["lkmin"]) * torch.rand((k, 1), generator=g, device="cuda",
+                                                                           dtype=torch.float64)
+            t = torch.linspace(0.0, 1.0, int(n), device="cuda", dtype=torch.float64)[None, :]
+            sv = torch.pow(10.0, -lk * t)
+            A = torch.bmm(Q * sv[:, None, :], Q.transpose(1, 2).contiguous())
+            A = 0.5 * (A + A.transpose(1, 2))
+        else:
+            Bm = torch.rand((k, n, n), generator=g, device="cuda", dtype=torch.float64) * 2 - 1
+            A = torch.bmm(Bm, Bm.transpose(1, 2).contiguous())
+            A = 0.5 * (A + A.transpose(1, 2)) + n * torch.eye(int(n), device="cuda", dtype=torch.float64)
         pos = torch.from_numpy((offA[idx][:, None] + np.arange(n * n)[None, :]).reshape(-1)).cuda()
         Abuf.index_copy_(0, pos, A.reshape(-1))
     return ns, ns_h, offA, offB, Abuf, Bbuf
@@ -196,7 +210,7 @@ def cpu_baseline_vbatch(cfg, vb, K, unit):
     L.orc_set_threads(cores)
     ns, ns_h, offA, offB, Abuf, Bbuf = vb
     nrhs = cfg["nrhs"]
-    bs = min(2000, cfg["batch"])
+    bs = min(2000 if cfg["op"] == "vposv" else 40, cfg["batch"])
     A = Abuf[: int(offA[bs])].cpu().numpy()
     B = Bbuf[: int(offB[bs])].cpu().numpy()
     best = None
@@ -240,7 +254,7 @@ def main():
     K, W = args.steps, args.warmup
     print("bench: config %s (%s), generating inputs" % (args.config, cfg["workload"]), file=sys.stderr, flush=True)
     vb = None
-    if cfg["op"] == "vposv":
+    if cfg["op"] in ("vposv", "vsposv"):
         vb = gen_vbatch(torch, cfg, 1234 + int(args.config[1]))
         A0, B0, kappa = None, None, None
     else:
@@ -253,7 +267,45 @@ def main():
     NA = n * n
     out = {}
 
-    if cfg["op"] == "vposv":
+    if cfg["op"] == "vsposv":
+        # non-uniform mixed POSV: A, B read-only, X written; every step restores A
+        # (the full-precision path overwrites it with L), outside the timed region
+        ns, ns_h, offA, offB, Abuf0, Bbuf0 = vb
+        nrhs_arr = torch.full((bc,), nrhs, dtype=torch.int32, device="cuda")
+        lib.kblas_posv_vbatch_wsquery(h.h, n, bc)
+        assert lib.kblasAllocateWorkspace(h.h) == pykblas.KBLAS_Success
+        offA_d = torch.from_numpy(offA[:-1] * 8).cuda()
+        offB_d = torch.from_numpy(offB[:-1] * 8).cuda()
+        As = [Abuf0.clone() for _ in range(K + W)]
+        Xbuf = torch.empty_like(Bbuf0)
+        pA = [offA_d + a.data_ptr() for a in As]
+        pB = offB_d + Bbuf0.data_ptr()
+        pX = offB_d + Xbuf.data_ptr()
+        torch.cuda.synchronize()
+
+        def call(i):
+            st = lib.kblasDsposv_vbatch(h.h, b"R", b"L", nrhs_arr.data_ptr(), ns.data_ptr(), nrhs, n,
+                                        pA[i].data_ptr(), ns.data_ptr(), pB.data_ptr(), nrhs_arr.data_ptr(),
+                                        pX.data_ptr(), nrhs_arr.data_ptr(), bc, info.data_ptr(), iters.data_ptr())
+            assert st == pykblas.KBLAS_Success, st
+        t, wall = time_steps(torch, call, K, W)
+        nn = ns_h.astype(np.float64)
+        per_unit_bytes = float(np.mean(4 * nn * (nn + 1) + 16 * nn * nrhs))
+        flops = float(np.sum(nn ** 3 / 3 + nn ** 2 / 2 + nn / 6 + 2 * nrhs * nn ** 2))
+        value = bc * K / t
+        metric, unit = "solves/s (non-uniform mixed DSPOSV n~U{128..512}, 64 RHS)", "solves/s"
+        dtype = "f32+f64"
+        X_sample = None
+        out["gflops"] = flops * K / t / 1e9
+        out["fp64_peak_frac"] = out["gflops"] / 78600.0
+        it_h = iters.cpu().numpy()
+        hist = {}
+        for v in it_h.tolist():
+            hist[str(v)] = hist.get(str(v), 0) + 1
+        out["iter_hist"] = hist
+        out["info_nonzero"] = int((info != 0).sum().item())
+        del As
+    elif cfg["op"] == "vposv":
         ns, ns_h, offA, offB, Abuf0, Bbuf0 = vb
         nrhs_arr = torch.full((bc,), nrhs, dtype=torch.int32, device="cuda")
         lib.kblas_posv_vbatch_wsquery(h.h, n, bc)
@@ -391,7 +443,7 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    if not args.no_cpu and cfg["op"] == "vposv":
+    if not args.no_cpu and cfg["op"] in ("vposv", "vsposv"):
         line["cpu_baseline"] = cpu_baseline_vbatch(cfg, vb, K, unit)
     elif not args.no_cpu:
         cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)

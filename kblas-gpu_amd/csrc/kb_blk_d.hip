@@ -9,28 +9,18 @@ extern "C" int kb_blk_chol_d(int mode, int n, int nrhs, int side_left, kb_mat_t 
 
 // Mixed-precision POSV entry for orders above the register range: LAPACK
 // dsposv's full-precision path (ITER = -1, "fell back to full precision for
-// implementation reasons"): X := B, A := L in FP64, X := (L L^T)^-1 X.
-__global__ void __launch_bounds__(KB_BLK_WG)
-k_blk_sposv64(int n, int nrhs, int side_left, BatchArg<double> A, BatchArg<double> B, BatchArg<double> X,
-              int* info, int* iter, int batch)
+// implementation reasons"): X := B (and ITER := -1) here, then the FP64 POSV
+// on (A, X), so A := L and X := (L L^T)^-1 B.
+__global__ void __launch_bounds__(256)
+k_sposv_full_prep(int n, int nrhs, int side_left, BatchArg<double> B, BatchArg<double> X, int* iter, int batch)
 {
-    __shared__ BlkSmem<double> S;
     for (long mat = blockIdx.x; mat < batch; mat += gridDim.x) {
-        double* Ap = A.get(mat);
-        const int lda = A.ldim(mat);
-        double* Bp = B.get(mat);
+        const double* Bp = B.get(mat);
         double* Xp = X.get(mat);
         const int ldb = B.ldim(mat), ldx = X.ldim(mat);
-        for (int e = threadIdx.x; e < n * nrhs; e += KB_BLK_WG)
-            rhs_at(Xp, ldx, side_left, e % n, e / n) = rhs_at(Bp, ldb, side_left, e % n, e / n);
-        __syncthreads();
-        const int inf = blk_potrf<double>(Ap, lda, n, S);
-        if (threadIdx.x == 0) {
-            if (info) info[mat] = inf;
-            if (iter) iter[mat] = -1;
-        }
-        if (inf == 0) blk_solve<double>(Ap, lda, n, Xp, ldx, side_left, nrhs, S);
-        __syncthreads();
+        for (int e = threadIdx.x; e < n * nrhs; e += blockDim.x)
+            rhs_at(Xp, ldx, side_left, e % n, e / n) = rhs_at(const_cast<double*>(Bp), ldb, side_left, e % n, e / n);
+        if (threadIdx.x == 0 && iter) iter[mat] = -1;
     }
 }
 
@@ -38,7 +28,9 @@ extern "C" int kb_launch_blk_sposv(int n, int nrhs, int side_left, kb_mat_t A, k
                                    int* iter, int batch, hipStream_t s)
 {
     const int grid = batch < 2048 ? batch : 2048;
-    hipLaunchKernelGGL(k_blk_sposv64, dim3(grid), dim3(KB_BLK_WG), 0, s, n, nrhs, side_left, to_arg<double>(A),
-                       to_arg<double>(B), to_arg<double>(X), info, iter, batch);
-    return (int)hipGetLastError();
+    hipLaunchKernelGGL(k_sposv_full_prep, dim3(grid), dim3(256), 0, s, n, nrhs, side_left, to_arg<double>(B),
+                       to_arg<double>(X), iter, batch);
+    const int e = (int)hipGetLastError();
+    if (e) return e;
+    return launch_blk<double>(KB_MODE_POSV, n, nrhs, side_left, A, X, info, batch, s);
 }

@@ -40,59 +40,63 @@ KB_DEV T& rhs_at(T* B, int ldb, int side_left, int i, int c)
     return side_left ? B[i + (long)c * ldb] : B[c + (long)i * ldb];
 }
 
-// C (m2 x m2, lower) -= L21 L21^T, L21 is m2 x kb (ld lda).
+// C (m2 x m2, lower) -= L21 L21^T, L21 is m2 x kb (ld lda), on the matrix
+// cores.  C is walked in 64 x 64 tiles; the 64-row slices of L21 are staged in
+// LDS (slice I once per row of tiles).  Wave w computes rows 16w..16w+15 of a
+// tile as four 16 x 16 products D' = L21_J(16 x kb) L21_I(16 x kb)^T, i.e.
+// the transpose of the C block, whose accumulator layout puts consecutive C
+// rows in consecutive lanes (coalesced column-major read-modify-write):
+//   FP64: lane (g, i), register r -> C(16w + i, 16q + 4r + g)
+//   FP32: lane (g, i), register r -> C(16w + i, 16q + 4g + r)
 template <typename T>
 KB_DEV void blk_syrk_update(const T* L21, T* C, int lda, int m2, int kb, BlkSmem<T>& S)
 {
     constexpr int NBK = KB_BLK_NBK, TL = KB_BLK_TILE;
     const int tid = threadIdx.x;
-    const int tx = tid & 15, ty = tid >> 4;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int li = lane & 15, lg = lane >> 4;
     const int nt = (m2 + TL - 1) / TL;
     for (int I = 0; I < nt; ++I) {
+        for (int e = tid; e < TL * NBK; e += KB_BLK_WG) {
+            const int r = e % TL, p = e / TL;
+            const int gi = I * TL + r;
+            S.sA[r * (NBK + 1) + p] = (gi < m2 && p < kb) ? L21[gi + (long)p * lda] : T(0);
+        }
         for (int J = 0; J <= I; ++J) {
-            // stage rows I*64.. and J*64.. of L21 (kb columns)
-            for (int e = tid; e < TL * NBK; e += KB_BLK_WG) {
-                const int r = e % TL, p = e / TL;
-                const int gi = I * TL + r, gj = J * TL + r;
-                S.sA[r * (NBK + 1) + p] = (gi < m2 && p < kb) ? L21[gi + (long)p * lda] : T(0);
-                S.sB[r * (NBK + 1) + p] = (gj < m2 && p < kb) ? L21[gj + (long)p * lda] : T(0);
-            }
-            // the tile of C, fetched before the rank-kb product so that its
-            // latency overlaps the FMAs
-            T cv[4][4];
-#pragma unroll
-            for (int a = 0; a < 4; ++a)
-#pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    const int gi = I * TL + ty + 16 * a, gj = J * TL + tx + 16 * b;
-                    const bool in = gi < m2 && gj <= gi;
-                    const T v = gld(C + (in ? gi + (long)gj * lda : 0));
-                    cv[a][b] = in ? v : T(0);
+            if (J != I) {
+                for (int e = tid; e < TL * NBK; e += KB_BLK_WG) {
+                    const int r = e % TL, p = e / TL;
+                    const int gj = J * TL + r;
+                    S.sB[r * (NBK + 1) + p] = (gj < m2 && p < kb) ? L21[gj + (long)p * lda] : T(0);
                 }
+            }
             __syncthreads();
-            T acc[4][4];
+            const T* sJ = (J == I) ? S.sA : S.sB;
+            const int gi = I * TL + 16 * wave + li;
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
+            for (int q = 0; q < 4; ++q) {
+                if (J == I && q > wave) continue;          // above the diagonal
+                int col[4];
 #pragma unroll
-                for (int b = 0; b < 4; ++b) acc[a][b] = T(0);
-            for (int p = 0; p < kb; ++p) {
-                T va[4], vb[4];
+                for (int r = 0; r < 4; ++r)
+                    col[r] = J * TL + 16 * q + ((sizeof(T) == 8) ? 4 * r + lg : 4 * lg + r);
+                // the C block first, so its latency overlaps the products
+                T cv[4];
 #pragma unroll
-                for (int a = 0; a < 4; ++a) va[a] = S.sA[(ty + 16 * a) * (NBK + 1) + p];
-#pragma unroll
-                for (int b = 0; b < 4; ++b) vb[b] = S.sB[(tx + 16 * b) * (NBK + 1) + p];
-#pragma unroll
-                for (int a = 0; a < 4; ++a)
-#pragma unroll
-                    for (int b = 0; b < 4; ++b) acc[a][b] = kb_fma<T>(va[a], vb[b], acc[a][b]);
-            }
-#pragma unroll
-            for (int a = 0; a < 4; ++a) {
-#pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    const int gi = I * TL + ty + 16 * a, gj = J * TL + tx + 16 * b;
-                    if (gi < m2 && gj <= gi) gst(C + gi + (long)gj * lda, cv[a][b] - acc[a][b]);
+                for (int r = 0; r < 4; ++r) {
+                    const bool in = gi < m2 && col[r] <= gi;
+                    const T v = gld(C + (in ? gi + (long)col[r] * lda : 0));
+                    cv[r] = in ? v : T(0);
                 }
+                typename TileV<T>::v4 acc = {T(0), T(0), T(0), T(0)};
+                for (int p0 = 0; p0 < kb; p0 += 4) {
+                    const T a = sJ[(16 * q + li) * (NBK + 1) + p0 + lg];
+                    const T b = S.sA[(16 * wave + li) * (NBK + 1) + p0 + lg];
+                    acc = mfma16x16x4(a, b, acc);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    if (gi < m2 && col[r] <= gi) gst(C + gi + (long)col[r] * lda, cv[r] - acc[r]);
             }
             __syncthreads();
         }
@@ -207,10 +211,12 @@ KB_DEV void blk_solve(const T* L, int lda, int n, T* B, int ldb, int side_left, 
             const int rows = n - (k0 + kb);
             for (int e = tid; e < rows * rc; e += KB_BLK_WG) {
                 const int i = k0 + kb + e % rows, c = e / rows;
-                T s = T(0);
-                for (int j = 0; j < kb; ++j) s = kb_fma<T>(L[i + (long)(k0 + j) * lda], sY[c * (NBK + 1) + j], s);
                 T& bi = rhs_at(B, ldb, side_left, i, c0 + c);
-                bi = bi - s;
+                const T b0 = bi;
+                T s = T(0);
+#pragma unroll 8
+                for (int j = 0; j < kb; ++j) s = kb_fma<T>(L[i + (long)(k0 + j) * lda], sY[c * (NBK + 1) + j], s);
+                bi = b0 - s;
             }
             __syncthreads();
         }
@@ -224,15 +230,42 @@ KB_DEV void blk_solve(const T* L, int lda, int n, T* B, int ldb, int side_left, 
         for (int c0 = 0; c0 < nrhs; c0 += KB_BLK_RC) {
             const int rc = min(KB_BLK_RC, nrhs - c0);
             const int below = n - (k0 + kb);
-            // y_p -= sum_{i below} L(i, k0+p) x_i
-            for (int e = tid; e < kb * rc; e += KB_BLK_WG) {
-                const int p = e % kb, c = e / kb;
-                T s = T(0);
-                const T* Lc = L + (long)(k0 + p) * lda;
-                for (int i = k0 + kb; i < k0 + kb + below; ++i)
-                    s = kb_fma<T>(Lc[i], rhs_at(B, ldb, side_left, i, c0 + c), s);
-                T& bp = rhs_at(B, ldb, side_left, k0 + p, c0 + c);
-                bp = bp - s;
+            // y_p -= sum_{i below} L(i, k0+p) x_i: the solved rows below are
+            // staged in LDS 64 at a time; thread (p, c) runs down column k0+p
+            // of L (consecutive addresses, unrolled for loads in flight)
+            // (p, c) pairs: 256 threads cover 32 x 8, a second pass the rest of the 16
+            static_assert(KB_BLK_RC <= 2 * (KB_BLK_WG / KB_BLK_NBK), "two passes of right-hand sides");
+            T* sX = S.sB;   // KB_BLK_TILE x (NBK + 1): rows of the chunk, one column per right-hand side
+            const int p = tid % NBK, cA = tid / NBK, cB = cA + KB_BLK_WG / NBK;
+            const bool actA = p < kb && cA < rc, actB = p < kb && cB < rc;
+            T sa = T(0), sb = T(0);
+            for (int i0 = k0 + kb; i0 < n; i0 += KB_BLK_TILE) {
+                const int ch = min(KB_BLK_TILE, n - i0);
+                for (int e = tid; e < ch * rc; e += KB_BLK_WG) {
+                    const int r = e % ch, cc = e / ch;
+                    sX[r * (NBK + 1) + cc] = rhs_at(B, ldb, side_left, i0 + r, c0 + cc);
+                }
+                __syncthreads();
+                if (actA) {
+                    const T* Lc = L + i0 + (long)(k0 + p) * lda;
+#pragma unroll 8
+                    for (int r = 0; r < ch; ++r) {
+                        const T l = Lc[r];
+                        sa = kb_fma<T>(l, sX[r * (NBK + 1) + cA], sa);
+                        if (actB) sb = kb_fma<T>(l, sX[r * (NBK + 1) + cB], sb);
+                    }
+                }
+                __syncthreads();
+            }
+            if (below > 0) {
+                if (actA) {
+                    T& bp = rhs_at(B, ldb, side_left, k0 + p, c0 + cA);
+                    bp = bp - sa;
+                }
+                if (actB) {
+                    T& bp = rhs_at(B, ldb, side_left, k0 + p, c0 + cB);
+                    bp = bp - sb;
+                }
             }
             __syncthreads();
             for (int c = wave; c < rc; c += KB_BLK_WG / 64) {
@@ -253,7 +286,7 @@ KB_DEV void blk_solve(const T* L, int lda, int n, T* B, int ldb, int side_left, 
 }
 
 template <typename T, int MODE>
-__global__ void __launch_bounds__(KB_BLK_WG)
+__global__ void __launch_bounds__(KB_BLK_WG) __attribute__((amdgpu_waves_per_eu(2)))
 k_blk_chol(int n, int nrhs, int side_left, BatchArg<T> A, BatchArg<T> B, int* info, int batch)
 {
     __shared__ BlkSmem<T> S;

@@ -262,6 +262,19 @@ KB_DEV void fnmac_rowk(T& acc, T src, T mul, int k)
 }
 #undef KB_FNR
 
+// ---------------------------------------------------------------------------
+// 16x16x4 matrix-core products (FP64 and FP32 accumulators of 4 per lane)
+// ---------------------------------------------------------------------------
+typedef double kb_d4 __attribute__((ext_vector_type(4)));
+typedef float kb_f4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct TileV;
+template <> struct TileV<double> { typedef kb_d4 v4; };
+template <> struct TileV<float> { typedef kb_f4 v4; };
+
+KB_DEV kb_d4 mfma16x16x4(double a, double b, kb_d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+KB_DEV kb_f4 mfma16x16x4(float a, float b, kb_f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+
 template <typename T> KB_DEV T kb_rsqrt(T x);
 template <> KB_DEV double kb_rsqrt<double>(double x)
 {

@@ -29,15 +29,6 @@
 #include "kb_launch.h"
 #include "kb_reg_impl.h"
 
-typedef double kb_d4 __attribute__((ext_vector_type(4)));
-typedef float kb_f4 __attribute__((ext_vector_type(4)));
-
-template <typename T> struct TileV;
-template <> struct TileV<double> { typedef kb_d4 v4; };
-template <> struct TileV<float> { typedef kb_f4 v4; };
-
-KB_DEV kb_d4 mfma16x16x4(double a, double b, kb_d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
-KB_DEV kb_f4 mfma16x16x4(float a, float b, kb_f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
 
 constexpr KB_DEV int tix(int I, int J) { return I * (I + 1) / 2 + J; }
 
@@ -178,9 +169,10 @@ KB_DEV int tile_potrf(typename TileV<T>::v4 (&t)[10], T (&invd)[4], int n, T (&p
 // ---------------------------------------------------------------------------
 // Stage factor tile (I, J) from L in global memory (padding rows/columns at or
 // beyond n read as the identity) into LDS and read row i (ROW) or column i of
-// it into v[16].  STRICT (diagonal tiles) keeps strictly-lower entries only and
-// scales entry j by 1/L(j,j) (invk = this lane's 1/L(i,i)), i.e. hands out the
-// unit-triangular factor so the sweeps need no per-step division.
+// it into v[16].  STRICT (diagonal tiles) hands out the strictly lower part of
+// the unit factor Lu = L D^-1 (column c scaled by 1/L(c,c); invk = this lane's
+// 1/L(i,i)): row i gives Lu(i, j), column i gives Lu(j, i), so the sweeps need
+// no per-step division.
 template <typename T, bool ROW, bool STRICT>
 KB_DEV void tile_stage(const T* L, int lda, int n, int I, int J, T (&v)[16], TileLds<T>& S, int lane,
                        T invk = T(1))
@@ -200,7 +192,7 @@ KB_DEV void tile_stage(const T* L, int lda, int n, int I, int J, T (&v)[16], Til
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
         T x = ROW ? S.tile[i][j] : S.tile[j][i];
-        if (STRICT) x = (ROW ? (j < i) : (j > i)) ? x * S.inv[j] : T(0);
+        if (STRICT) x = ROW ? ((j < i) ? x * S.inv[j] : T(0)) : ((j > i) ? x * invk : T(0));
         v[j] = x;
     }
     wave_lds_sync();
@@ -252,7 +244,7 @@ struct RegTiles {
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
             T y = ROW ? S->tile[i][j] : S->tile[j][i];
-            if (STRICT) y = (ROW ? (j < i) : (j > i)) ? y * S->inv[j] : T(0);
+            if (STRICT) y = ROW ? ((j < i) ? y * S->inv[j] : T(0)) : ((j > i) ? y * invk : T(0));
             v[j] = y;
         }
         wave_lds_sync();
@@ -288,7 +280,8 @@ KB_DEV void tile_solve(const Src& src, T (&y)[4][64], const T (&invd)[4][64], in
             y[I][lane] = yi;
         }
     }
-    // backward: L^T x = y, v[j] = L(j,i)/L(j,j) on the diagonal tiles
+    // backward: L^T x = y; on the diagonal tile L^T = D Lu^T, so scale by
+    // 1/L(i,i) first, then the unit sweep with v[j] = Lu(j, i)
 #pragma nounroll
     for (int K = 3; K >= 0; --K) {
         const T ik = invd[K][lane];
@@ -301,9 +294,9 @@ KB_DEV void tile_solve(const Src& src, T (&y)[4][64], const T (&invd)[4][64], in
             for (int j = 0; j < 16; ++j) fnmac_rowk<T>(yk, xi, v[j], j);
         }
         src.template get<false, true>(K, K, v, lane, ik);
+        yk *= ik;
 #pragma unroll
         for (int j = 15; j >= 0; --j) fnmac_rowk<T>(yk, yk, v[j], j);
-        yk *= ik;
         y[K][lane] = yk;
     }
 }

@@ -28,7 +28,9 @@ struct MixLds {
     };
 };
 
-// FP32 factor tiles read straight from LDS.
+// FP32 factor tiles read straight from LDS; the diagonal tiles are stored
+// already unit-scaled (strictly lower part, entry (i,j) times 1/L(j,j), zero
+// elsewhere), as the sweeps consume them.
 struct LdsTiles32 {
     MixLds* S;
     template <bool ROW, bool STRICT>
@@ -37,13 +39,8 @@ struct LdsTiles32 {
         (void)invk;
         const int i = lane & 15;
         const float(*t)[17] = S->m.l32[tix(I, J)];
-        const float* inv = S->m.inv32[J];   // row j of block J: lane j holds 1/L(16J+j)
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            float x = ROW ? t[i][j] : t[j][i];
-            if (STRICT) x = (ROW ? (j < i) : (j > i)) ? x * inv[j] : 0.0f;
-            v[j] = x;
-        }
+        for (int j = 0; j < 16; ++j) v[j] = ROW ? t[i][j] : t[j][i];
     }
 };
 
@@ -116,11 +113,20 @@ k_tile_sposv(int n, int nrhs, int side_left, BatchArg<double> A, BatchArg<double
         ok = (finfo == 0);
         if (ok) {
 #pragma unroll
-            for (int t = 0; t < 10; ++t)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) S.m.l32[t][i][4 * g + r] = tf[t][r];
-#pragma unroll
             for (int K = 0; K < 4; ++K) S.m.inv32[K][lane] = inv32[K];
+            wave_lds_sync();
+#pragma unroll
+            for (int I = 0; I < 4; ++I)
+#pragma unroll
+                for (int J = 0; J <= I; ++J)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int j = 4 * g + r;
+                        float v = tf[tix(I, J)][r];
+                        // diagonal tiles: unit lower part, 1/L(16J+j, 16J+j) sits in lane j
+                        if (I == J) v = (j < i) ? v * S.m.inv32[J][j] : 0.0f;
+                        S.m.l32[tix(I, J)][i][j] = v;
+                    }
         } else {
             iters = -3;
         }

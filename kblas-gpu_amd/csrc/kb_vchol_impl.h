@@ -135,7 +135,7 @@ KB_DEV void v_small(long mat, int n, int nrhs, int side_left, BatchArg<T>& A, Ba
 }
 
 template <typename T, int MODE>
-__global__ void __launch_bounds__(KB_BLK_WG)
+__global__ void __launch_bounds__(KB_BLK_WG) __attribute__((amdgpu_waves_per_eu(2)))
 k_vchol(const int* vm, const int* vn, int side_left, BatchArg<T> A, BatchArg<T> B, BatchArg<T> X,
         int* info, int* iter, int batch, void* ws)
 {
