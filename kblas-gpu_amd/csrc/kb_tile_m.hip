@@ -19,9 +19,6 @@ struct MixLds {
             float inv32[4][64];       // 1/L32(16K+i, 16K+i) per lane
             float z[4][64];           // FP32 correction blocks
             double tile[16][17];      // FP64 A-tile staging for the residual
-            double x[4][64];          // solution blocks
-            double r[4][64];          // residual blocks
-            double b[4][64];          // right-hand-side blocks
             double rs[64];            // row sums of |A|
         } m;
         TileLds<double> d;            // FP64 fallback
@@ -136,23 +133,23 @@ k_tile_sposv(int n, int nrhs, int side_left, BatchArg<double> A, BatchArg<double
     const int c = lane >> 4;
     for (int c0 = 0; c0 < nrhs && ok; c0 += 4) {
         const bool act = c0 + c < nrhs;
-        // x = 0, r = b
+        // x = 0, r = b; block K of right-hand side c of the group: row 16K + i in lane 16c + i
+        double xr[4], rr[4], br[4];
 #pragma unroll
         for (int K = 0; K < 4; ++K) {
             const int row = 16 * K + i;
             const bool in = act && row < n;
             const double ld = gld(rhs_elem(Bp, ldb, side_left, in ? row : 0, in ? c0 + c : 0));
-            S.m.x[K][lane] = 0.0;
-            S.m.r[K][lane] = in ? ld : 0.0;
-            S.m.b[K][lane] = in ? ld : 0.0;
+            xr[K] = 0.0;
+            br[K] = in ? ld : 0.0;
+            rr[K] = br[K];
         }
-        wave_lds_sync();
         bool conv = false;
         int it = 0;
         for (; it <= KB_TILE_ITERMAX; ++it) {
             // z = (L32 L32^T)^-1 r
 #pragma unroll
-            for (int K = 0; K < 4; ++K) S.m.z[K][lane] = (float)S.m.r[K][lane];
+            for (int K = 0; K < 4; ++K) S.m.z[K][lane] = (float)rr[K];
             wave_lds_sync();
             LdsTiles32 src{&S};
             tile_solve<float>(src, S.m.z, S.m.inv32, lane);
@@ -160,37 +157,28 @@ k_tile_sposv(int n, int nrhs, int side_left, BatchArg<double> A, BatchArg<double
             // x += z ; r = b - A x
 #pragma unroll
             for (int K = 0; K < 4; ++K) {
-                S.m.x[K][lane] += (double)S.m.z[K][lane];
-                S.m.r[K][lane] = S.m.b[K][lane];
+                xr[K] += (double)S.m.z[K][lane];
+                rr[K] = br[K];
             }
-            wave_lds_sync();
 #pragma unroll
             for (int I = 0; I < 4; ++I) {
 #pragma unroll
                 for (int J = 0; J <= I; ++J) {
-                    int ln = lane;
-                    asm volatile("" : "+v"(ln));
 #pragma unroll
                     for (int r = 0; r < 4; ++r) S.m.tile[i][4 * g + r] = ta[tix(I, J)][r];
                     wave_lds_sync();
                     double v[16];
                     {   // r_I -= A_IJ x_J   (row i of the tile)
-                        const double xj = S.m.x[J][ln];
 #pragma unroll
                         for (int j = 0; j < 16; ++j) v[j] = (I == J && j > i) ? S.m.tile[j][i] : S.m.tile[i][j];
-                        double ri = S.m.r[I][ln];
 #pragma unroll
-                        for (int j = 0; j < 16; ++j) fnmac_rowk<double>(ri, xj, v[j], j);
-                        S.m.r[I][ln] = ri;
+                        for (int j = 0; j < 16; ++j) fnmac_rowk<double>(rr[I], xr[J], v[j], j);
                     }
                     if (I != J) {   // r_J -= A_IJ^T x_I   (column i of the tile)
-                        const double xi = S.m.x[I][ln];
 #pragma unroll
                         for (int j = 0; j < 16; ++j) v[j] = S.m.tile[j][i];
-                        double rj = S.m.r[J][ln];
 #pragma unroll
-                        for (int j = 0; j < 16; ++j) fnmac_rowk<double>(rj, xi, v[j], j);
-                        S.m.r[J][ln] = rj;
+                        for (int j = 0; j < 16; ++j) fnmac_rowk<double>(rr[J], xr[I], v[j], j);
                     }
                     wave_lds_sync();
                 }
@@ -199,7 +187,7 @@ k_tile_sposv(int n, int nrhs, int side_left, BatchArg<double> A, BatchArg<double
             double rn = 0.0, xn = 0.0;
 #pragma unroll
             for (int K = 0; K < 4; ++K) {
-                const double ar = __builtin_fabs(S.m.r[K][lane]), ax = __builtin_fabs(S.m.x[K][lane]);
+                const double ar = __builtin_fabs(rr[K]), ax = __builtin_fabs(xr[K]);
                 rn = (ar > rn || ar != ar) ? ar : rn;
                 xn = ax > xn ? ax : xn;
             }
@@ -213,7 +201,7 @@ k_tile_sposv(int n, int nrhs, int side_left, BatchArg<double> A, BatchArg<double
 #pragma unroll
         for (int K = 0; K < 4; ++K) {
             const int row = 16 * K + i;
-            if (act && row < n) gst(rhs_elem(Xp, ldx, side_left, row, c0 + c), S.m.x[K][lane]);
+            if (act && row < n) gst(rhs_elem(Xp, ldx, side_left, row, c0 + c), xr[K]);
         }
     }
 

@@ -103,11 +103,13 @@ __global__ void k_vq_scatter(void* ws, const int* vm, const int* vn, int side_le
 }  // namespace
 
 // Shared memory of the persistent kernel: the blocked workgroup routine or,
-// in the register phase, one float LDS area per wavefront.
-template <typename T>
+// in the register phase, one float LDS area per wavefront (only the FP32 and
+// mixed routines use it; the FP64 ones read their transposed factor back from
+// memory, and leaving the area out lets more workgroups share a CU).
+template <typename T, bool LT>
 union VSmem {
     BlkSmem<T> blk;
-    float lt[4][64 * 65];
+    float lt[4][LT ? 64 * 65 : 1];
 };
 
 template <typename T, int MODE>
@@ -139,7 +141,7 @@ __global__ void __launch_bounds__(KB_BLK_WG) __attribute__((amdgpu_waves_per_eu(
 k_vchol(const int* vm, const int* vn, int side_left, BatchArg<T> A, BatchArg<T> B, BatchArg<T> X,
         int* info, int* iter, int batch, void* ws)
 {
-    __shared__ VSmem<T> S;
+    __shared__ VSmem<T, (sizeof(T) == 4 || MODE == KB_MODE_SPOSV)> S;
     __shared__ int s_ticket;
     VQueue q = vq_view(ws, batch);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -215,7 +217,12 @@ static int launch_vchol(const int* vm, const int* vn, int side_left, kb_mat_t A,
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
         cus = 256;
-    int grid = cus * 2;
+    // persistent grid: as many workgroups as can be resident at once
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_vchol<T, MODE>, KB_BLK_WG, 0) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 2;
+    int grid = cus * per_cu;
     if (grid > batch) grid = batch;
     hipLaunchKernelGGL((k_vchol<T, MODE>), dim3(grid), dim3(KB_BLK_WG), 0, s, vm, vn, side_left,
                        to_arg<T>(A), to_arg<T>(B), to_arg<T>(X), info, iter, batch, ws);
