@@ -18,6 +18,7 @@
 #include "kb_device.h"
 #include "kb_launch.h"
 #include "kb_reg_impl.h"
+#include "kb_rg_impl.h"
 
 constexpr int KB_BLK_NBK = 32;
 constexpr int KB_BLK_WG = 256;
@@ -103,6 +104,23 @@ KB_DEV void blk_syrk_update(const T* L21, T* C, int lda, int m2, int kb, BlkSmem
     }
 }
 
+// Load the diagonal block L(k0.., k0..) of the factor into LDS with its inverse diagonal.
+template <typename T>
+KB_DEV void blk_stage_diag(const T* L, int lda, int k0, int kb, BlkSmem<T>& S)
+{
+    constexpr int NBK = KB_BLK_NBK;
+    for (int e = threadIdx.x; e < NBK * NBK; e += KB_BLK_WG) {
+        const int r = e % NBK, c = e / NBK;
+        S.sD[r * (NBK + 1) + c] = (r < kb && c <= r) ? L[(k0 + r) + (long)(k0 + c) * lda] : T(0);
+    }
+    if (threadIdx.x < NBK) {
+        const int r = threadIdx.x;
+        S.sInv[r] = (r < kb) ? T(1) / L[(k0 + r) + (long)(k0 + r) * lda] : T(1);
+    }
+    __syncthreads();
+}
+
+// Solve (L L^T) X = B for nrhs vectors stored by rhs_at(); L lower in A.
 // In-place lower Cholesky of an n x n matrix by the whole workgroup.
 // Returns LAPACK info (identical in every thread).
 template <typename T>
@@ -114,27 +132,20 @@ KB_DEV int blk_potrf(T* A, int lda, int n, BlkSmem<T>& S)
     for (int k0 = 0; k0 < n; k0 += NBK) {
         const int kb = min(NBK, n - k0);
         if (wave == 0) {
-            T a[NBK];
-            const int r = lane;
-#pragma unroll
-            for (int c = 0; c < NBK; ++c) {
-                T v = T(0);
-                if (r < kb && c <= r) v = A[(k0 + r) + (long)(k0 + c) * lda];
-                a[c] = v;
-            }
-            T dg = T(1), inv = T(1);
-            const int inf = reg_potrf<T, NBK, 64>(a, kb, r, dg, inv);
-#pragma unroll
-            for (int c = 0; c < NBK; ++c) {
-                if (r < kb && c <= r) A[(k0 + r) + (long)(k0 + c) * lda] = a[c];
-                if (r < NBK) S.sD[r * (NBK + 1) + c] = (c <= r && r < kb) ? a[c] : T(0);
-            }
-            if (r < NBK) S.sInv[r] = inv;
+            // the 32 x 32 diagonal block on one DPP row of wave 0 (lane l: rows
+            // l and 16 + l), fused DPP FMAs; the other rows of the wave repeat
+            // the work without writing
+            T* D = A + k0 + (long)k0 * lda;
+            T a[2][NBK];
+            T dinv2[2] = {T(1), T(1)};
+            rg_load<T, NBK, false>(a, D, lda, kb, lane & 15);
+            const int inf = rg_potrf<T, NBK, false, false>(a, dinv2, kb, lane & 15, D, lda, lane < 16);
             if (lane == 0) S.flag = inf ? k0 + inf : 0;
         }
         __syncthreads();
         const int f = S.flag;
         if (f) return f;
+        blk_stage_diag<T>(A, lda, k0, kb, S);      // the factored block, with 1 / L(r,r), into LDS
         // panel below the diagonal block: X L_D^T = A_panel, one row per thread
         for (int i = k0 + kb + tid; i < n; i += KB_BLK_WG) {
             T x[NBK];
@@ -160,29 +171,22 @@ KB_DEV int blk_potrf(T* A, int lda, int n, BlkSmem<T>& S)
     return 0;
 }
 
-// Load the diagonal block L(k0.., k0..) of the factor into LDS with its inverse diagonal.
+// Row of a 16x16 matrix-core accumulator held in register q by lanes 16g..16g+15
+// (the column is the lane's position in its row of 16): FP64 interleaves the
+// rows, FP32 blocks them.
 template <typename T>
-KB_DEV void blk_stage_diag(const T* L, int lda, int k0, int kb, BlkSmem<T>& S)
+KB_DEV int blk_drow(int q, int g)
 {
-    constexpr int NBK = KB_BLK_NBK;
-    for (int e = threadIdx.x; e < NBK * NBK; e += KB_BLK_WG) {
-        const int r = e % NBK, c = e / NBK;
-        S.sD[r * (NBK + 1) + c] = (r < kb && c <= r) ? L[(k0 + r) + (long)(k0 + c) * lda] : T(0);
-    }
-    if (threadIdx.x < NBK) {
-        const int r = threadIdx.x;
-        S.sInv[r] = (r < kb) ? T(1) / L[(k0 + r) + (long)(k0 + r) * lda] : T(1);
-    }
-    __syncthreads();
+    return (sizeof(T) == 8) ? 4 * q + g : 4 * g + q;
 }
 
-// Solve (L L^T) X = B for nrhs vectors stored by rhs_at(); L lower in A.
 template <typename T>
 KB_DEV void blk_solve(const T* L, int lda, int n, T* B, int ldb, int side_left, int nrhs, BlkSmem<T>& S)
 {
     constexpr int NBK = KB_BLK_NBK;
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
+    const int li = lane & 15, lg = lane >> 4;
     T* sY = S.sA;   // KB_BLK_RC x (NBK + 1) staging of the current block's solution
     // ---------------- forward: L Y = B ----------------
     for (int k0 = 0; k0 < n; k0 += NBK) {
@@ -190,33 +194,72 @@ KB_DEV void blk_solve(const T* L, int lda, int n, T* B, int ldb, int side_left, 
         blk_stage_diag<T>(L, lda, k0, kb, S);
         for (int c0 = 0; c0 < nrhs; c0 += KB_BLK_RC) {
             const int rc = min(KB_BLK_RC, nrhs - c0);
-            for (int c = wave; c < rc; c += KB_BLK_WG / 64) {
-                const int r = lane;
-                T y = (r < kb) ? rhs_at(B, ldb, side_left, k0 + r, c0 + c) : T(0);
+            // diagonal block: right-hand side c = 4 wave + DPP row, lane l of the
+            // row holds rows l and 16 + l; y(j) comes from lane j by DPP
+            {
+                const int c = 4 * wave + lg;
+                const bool act = c < rc;
+                T y[2], iv[2];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int r = 16 * h + li;
+                    const bool in = act && r < kb;
+                    y[h] = in ? rhs_at(B, ldb, side_left, in ? k0 + r : 0, in ? c0 + c : 0) : T(0);
+                    iv[h] = S.sInv[r];
+                }
 #pragma unroll
                 for (int j = 0; j < NBK; ++j) {
-                    if (j < kb) {
-                        const T yj = bcastk<64>(y, j) * S.sInv[j];
-                        if (r > j && r < NBK) y = kb_fma<T>(-S.sD[r * (NBK + 1) + j], yj, y);
+                    const T t = y[j >> 4] * S.sInv[j];      // lane j: y(j) / L(j,j)
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const int r = 16 * h + li;
+                        if (16 * h + 15 > j) {
+                            const T l = (r > j) ? S.sD[r * (NBK + 1) + j] : T(0);
+                            fnmac_rowk<T>(y[h], t, l, j);
+                        }
                     }
                 }
-                if (r < kb) {
-                    y = y * S.sInv[r];
-                    rhs_at(B, ldb, side_left, k0 + r, c0 + c) = y;
-                    sY[c * (NBK + 1) + r] = y;
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int r = 16 * h + li;
+                    const T v = y[h] * iv[h];
+                    if (act && r < kb) {
+                        rhs_at(B, ldb, side_left, k0 + r, c0 + c) = v;
+                        sY[c * (NBK + 1) + r] = v;
+                    }
                 }
             }
             __syncthreads();
-            // rows below: y_i -= L(i, block) * y_block
+            // rows below: B(i, c) -= sum_j L(i, k0 + j) Y(j, c) on the matrix cores:
+            // 16-row blocks of the rows below (one per wave at a time), K = the
+            // block's columns in steps of 4 (zero beyond kb), N = the rc
+            // right-hand sides (zero beyond rc)
             const int rows = n - (k0 + kb);
-            for (int e = tid; e < rows * rc; e += KB_BLK_WG) {
-                const int i = k0 + kb + e % rows, c = e / rows;
-                T& bi = rhs_at(B, ldb, side_left, i, c0 + c);
-                const T b0 = bi;
-                T s = T(0);
-#pragma unroll 8
-                for (int j = 0; j < kb; ++j) s = kb_fma<T>(L[i + (long)(k0 + j) * lda], sY[c * (NBK + 1) + j], s);
-                bi = b0 - s;
+            for (int rb = wave * 16; rb < rows; rb += KB_BLK_WG / 4) {
+                const int r0 = k0 + kb + rb;
+                T bv[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int row = r0 + blk_drow<T>(q, lg);
+                    const bool in = row < n && li < rc;
+                    bv[q] = in ? rhs_at(B, ldb, side_left, row, c0 + li) : T(0);
+                }
+                typename TileV<T>::v4 acc = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+                for (int q = 0; q < NBK / 4; ++q) {
+                    const int kk = 4 * q + lg;
+                    const int row = r0 + li;
+                    const bool ain = row < n && kk < kb;
+                    const T av = gld(L + (ain ? row + (long)(k0 + kk) * lda : 0));
+                    const T a = ain ? av : T(0);
+                    const T b = (kk < kb && li < rc) ? sY[li * (NBK + 1) + kk] : T(0);
+                    acc = mfma16x16x4(a, b, acc);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int row = r0 + blk_drow<T>(q, lg);
+                    if (row < n && li < rc) rhs_at(B, ldb, side_left, row, c0 + li) = bv[q] - acc[q];
+                }
             }
             __syncthreads();
         }
@@ -230,55 +273,69 @@ KB_DEV void blk_solve(const T* L, int lda, int n, T* B, int ldb, int side_left, 
         for (int c0 = 0; c0 < nrhs; c0 += KB_BLK_RC) {
             const int rc = min(KB_BLK_RC, nrhs - c0);
             const int below = n - (k0 + kb);
-            // y_p -= sum_{i below} L(i, k0+p) x_i: the solved rows below are
-            // staged in LDS 64 at a time; thread (p, c) runs down column k0+p
-            // of L (consecutive addresses, unrolled for loads in flight)
-            // (p, c) pairs: 256 threads cover 32 x 8, a second pass the rest of the 16
-            static_assert(KB_BLK_RC <= 2 * (KB_BLK_WG / KB_BLK_NBK), "two passes of right-hand sides");
-            T* sX = S.sB;   // KB_BLK_TILE x (NBK + 1): rows of the chunk, one column per right-hand side
-            const int p = tid % NBK, cA = tid / NBK, cB = cA + KB_BLK_WG / NBK;
-            const bool actA = p < kb && cA < rc, actB = p < kb && cB < rc;
-            T sa = T(0), sb = T(0);
-            for (int i0 = k0 + kb; i0 < n; i0 += KB_BLK_TILE) {
-                const int ch = min(KB_BLK_TILE, n - i0);
-                for (int e = tid; e < ch * rc; e += KB_BLK_WG) {
-                    const int r = e % ch, cc = e / ch;
-                    sX[r * (NBK + 1) + cc] = rhs_at(B, ldb, side_left, i0 + r, c0 + cc);
+            // y_p -= sum_{i below} L(i, k0+p) x_i on the matrix cores: output rows
+            // p in two 16-row halves of the block, N = the rc right-hand sides,
+            // K = the rows below in steps of 4; wave w takes half (w & 1) and every
+            // other K step (w >> 1), and the two partial sums meet in LDS
+            {
+                const int ph = wave & 1, kh = wave >> 1;
+                const int pa = 16 * ph + li;
+                typename TileV<T>::v4 acc = {T(0), T(0), T(0), T(0)};
+                for (int r4 = 4 * kh; r4 < below; r4 += 8) {
+                    const int r = k0 + kb + r4 + lg;
+                    const bool ain = r < n && pa < kb;
+                    const T av = gld(L + (ain ? r + (long)(k0 + pa) * lda : 0));
+                    const T a = ain ? av : T(0);
+                    const bool bin = r < n && li < rc;
+                    const T b = bin ? rhs_at(B, ldb, side_left, bin ? r : 0, bin ? c0 + li : c0) : T(0);
+                    acc = mfma16x16x4(a, b, acc);
+                }
+                T* sR = S.sB;   // partial sums of waves 2 and 3: 2 x 4 x 64
+                if (kh == 1) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) sR[(ph * 4 + q) * 64 + lane] = acc[q];
                 }
                 __syncthreads();
-                if (actA) {
-                    const T* Lc = L + i0 + (long)(k0 + p) * lda;
-#pragma unroll 8
-                    for (int r = 0; r < ch; ++r) {
-                        const T l = Lc[r];
-                        sa = kb_fma<T>(l, sX[r * (NBK + 1) + cA], sa);
-                        if (actB) sb = kb_fma<T>(l, sX[r * (NBK + 1) + cB], sb);
+                if (kh == 0 && below > 0) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int po = 16 * ph + blk_drow<T>(q, lg);
+                        if (po < kb && li < rc) {
+                            T& bp = rhs_at(B, ldb, side_left, k0 + po, c0 + li);
+                            bp = bp - (acc[q] + sR[(ph * 4 + q) * 64 + lane]);
+                        }
                     }
                 }
                 __syncthreads();
             }
-            if (below > 0) {
-                if (actA) {
-                    T& bp = rhs_at(B, ldb, side_left, k0 + p, c0 + cA);
-                    bp = bp - sa;
+            {
+                const int c = 4 * wave + lg;
+                const bool act = c < rc;
+                T y[2], iv[2];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int r = 16 * h + li;
+                    const bool in = act && r < kb;
+                    y[h] = in ? rhs_at(B, ldb, side_left, in ? k0 + r : 0, in ? c0 + c : 0) : T(0);
+                    iv[h] = S.sInv[r];
                 }
-                if (actB) {
-                    T& bp = rhs_at(B, ldb, side_left, k0 + p, c0 + cB);
-                    bp = bp - sb;
-                }
-            }
-            __syncthreads();
-            for (int c = wave; c < rc; c += KB_BLK_WG / 64) {
-                const int r = lane;
-                T y = (r < kb) ? rhs_at(B, ldb, side_left, k0 + r, c0 + c) : T(0);
 #pragma unroll
                 for (int j = NBK - 1; j >= 0; --j) {
-                    if (j < kb) {
-                        const T xj = bcastk<64>(y, j) * S.sInv[j];
-                        if (r < j) y = kb_fma<T>(-S.sD[j * (NBK + 1) + r], xj, y);
+                    const T t = y[j >> 4] * S.sInv[j];      // lane j: x(j)
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const int r = 16 * h + li;
+                        if (16 * h < j) {
+                            const T l = (r < j) ? S.sD[j * (NBK + 1) + r] : T(0);
+                            fnmac_rowk<T>(y[h], t, l, j);
+                        }
                     }
                 }
-                if (r < kb) rhs_at(B, ldb, side_left, k0 + r, c0 + c) = y * S.sInv[r];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int r = 16 * h + li;
+                    if (act && r < kb) rhs_at(B, ldb, side_left, k0 + r, c0 + c) = y[h] * iv[h];
+                }
             }
             __syncthreads();
         }
