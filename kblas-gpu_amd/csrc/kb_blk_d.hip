@@ -19,7 +19,7 @@ k_sposv_full_prep(int n, int nrhs, int side_left, BatchArg<double> B, BatchArg<d
         double* Xp = X.get(mat);
         const int ldb = B.ldim(mat), ldx = X.ldim(mat);
         for (int e = threadIdx.x; e < n * nrhs; e += blockDim.x)
-            rhs_at(Xp, ldx, side_left, e % n, e / n) = rhs_at(const_cast<double*>(Bp), ldb, side_left, e % n, e / n);
+            gst(rhs_elem(Xp, ldx, side_left, e % n, e / n), gld(rhs_elem(const_cast<double*>(Bp), ldb, side_left, e % n, e / n)));
         if (threadIdx.x == 0 && iter) iter[mat] = -1;
     }
 }

@@ -61,14 +61,14 @@ KB_DEV void blk_syrk_update(const T* L21, T* C, int lda, int m2, int kb, BlkSmem
         for (int e = tid; e < TL * NBK; e += KB_BLK_WG) {
             const int r = e % TL, p = e / TL;
             const int gi = I * TL + r;
-            S.sA[r * (NBK + 1) + p] = (gi < m2 && p < kb) ? L21[gi + (long)p * lda] : T(0);
+            S.sA[r * (NBK + 1) + p] = (gi < m2 && p < kb) ? gld(L21 + gi + (long)p * lda) : T(0);
         }
         for (int J = 0; J <= I; ++J) {
             if (J != I) {
                 for (int e = tid; e < TL * NBK; e += KB_BLK_WG) {
                     const int r = e % TL, p = e / TL;
                     const int gj = J * TL + r;
-                    S.sB[r * (NBK + 1) + p] = (gj < m2 && p < kb) ? L21[gj + (long)p * lda] : T(0);
+                    S.sB[r * (NBK + 1) + p] = (gj < m2 && p < kb) ? gld(L21 + gj + (long)p * lda) : T(0);
                 }
             }
             __syncthreads();
@@ -111,11 +111,11 @@ KB_DEV void blk_stage_diag(const T* L, int lda, int k0, int kb, BlkSmem<T>& S)
     constexpr int NBK = KB_BLK_NBK;
     for (int e = threadIdx.x; e < NBK * NBK; e += KB_BLK_WG) {
         const int r = e % NBK, c = e / NBK;
-        S.sD[r * (NBK + 1) + c] = (r < kb && c <= r) ? L[(k0 + r) + (long)(k0 + c) * lda] : T(0);
+        S.sD[r * (NBK + 1) + c] = (r < kb && c <= r) ? gld(L + (k0 + r) + (long)(k0 + c) * lda) : T(0);
     }
     if (threadIdx.x < NBK) {
         const int r = threadIdx.x;
-        S.sInv[r] = (r < kb) ? T(1) / L[(k0 + r) + (long)(k0 + r) * lda] : T(1);
+        S.sInv[r] = (r < kb) ? T(1) / gld(L + (k0 + r) + (long)(k0 + r) * lda) : T(1);
     }
     __syncthreads();
 }
@@ -150,7 +150,7 @@ KB_DEV int blk_potrf(T* A, int lda, int n, BlkSmem<T>& S)
         for (int i = k0 + kb + tid; i < n; i += KB_BLK_WG) {
             T x[NBK];
 #pragma unroll
-            for (int c = 0; c < NBK; ++c) x[c] = (c < kb) ? A[i + (long)(k0 + c) * lda] : T(0);
+            for (int c = 0; c < NBK; ++c) x[c] = (c < kb) ? gld(A + i + (long)(k0 + c) * lda) : T(0);
 #pragma unroll
             for (int j = 0; j < NBK; ++j) {
                 if (j < kb) {
@@ -158,10 +158,13 @@ KB_DEV int blk_potrf(T* A, int lda, int n, BlkSmem<T>& S)
 #pragma unroll
                     for (int c = j + 1; c < NBK; ++c) x[c] = kb_fma<T>(-x[j], S.sD[c * (NBK + 1) + j], x[c]);
                 }
+                // keep each step's LDS reads with the step (hoisting them all would
+                // spill the 32-element row)
+                asm volatile("" ::: "memory");
             }
 #pragma unroll
             for (int c = 0; c < NBK; ++c)
-                if (c < kb) A[i + (long)(k0 + c) * lda] = x[c];
+                if (c < kb) gst(A + i + (long)(k0 + c) * lda, x[c]);
         }
         __syncthreads();
         const int t0 = k0 + kb;
@@ -204,7 +207,7 @@ KB_DEV void blk_solve(const T* L, int lda, int n, T* B, int ldb, int side_left, 
                 for (int h = 0; h < 2; ++h) {
                     const int r = 16 * h + li;
                     const bool in = act && r < kb;
-                    y[h] = in ? rhs_at(B, ldb, side_left, in ? k0 + r : 0, in ? c0 + c : 0) : T(0);
+                    y[h] = in ? gld(rhs_elem(B, ldb, side_left, in ? k0 + r : 0, in ? c0 + c : 0)) : T(0);
                     iv[h] = S.sInv[r];
                 }
 #pragma unroll
@@ -224,7 +227,7 @@ KB_DEV void blk_solve(const T* L, int lda, int n, T* B, int ldb, int side_left, 
                     const int r = 16 * h + li;
                     const T v = y[h] * iv[h];
                     if (act && r < kb) {
-                        rhs_at(B, ldb, side_left, k0 + r, c0 + c) = v;
+                        gst(rhs_elem(B, ldb, side_left, k0 + r, c0 + c), v);
                         sY[c * (NBK + 1) + r] = v;
                     }
                 }
@@ -242,7 +245,7 @@ KB_DEV void blk_solve(const T* L, int lda, int n, T* B, int ldb, int side_left, 
                 for (int q = 0; q < 4; ++q) {
                     const int row = r0 + blk_drow<T>(q, lg);
                     const bool in = row < n && li < rc;
-                    bv[q] = in ? rhs_at(B, ldb, side_left, row, c0 + li) : T(0);
+                    bv[q] = in ? gld(rhs_elem(B, ldb, side_left, row, c0 + li)) : T(0);
                 }
                 typename TileV<T>::v4 acc = {T(0), T(0), T(0), T(0)};
 #pragma unroll
@@ -258,7 +261,7 @@ KB_DEV void blk_solve(const T* L, int lda, int n, T* B, int ldb, int side_left, 
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int row = r0 + blk_drow<T>(q, lg);
-                    if (row < n && li < rc) rhs_at(B, ldb, side_left, row, c0 + li) = bv[q] - acc[q];
+                    if (row < n && li < rc) gst(rhs_elem(B, ldb, side_left, row, c0 + li), bv[q] - acc[q]);
                 }
             }
             __syncthreads();
@@ -287,7 +290,7 @@ KB_DEV void blk_solve(const T* L, int lda, int n, T* B, int ldb, int side_left, 
                     const T av = gld(L + (ain ? r + (long)(k0 + pa) * lda : 0));
                     const T a = ain ? av : T(0);
                     const bool bin = r < n && li < rc;
-                    const T b = bin ? rhs_at(B, ldb, side_left, bin ? r : 0, bin ? c0 + li : c0) : T(0);
+                    const T b = bin ? gld(rhs_elem(B, ldb, side_left, bin ? r : 0, bin ? c0 + li : c0)) : T(0);
                     acc = mfma16x16x4(a, b, acc);
                 }
                 T* sR = S.sB;   // partial sums of waves 2 and 3: 2 x 4 x 64
@@ -301,8 +304,8 @@ KB_DEV void blk_solve(const T* L, int lda, int n, T* B, int ldb, int side_left, 
                     for (int q = 0; q < 4; ++q) {
                         const int po = 16 * ph + blk_drow<T>(q, lg);
                         if (po < kb && li < rc) {
-                            T& bp = rhs_at(B, ldb, side_left, k0 + po, c0 + li);
-                            bp = bp - (acc[q] + sR[(ph * 4 + q) * 64 + lane]);
+                            T* bp = rhs_elem(B, ldb, side_left, k0 + po, c0 + li);
+                            gst(bp, gld(bp) - (acc[q] + sR[(ph * 4 + q) * 64 + lane]));
                         }
                     }
                 }
@@ -316,7 +319,7 @@ KB_DEV void blk_solve(const T* L, int lda, int n, T* B, int ldb, int side_left, 
                 for (int h = 0; h < 2; ++h) {
                     const int r = 16 * h + li;
                     const bool in = act && r < kb;
-                    y[h] = in ? rhs_at(B, ldb, side_left, in ? k0 + r : 0, in ? c0 + c : 0) : T(0);
+                    y[h] = in ? gld(rhs_elem(B, ldb, side_left, in ? k0 + r : 0, in ? c0 + c : 0)) : T(0);
                     iv[h] = S.sInv[r];
                 }
 #pragma unroll
@@ -334,7 +337,7 @@ KB_DEV void blk_solve(const T* L, int lda, int n, T* B, int ldb, int side_left, 
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
                     const int r = 16 * h + li;
-                    if (act && r < kb) rhs_at(B, ldb, side_left, k0 + r, c0 + c) = y[h] * iv[h];
+                    if (act && r < kb) gst(rhs_elem(B, ldb, side_left, k0 + r, c0 + c), y[h] * iv[h]);
                 }
             }
             __syncthreads();

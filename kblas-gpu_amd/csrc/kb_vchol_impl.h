@@ -167,7 +167,7 @@ k_vchol(const int* vm, const int* vn, int side_left, BatchArg<T> A, BatchArg<T> 
             T* Xp = X.get(mat);
             const int ldb = B.ldim(mat), ldx = X.ldim(mat);
             for (int e = threadIdx.x; e < n * nrhs; e += KB_BLK_WG)
-                rhs_at(Xp, ldx, side_left, e % n, e / n) = rhs_at(Bp, ldb, side_left, e % n, e / n);
+                gst(rhs_elem(Xp, ldx, side_left, e % n, e / n), gld(rhs_elem(Bp, ldb, side_left, e % n, e / n)));
             __syncthreads();
             const int inf = blk_potrf<T>(Ap, lda, n, S.blk);
             if (threadIdx.x == 0) { if (info) info[mat] = inf; if (iter) iter[mat] = -1; }
