@@ -170,6 +170,29 @@ def test_dposv_blocked_orders(h, n):
         assert np.all(np.isnan(po.col(L, i)[np.triu_indices(n, 1)]))
 
 
+@pytest.mark.parametrize("n", [70, 161])
+@pytest.mark.parametrize("side", ["L", "R"])
+def test_dposv_blocked_many_rhs(h, n, side):
+    # 20 right-hand sides: two passes (16 + 4) of the blocked solve
+    batch = 3
+    nrhs = 20
+    A = po.gen_spd_shift_batch(15, n, batch)
+    B = po.gen_rhs_batch(16, n, nrhs, batch) if side == "L" else po.gen_rhs_batch(16, nrhs, n, batch)
+    dA, dB = kt.to_dev(A), kt.to_dev(B)
+    info = kt.to_dev(np.full(batch, -7, np.int32))
+    rows, cols = B.shape[2], B.shape[1]
+    st = h.lib.kblasDposv_batch_strided(h.h, side.encode(), b"L", rows, cols, kt.P(dA), n, n * n, kt.P(dB), rows,
+                                        rows * cols, batch, kt.P(info))
+    assert st == pykblas.KBLAS_Success
+    assert np.all(kt.to_host(info) == 0)
+    X = kt.to_host(dB)
+    for i in range(batch):
+        Ai = po.col(A, i)
+        _, Xo, _ = po.posv(side, Ai, po.col(B, i))
+        assert rel(po.col(X, i), Xo) <= 64 * n * EPS
+        assert po.solve_berr(side, Ai, po.col(X, i), po.col(B, i)) <= 4 * n * EPS
+
+
 @pytest.mark.parametrize("n", [80, 150])
 def test_dpotrf_blocked_info_and_right_side_solve(h, n):
     batch = 4

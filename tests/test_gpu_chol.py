@@ -175,7 +175,7 @@ def test_dpotrs_uses_given_factor(h, n, side):
         assert np.max(np.abs(po.col(X, i) - Xo)) <= 64 * n * EPS * max(np.max(np.abs(Xo)), 1)
 
 
-@pytest.mark.parametrize("n", [3, 16, 32, 50, 64])
+@pytest.mark.parametrize("n", [3, 16, 32, 40, 50, 64])
 def test_spotrf_and_sposv_single_precision(h, n):
     batch = 17
     A = po.gen_spd_shift_batch(10, n, batch)
@@ -203,3 +203,19 @@ def test_argument_errors(h):
     assert lib.kblasDposv_batch_strided(h.h, b"R", b"U", 1, 8, kt.P(dA), 8, 64, kt.P(dA), 1, 8, 4, None) == pykblas.KBLAS_NotImplemented
     assert lib.kblasDposv_batch_strided(h.h, b"R", b"L", 2, 8, kt.P(dA), 8, 64, kt.P(dA), 1, 16, 4, None) == pykblas.KBLAS_Error_WrongInput
     assert lib.kblasDpotrf_batch(h.h, b"L", 8, P(0), 8, 4, None) == pykblas.KBLAS_Error_WrongInput
+
+
+@pytest.mark.parametrize("n", [33, 40, 48])
+def test_spotrf_blocked_rows_single_precision(h, n):
+    # 32 < n <= 48 POTRF runs the blocked DPP-row kernel (FP32 instance)
+    batch = 19
+    A = po.gen_spd_shift_batch(14, n, batch)
+    dA = kt.to_dev(nan_upper(A), np.float32)
+    info = kt.to_dev(np.full(batch, -7, np.int32))
+    st = h.lib.kblasSpotrf_batch_strided(h.h, b"L", n, kt.P(dA), n, n * n, batch, kt.P(info))
+    assert st == pykblas.KBLAS_Success
+    assert np.all(kt.to_host(info) == 0)
+    L = kt.to_host(dA).astype(np.float64)
+    for i in range(batch):
+        assert po.potrf_berr(po.col(A, i), po.col(L, i)) <= 4 * n * kt.EPS32
+        assert np.all(np.isnan(po.col(L, i)[np.triu_indices(n, 1)]))
