@@ -139,13 +139,13 @@ KB_DEV int blk_potrf(T* A, int lda, int n, BlkSmem<T>& S)
             T a[2][NBK];
             T dinv2[2] = {T(1), T(1)};
             rg_load<T, NBK, false>(a, D, lda, kb, lane & 15);
-            const int inf = rg_potrf<T, NBK, false, false>(a, dinv2, kb, lane & 15, D, lda, lane < 16);
+            // L_D also goes to LDS (sD, sInv) for the panel solve below
+            const int inf = rg_potrf<T, NBK, false, false>(a, dinv2, kb, lane & 15, D, lda, lane < 16, S.sD, S.sInv);
             if (lane == 0) S.flag = inf ? k0 + inf : 0;
         }
         __syncthreads();
         const int f = S.flag;
         if (f) return f;
-        blk_stage_diag<T>(A, lda, k0, kb, S);      // the factored block, with 1 / L(r,r), into LDS
         // panel below the diagonal block: X L_D^T = A_panel, one row per thread
         for (int i = k0 + kb + tid; i < n; i += KB_BLK_WG) {
             T x[NBK];

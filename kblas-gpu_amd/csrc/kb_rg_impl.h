@@ -65,8 +65,11 @@ KB_DEV void rg_load(T (&a)[NB / 16][NB], const T* Ap, int lda, int n, int l)
 // final (wr: this row of lanes writes).  KEEP: leave -Lu (strictly lower, zero elsewhere) in a and D^-2 of the
 // lane's own rows in dinv2 for the solves.  Returns the LAPACK info.
 // ---------------------------------------------------------------------------
+// lds_d / lds_inv (optional, written when wr): L as a dense NB x NB lower
+// triangle at pitch NB + 1 (zero above the diagonal) and 1 / L(j,j).
 template <typename T, int NB, bool EX, bool KEEP>
-KB_DEV int rg_potrf(T (&a)[NB / 16][NB], T (&dinv2)[NB / 16], int n, int l, T* Ap, int lda, bool wr)
+KB_DEV int rg_potrf(T (&a)[NB / 16][NB], T (&dinv2)[NB / 16], int n, int l, T* Ap, int lda, bool wr,
+                    T* lds_d = nullptr, T* lds_inv = nullptr)
 {
     constexpr int R = NB / 16;
     int info = 0;
@@ -85,6 +88,13 @@ KB_DEV int rg_potrf(T (&a)[NB / 16][NB], T (&dinv2)[NB / 16], int n, int l, T* A
             const T lv = a[s][j] * inv;          // the diagonal gives d / sqrt(d)
             if (wr && i >= j && (EX || i < n)) gst(Ap + i + (long)j * lda, lv);
             nl[s] = a[s][j] * ninv2;
+            if (lds_d && wr) lds_d[i * (NB + 1) + j] = (i >= j) ? lv : T(0);
+        }
+        if (lds_d && wr) {
+#pragma unroll
+            for (int s = 0; s < R; ++s)
+                if (s < sj) lds_d[(16 * s + l) * (NB + 1) + j] = T(0);
+            if (l == 0) lds_inv[j] = inv;
         }
         if (KEEP) dinv2[sj] = (l == lj) ? -ninv2 : dinv2[sj];
 #pragma unroll
