@@ -3,11 +3,12 @@
 
 // ---------------------------------------------------------------------------
 // Mixed precision POSV (LAPACK dsposv restated), 32 < n <= 64, one matrix per
-// wavefront: FP64 A kept as ten register tiles for the residual, its FP32 copy
-// factored with FP32 MFMA (2x the FP64 matrix-core rate), the FP32 factor
-// kept in LDS for the refinement sweeps; on a non-positive FP32 pivot or no
-// convergence after 30 steps the FP64 tiles are factored in place (A then
-// holds L, as dsposv) and solved in FP64.
+// wavefront: FP64 A kept in registers as all sixteen 16x16 tiles (the lower
+// ten as loaded, the upper six built by transposition) for the residual, its
+// FP32 copy factored with FP32 MFMA (2x the FP64 matrix-core rate), the unit
+// FP32 factor kept in LDS for the refinement sweeps; on a non-positive FP32
+// pivot or no convergence after 30 steps the FP64 tiles are factored in place
+// (A then holds L, as dsposv) and solved in FP64.
 // ---------------------------------------------------------------------------
 constexpr int KB_TILE_ITERMAX = 30;
 
@@ -15,30 +16,12 @@ struct MixLds {
     union {
         struct {
             float pan[4][16][4];
-            float l32[10][16][17];    // FP32 factor tiles, row-major
-            float inv32[4][64];       // 1/L32(16K+i, 16K+i) per lane
-            float z[4][64];           // FP32 correction blocks
-            double tile[16][17];      // FP64 A-tile staging for the residual
-            double rs[64];            // row sums of |A|
+            float l32[10][16][17];    // unit FP32 factor Lu = L D^-1, row-major tiles (diagonal: strictly lower)
+            float d2[4][64];          // 1/L32(16K+i, 16K+i)^2 per lane
+            double tile[16][17];      // FP64 tile transposes, then the solution blocks of the residual
         } m;
         TileLds<double> d;            // FP64 fallback
     };
-};
-
-// FP32 factor tiles read straight from LDS; the diagonal tiles are stored
-// already unit-scaled (strictly lower part, entry (i,j) times 1/L(j,j), zero
-// elsewhere), as the sweeps consume them.
-struct LdsTiles32 {
-    MixLds* S;
-    template <bool ROW, bool STRICT>
-    KB_DEV void get(int I, int J, float (&v)[16], int lane, float invk) const
-    {
-        (void)invk;
-        const int i = lane & 15;
-        const float(*t)[17] = S->m.l32[tix(I, J)];
-#pragma unroll
-        for (int j = 0; j < 16; ++j) v[j] = ROW ? t[i][j] : t[j][i];
-    }
 };
 
 KB_DEV double row16_max(double x)
@@ -51,7 +34,144 @@ KB_DEV double row16_max(double x)
     return x;
 }
 
-__global__ void __launch_bounds__(128)
+// Full symmetric A in tile storage: tu[tix(I, J)] (I > J) receives tile (J, I)
+// = T(I, J)^T and the diagonal tiles get their upper triangle mirrored.
+KB_DEV void mix_symmetrize(kb_d4 (&ta)[10], kb_d4 (&tu)[10], double (*tl)[17], int lane)
+{
+    const int i = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int I = 0; I < 4; ++I)
+#pragma unroll
+        for (int J = 0; J <= I; ++J) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) tl[i][4 * g + r] = ta[tix(I, J)][r];
+            wave_lds_sync();
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const double y = tl[4 * g + r][i];
+                if (I == J) {
+                    if (4 * g + r > i) ta[tix(I, I)][r] = y;
+                } else {
+                    tu[tix(I, J)][r] = y;
+                }
+            }
+            wave_lds_sync();
+        }
+}
+
+// element (16I + i, 16J + 4g + r) of the full symmetric A
+KB_DEV double mix_a(const kb_d4 (&ta)[10], const kb_d4 (&tu)[10], int I, int J, int r)
+{
+    return (I >= J) ? ta[tix(I, J)][r] : tu[tix(J, I)][r];
+}
+
+// ||A||_inf over the leading n x n block: row sums of |A| reduced over the
+// four lanes that share a row, then the maximum over the rows.
+KB_DEV double mix_anorm(const kb_d4 (&ta)[10], const kb_d4 (&tu)[10], int n, int lane)
+{
+    const int i = lane & 15;
+    double anrm = 0.0;
+#pragma unroll
+    for (int I = 0; I < 4; ++I) {
+        double s = 0.0;
+#pragma unroll
+        for (int J = 0; J < 4; ++J)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) s += __builtin_fabs(mix_a(ta, tu, I, J, r));
+        s += __shfl_xor(s, 16, 64);
+        s += __shfl_xor(s, 32, 64);
+        if (16 * I + i >= n) s = 0.0;     // identity padding
+        anrm = (s > anrm || s != s) ? s : anrm;
+    }
+    return row16_max(anrm);
+}
+
+// z := (Lu D^2 Lu^T)^-1 z in FP32 for the group's four right-hand sides
+// (lane 16c + i holds row 16K + i of right-hand side c in y[K]): a unit forward
+// sweep, the D^-2 scaling, a unit backward sweep.  Step j of a diagonal-tile
+// substitution is interleaved with the off-diagonal products that use the
+// value it has made final (lane j), so four dependency chains run side by side.
+KB_DEV void mix_solve32(float (&y)[4], const MixLds& S, int lane)
+{
+    const int i = lane & 15;
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int K = 0; K < 4; ++K) {
+        const float(*t)[17] = S.m.l32[tix(K, K)];
+        float yk = y[K] + acc[K];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+#pragma unroll
+            for (int I = K + 1; I < 4; ++I) fnmac_rowk<float>(acc[I], yk, S.m.l32[tix(I, K)][i][j], j);
+            fnmac_rowk<float>(yk, yk, t[i][j], j);
+        }
+        y[K] = yk * S.m.d2[K][lane];
+    }
+#pragma unroll
+    for (int K = 0; K < 4; ++K) acc[K] = 0.f;
+#pragma unroll
+    for (int K = 3; K >= 0; --K) {
+        const float(*t)[17] = S.m.l32[tix(K, K)];
+        float yk = y[K] + acc[K];
+#pragma unroll
+        for (int j = 15; j >= 0; --j) {
+#pragma unroll
+            for (int I = 0; I < K; ++I) fnmac_rowk<float>(acc[I], yk, S.m.l32[tix(K, I)][j][i], j);
+            fnmac_rowk<float>(yk, yk, t[j][i], j);
+        }
+        y[K] = yk;
+    }
+}
+
+// r := b - A x in FP64.  The solution blocks go through LDS (xs[J][row][c]);
+// lane (i, g) forms the partial products of its columns 4g..4g+3 for all four
+// right-hand sides (sixteen independent accumulators), and a two-step
+// exchange across the lane groups (xor 32, xor 16) sums them so that lane
+// 16c + i ends with row 16I + i of right-hand side c.
+KB_DEV void mix_residual(const kb_d4 (&ta)[10], const kb_d4 (&tu)[10], const double (&xr)[4],
+                         const double (&br)[4], double (&rr)[4], double (*xs)[16][4], int lane)
+{
+    const int i = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int K = 0; K < 4; ++K) xs[K][i][g] = xr[K];
+    wave_lds_sync();
+    double q[4][4];
+#pragma unroll
+    for (int I = 0; I < 4; ++I)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) q[I][c] = 0.0;
+#pragma unroll
+    for (int J = 0; J < 4; ++J)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            double x4[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) x4[c] = xs[J][4 * g + r][c];
+#pragma unroll
+            for (int I = 0; I < 4; ++I) {
+                const double a = mix_a(ta, tu, I, J, r);
+#pragma unroll
+                for (int c = 0; c < 4; ++c) q[I][c] = __builtin_fma(a, x4[c], q[I][c]);
+            }
+        }
+    wave_lds_sync();
+    const bool b1 = (g & 2) != 0, b0 = (g & 1) != 0;
+#pragma unroll
+    for (int I = 0; I < 4; ++I) {
+        double h[2];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const double send = b1 ? q[I][c] : q[I][2 + c];
+            const double keep = b1 ? q[I][2 + c] : q[I][c];
+            h[c] = keep + __shfl_xor(send, 32, 64);
+        }
+        const double send = b0 ? h[0] : h[1];
+        const double keep = b0 ? h[1] : h[0];
+        rr[I] = br[I] - (keep + __shfl_xor(send, 16, 64));
+    }
+}
+
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
 k_tile_sposv(int n, int nrhs, int side_left, BatchArg<double> A, BatchArg<double> B, BatchArg<double> X,
              int* info, int* iter, int batch)
 {
@@ -71,32 +191,7 @@ k_tile_sposv(int n, int nrhs, int side_left, BatchArg<double> A, BatchArg<double
     kb_d4 ta[10];
     tile_load<double>(ta, Ap, lda, n, lane);
 
-    // ||A||_inf from the lower triangle (row sums of |A| through LDS atomics)
-    S.m.rs[lane] = 0.0;
-    wave_lds_sync();
-#pragma unroll
-    for (int I = 0; I < 4; ++I)
-#pragma unroll
-        for (int J = 0; J <= I; ++J)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = 16 * I + i, col = 16 * J + 4 * g + r;
-                if (row < n && col <= row) {
-                    const double a = __builtin_fabs(ta[tix(I, J)][r]);
-                    atomicAdd(&S.m.rs[row], a);
-                    if (col < row) atomicAdd(&S.m.rs[col], a);
-                }
-            }
-    wave_lds_sync();
-    double anrm = S.m.rs[lane];
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        double y = __shfl_xor(anrm, off, 64);
-        anrm = y > anrm ? y : anrm;
-    }
-    const double cte = anrm * 1.1102230246251565e-16 * __builtin_sqrt((double)n);
-
-    // FP32 factorization
+    // FP32 factorization; the unit factor Lu = L D^-1 goes to LDS
     int iters = 0;
     bool ok;
     {
@@ -110,8 +205,7 @@ k_tile_sposv(int n, int nrhs, int side_left, BatchArg<double> A, BatchArg<double
         ok = (finfo == 0);
         if (ok) {
 #pragma unroll
-            for (int K = 0; K < 4; ++K) S.m.inv32[K][lane] = inv32[K];
-            wave_lds_sync();
+            for (int K = 0; K < 4; ++K) S.m.d2[K][lane] = inv32[K] * inv32[K];
 #pragma unroll
             for (int I = 0; I < 4; ++I)
 #pragma unroll
@@ -119,9 +213,9 @@ k_tile_sposv(int n, int nrhs, int side_left, BatchArg<double> A, BatchArg<double
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const int j = 4 * g + r;
-                        float v = tf[tix(I, J)][r];
-                        // diagonal tiles: unit lower part, 1/L(16J+j, 16J+j) sits in lane j
-                        if (I == J) v = (j < i) ? v * S.m.inv32[J][j] : 0.0f;
+                        // column j scaled by 1/L(16J+j, 16J+j), which lane j holds
+                        float v = tf[tix(I, J)][r] * __shfl(inv32[J], j, 64);
+                        if (I == J && j >= i) v = 0.0f;
                         S.m.l32[tix(I, J)][i][j] = v;
                     }
         } else {
@@ -129,6 +223,12 @@ k_tile_sposv(int n, int nrhs, int side_left, BatchArg<double> A, BatchArg<double
         }
     }
     wave_lds_sync();
+
+    kb_d4 tu[10];
+    mix_symmetrize(ta, tu, S.m.tile, lane);
+    const double anrm = mix_anorm(ta, tu, n, lane);
+    const double cte = anrm * 1.1102230246251565e-16 * __builtin_sqrt((double)n);
+    double(*xs)[16][4] = (double(*)[16][4]) & S.m.tile[0][0];
 
     const int c = lane >> 4;
     for (int c0 = 0; c0 < nrhs && ok; c0 += 4) {
@@ -147,42 +247,14 @@ k_tile_sposv(int n, int nrhs, int side_left, BatchArg<double> A, BatchArg<double
         bool conv = false;
         int it = 0;
         for (; it <= KB_TILE_ITERMAX; ++it) {
-            // z = (L32 L32^T)^-1 r
+            // x += (L32 L32^T)^-1 r ; r = b - A x
+            float z[4];
 #pragma unroll
-            for (int K = 0; K < 4; ++K) S.m.z[K][lane] = (float)rr[K];
-            wave_lds_sync();
-            LdsTiles32 src{&S};
-            tile_solve<float>(src, S.m.z, S.m.inv32, lane);
-            wave_lds_sync();
-            // x += z ; r = b - A x
+            for (int K = 0; K < 4; ++K) z[K] = (float)rr[K];
+            mix_solve32(z, S, lane);
 #pragma unroll
-            for (int K = 0; K < 4; ++K) {
-                xr[K] += (double)S.m.z[K][lane];
-                rr[K] = br[K];
-            }
-#pragma unroll
-            for (int I = 0; I < 4; ++I) {
-#pragma unroll
-                for (int J = 0; J <= I; ++J) {
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) S.m.tile[i][4 * g + r] = ta[tix(I, J)][r];
-                    wave_lds_sync();
-                    double v[16];
-                    {   // r_I -= A_IJ x_J   (row i of the tile)
-#pragma unroll
-                        for (int j = 0; j < 16; ++j) v[j] = (I == J && j > i) ? S.m.tile[j][i] : S.m.tile[i][j];
-#pragma unroll
-                        for (int j = 0; j < 16; ++j) fnmac_rowk<double>(rr[I], xr[J], v[j], j);
-                    }
-                    if (I != J) {   // r_J -= A_IJ^T x_I   (column i of the tile)
-#pragma unroll
-                        for (int j = 0; j < 16; ++j) v[j] = S.m.tile[j][i];
-#pragma unroll
-                        for (int j = 0; j < 16; ++j) fnmac_rowk<double>(rr[J], xr[I], v[j], j);
-                    }
-                    wave_lds_sync();
-                }
-            }
+            for (int K = 0; K < 4; ++K) xr[K] += (double)z[K];
+            mix_residual(ta, tu, xr, br, rr, xs, lane);
             // convergence: ||r||inf <= ||x||inf * cte for every right-hand side of the group
             double rn = 0.0, xn = 0.0;
 #pragma unroll
@@ -209,6 +281,8 @@ k_tile_sposv(int n, int nrhs, int side_left, BatchArg<double> A, BatchArg<double
     if (!ok) {
         if (iters != -3) iters = -(KB_TILE_ITERMAX + 1);
         wave_lds_sync();
+        // the lower tiles of ta are A's (the mirrored upper parts of the
+        // diagonal tiles never feed the lower triangle and are not stored)
         double invd[4] = {1.0, 1.0, 1.0, 1.0};
         inf = tile_potrf<double>(ta, invd, n, S.d.pan, lane);
         tile_store<double>(ta, Ap, lda, n, lane);
