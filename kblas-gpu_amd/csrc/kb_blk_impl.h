@@ -11,9 +11,10 @@
 //   - the panel below is solved one row per thread against the LDS block;
 //   - the trailing lower triangle is updated in 64x64 tiles (4x4 per thread)
 //     from two LDS-staged 64x32 slices of the panel.
-// The solve (POSV/POTRS) runs blocked forward and backward sweeps: the
-// diagonal block by one wave per right-hand side, the off-diagonal updates
-// spread over (row, right-hand side) pairs.
+// The solve (POSV/POTRS) runs blocked forward and backward sweeps over up to
+// 64 right-hand sides per pass (so L is streamed once per sweep for up to 64
+// right-hand sides): the diagonal blocks by DPP-row substitutions, the
+// off-diagonal block products on the matrix cores.
 #pragma once
 #include "kb_device.h"
 #include "kb_launch.h"
@@ -23,7 +24,7 @@
 constexpr int KB_BLK_NBK = 32;
 constexpr int KB_BLK_WG = 256;
 constexpr int KB_BLK_TILE = 64;
-constexpr int KB_BLK_RC = 16;   // right-hand sides staged per pass of the solve
+constexpr int KB_BLK_RC = 64;   // right-hand sides per pass of the solve
 
 template <typename T>
 struct BlkSmem {
@@ -53,7 +54,8 @@ template <typename T>
 KB_DEV void blk_syrk_update(const T* L21, T* C, int lda, int m2, int kb, BlkSmem<T>& S)
 {
     constexpr int NBK = KB_BLK_NBK, TL = KB_BLK_TILE;
-    const int tid = threadIdx.x;
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));   // lane-dependent values formed per call, not kept live across
     const int wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lg = lane >> 4;
     const int nt = (m2 + TL - 1) / TL;
@@ -120,16 +122,16 @@ KB_DEV void blk_stage_diag(const T* L, int lda, int k0, int kb, BlkSmem<T>& S)
     __syncthreads();
 }
 
-// Solve (L L^T) X = B for nrhs vectors stored by rhs_at(); L lower in A.
 // In-place lower Cholesky of an n x n matrix by the whole workgroup.
 // Returns LAPACK info (identical in every thread).
 template <typename T>
 KB_DEV int blk_potrf(T* A, int lda, int n, BlkSmem<T>& S)
 {
     constexpr int NBK = KB_BLK_NBK;
-    const int tid = threadIdx.x;
-    const int wave = tid >> 6, lane = tid & 63;
     for (int k0 = 0; k0 < n; k0 += NBK) {
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));   // lane-dependent values formed per panel, not kept live across
+        const int wave = tid >> 6, lane = tid & 63;
         const int kb = min(NBK, n - k0);
         if (wave == 0) {
             // the 32 x 32 diagonal block on one DPP row of wave 0 (lane l: rows
@@ -183,87 +185,177 @@ KB_DEV int blk_drow(int q, int g)
     return (sizeof(T) == 8) ? 4 * q + g : 4 * g + q;
 }
 
+// Substitution with the staged diagonal block (sD, sInv) for the NS groups of
+// 16 right-hand sides of a pass: DPP row (wave, lg) of the workgroup takes
+// right-hand sides c = 16 s + 4 wave + lg, lane li of the row holds block rows
+// li and 16 + li, and y(j) comes from lane j by DPP; the NS independent
+// substitutions are interleaved and share the LDS reads of L.
+// FWD: L_D y = b, else L_D^T x = y; the result also goes to sY (right-hand
+// side major) for the off-diagonal update that follows.
+template <typename T, int NS, bool FWD>
+KB_DEV void blk_diag_subst(T* B, int ldb, int side_left, int k0, int kb, int c0, int rc, BlkSmem<T>& S, T* sY)
+{
+    constexpr int NBK = KB_BLK_NBK;
+    int tid = threadIdx.x;
+    // opaque to the optimizer, so the lane-dependent offsets below are formed
+    // here and not hoisted out of the caller's loops (where they would stay
+    // live through the factorization)
+    asm volatile("" : "+v"(tid));
+    const int lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lg = lane >> 4;
+    T y[NS][2], iv[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) iv[h] = S.sInv[16 * h + li];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const int c = 16 * s + 4 * wave + lg;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int r = 16 * h + li;
+            const bool in = c < rc && r < kb;
+            y[s][h] = in ? gld(rhs_elem(B, ldb, side_left, in ? k0 + r : 0, in ? c0 + c : 0)) : T(0);
+        }
+    }
+#pragma unroll
+    for (int jj = 0; jj < NBK; ++jj) {
+        const int j = FWD ? jj : NBK - 1 - jj;
+        const T ij = S.sInv[j];
+        T t[NS];     // lane j: y(j) / L(j, j), read through DPP (lane j is not updated at step j)
+#pragma unroll
+        for (int s = 0; s < NS; ++s) t[s] = y[s][j >> 4] * ij;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int r = 16 * h + li;
+            if (FWD ? (16 * h + 15 > j) : (16 * h < j)) {
+                const T l = FWD ? ((r > j) ? S.sD[r * (NBK + 1) + j] : T(0))
+                                : ((r < j) ? S.sD[j * (NBK + 1) + r] : T(0));
+#pragma unroll
+                for (int s = 0; s < NS; ++s) fnmac_rowk<T>(y[s][h], t[s], l, j);
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const int c = 16 * s + 4 * wave + lg;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int r = 16 * h + li;
+            const T v = y[s][h] * iv[h];
+            if (c < rc && r < kb) {
+                gst(rhs_elem(B, ldb, side_left, k0 + r, c0 + c), v);
+                sY[c * (NBK + 1) + r] = v;
+            }
+        }
+    }
+}
+
+template <typename T, bool FWD>
+KB_DEV void blk_diag_subst_n(int ns, T* B, int ldb, int side_left, int k0, int kb, int c0, int rc, BlkSmem<T>& S,
+                             T* sY)
+{
+    switch (ns) {
+    case 1: blk_diag_subst<T, 1, FWD>(B, ldb, side_left, k0, kb, c0, rc, S, sY); break;
+    case 2: blk_diag_subst<T, 2, FWD>(B, ldb, side_left, k0, kb, c0, rc, S, sY); break;
+    case 3: blk_diag_subst<T, 3, FWD>(B, ldb, side_left, k0, kb, c0, rc, S, sY); break;
+    default: blk_diag_subst<T, 4, FWD>(B, ldb, side_left, k0, kb, c0, rc, S, sY); break;
+    }
+}
+
+// B(r, c) -= sum_k Lk(r, k) Y(k, c) for the rows r in [r_lo, r_hi), k < kb and
+// the pass's right-hand sides, on the matrix cores.  Lk(r, k) = L(r, k0 + k)
+// (TRANS false: the block column below the diagonal block, forward sweep) or
+// L(k0 + k, r) (TRANS true: the block row left of it, backward sweep); Y is
+// staged in sY (right-hand side major).  Each wave takes 16-row blocks: the
+// L fragment (eight K steps of 4) and the B blocks of the NCB column blocks of
+// 16 right-hand sides are loaded together, then NCB independent MFMA chains.
+template <typename T, bool TRANS, int NCB>
+KB_DEV void blk_update_rows(const T* L, int lda, T* B, int ldb, int side_left, int k0, int kb, int r_lo,
+                            int r_hi, int c0, int rc, const T* sY)
+{
+    constexpr int NBK = KB_BLK_NBK;
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));   // lane-dependent offsets stay local (as in blk_diag_subst)
+    const int wave = tid >> 6, lane = tid & 63;
+    const int li = lane & 15, lg = lane >> 4;
+    for (int r0 = r_lo + 16 * wave; r0 < r_hi; r0 += KB_BLK_WG / 4) {
+        T a[NBK / 4];
+#pragma unroll
+        for (int q = 0; q < NBK / 4; ++q) {
+            const int kk = 4 * q + lg;
+            const int row = r0 + li;
+            const bool ain = row < r_hi && kk < kb;
+            const long off = TRANS ? (k0 + kk) + (long)row * lda : row + (long)(k0 + kk) * lda;
+            const T av = gld(L + (ain ? off : 0));
+            a[q] = ain ? av : T(0);
+        }
+        T bv[NCB][4];
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb) {
+            const int cc = 16 * cb + li;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int row = r0 + blk_drow<T>(q, lg);
+                const bool in = row < r_hi && cc < rc;
+                const T v = gld(rhs_elem(B, ldb, side_left, in ? row : 0, in ? c0 + cc : c0));
+                bv[cb][q] = in ? v : T(0);
+            }
+        }
+        typename TileV<T>::v4 acc[NCB];
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb) acc[cb] = typename TileV<T>::v4{T(0), T(0), T(0), T(0)};
+#pragma unroll
+        for (int q = 0; q < NBK / 4; ++q) {
+            const int kk = 4 * q + lg;
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb) {
+                const int cc = 16 * cb + li;
+                const T b = (kk < kb && cc < rc) ? sY[cc * (NBK + 1) + kk] : T(0);
+                acc[cb] = mfma16x16x4(a[q], b, acc[cb]);
+            }
+        }
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb) {
+            const int cc = 16 * cb + li;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int row = r0 + blk_drow<T>(q, lg);
+                if (row < r_hi && cc < rc) gst(rhs_elem(B, ldb, side_left, row, c0 + cc), bv[cb][q] - acc[cb][q]);
+            }
+        }
+    }
+}
+
+template <typename T, bool TRANS>
+KB_DEV void blk_update_rows_n(int ncb, const T* L, int lda, T* B, int ldb, int side_left, int k0, int kb, int r_lo,
+                              int r_hi, int c0, int rc, const T* sY)
+{
+    switch (ncb) {
+    case 1: blk_update_rows<T, TRANS, 1>(L, lda, B, ldb, side_left, k0, kb, r_lo, r_hi, c0, rc, sY); break;
+    case 2: blk_update_rows<T, TRANS, 2>(L, lda, B, ldb, side_left, k0, kb, r_lo, r_hi, c0, rc, sY); break;
+    case 3: blk_update_rows<T, TRANS, 3>(L, lda, B, ldb, side_left, k0, kb, r_lo, r_hi, c0, rc, sY); break;
+    default: blk_update_rows<T, TRANS, 4>(L, lda, B, ldb, side_left, k0, kb, r_lo, r_hi, c0, rc, sY); break;
+    }
+}
+
+// Solve (L L^T) X = B for nrhs vectors stored by rhs_at(); L lower in A.
+// Both sweeps are right-looking over 32-row blocks: substitution with the
+// diagonal block, then the update of the rows not yet solved (below it in
+// the forward sweep, above it in the backward one).
 template <typename T>
 KB_DEV void blk_solve(const T* L, int lda, int n, T* B, int ldb, int side_left, int nrhs, BlkSmem<T>& S)
 {
-    constexpr int NBK = KB_BLK_NBK;
-    const int tid = threadIdx.x;
-    const int wave = tid >> 6, lane = tid & 63;
-    const int li = lane & 15, lg = lane >> 4;
-    T* sY = S.sA;   // KB_BLK_RC x (NBK + 1) staging of the current block's solution
+    constexpr int NBK = KB_BLK_NBK, RC = KB_BLK_RC;
+    T* sY = S.sA;   // RC x (NBK + 1) staging of the current block's solution
     // ---------------- forward: L Y = B ----------------
     for (int k0 = 0; k0 < n; k0 += NBK) {
         const int kb = min(NBK, n - k0);
         blk_stage_diag<T>(L, lda, k0, kb, S);
-        for (int c0 = 0; c0 < nrhs; c0 += KB_BLK_RC) {
-            const int rc = min(KB_BLK_RC, nrhs - c0);
-            // diagonal block: right-hand side c = 4 wave + DPP row, lane l of the
-            // row holds rows l and 16 + l; y(j) comes from lane j by DPP
-            {
-                const int c = 4 * wave + lg;
-                const bool act = c < rc;
-                T y[2], iv[2];
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int r = 16 * h + li;
-                    const bool in = act && r < kb;
-                    y[h] = in ? gld(rhs_elem(B, ldb, side_left, in ? k0 + r : 0, in ? c0 + c : 0)) : T(0);
-                    iv[h] = S.sInv[r];
-                }
-#pragma unroll
-                for (int j = 0; j < NBK; ++j) {
-                    const T t = y[j >> 4] * S.sInv[j];      // lane j: y(j) / L(j,j)
-#pragma unroll
-                    for (int h = 0; h < 2; ++h) {
-                        const int r = 16 * h + li;
-                        if (16 * h + 15 > j) {
-                            const T l = (r > j) ? S.sD[r * (NBK + 1) + j] : T(0);
-                            fnmac_rowk<T>(y[h], t, l, j);
-                        }
-                    }
-                }
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int r = 16 * h + li;
-                    const T v = y[h] * iv[h];
-                    if (act && r < kb) {
-                        gst(rhs_elem(B, ldb, side_left, k0 + r, c0 + c), v);
-                        sY[c * (NBK + 1) + r] = v;
-                    }
-                }
-            }
+        for (int c0 = 0; c0 < nrhs; c0 += RC) {
+            const int rc = min(RC, nrhs - c0);
+            const int ncb = (rc + 15) / 16;
+            blk_diag_subst_n<T, true>(ncb, B, ldb, side_left, k0, kb, c0, rc, S, sY);
             __syncthreads();
-            // rows below: B(i, c) -= sum_j L(i, k0 + j) Y(j, c) on the matrix cores:
-            // 16-row blocks of the rows below (one per wave at a time), K = the
-            // block's columns in steps of 4 (zero beyond kb), N = the rc
-            // right-hand sides (zero beyond rc)
-            const int rows = n - (k0 + kb);
-            for (int rb = wave * 16; rb < rows; rb += KB_BLK_WG / 4) {
-                const int r0 = k0 + kb + rb;
-                T bv[4];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int row = r0 + blk_drow<T>(q, lg);
-                    const bool in = row < n && li < rc;
-                    bv[q] = in ? gld(rhs_elem(B, ldb, side_left, row, c0 + li)) : T(0);
-                }
-                typename TileV<T>::v4 acc = {T(0), T(0), T(0), T(0)};
-#pragma unroll
-                for (int q = 0; q < NBK / 4; ++q) {
-                    const int kk = 4 * q + lg;
-                    const int row = r0 + li;
-                    const bool ain = row < n && kk < kb;
-                    const T av = gld(L + (ain ? row + (long)(k0 + kk) * lda : 0));
-                    const T a = ain ? av : T(0);
-                    const T b = (kk < kb && li < rc) ? sY[li * (NBK + 1) + kk] : T(0);
-                    acc = mfma16x16x4(a, b, acc);
-                }
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int row = r0 + blk_drow<T>(q, lg);
-                    if (row < n && li < rc) gst(rhs_elem(B, ldb, side_left, row, c0 + li), bv[q] - acc[q]);
-                }
-            }
+            blk_update_rows_n<T, false>(ncb, L, lda, B, ldb, side_left, k0, kb, k0 + kb, n, c0, rc, sY);
             __syncthreads();
         }
     }
@@ -273,73 +365,12 @@ KB_DEV void blk_solve(const T* L, int lda, int n, T* B, int ldb, int side_left, 
         const int k0 = bk * NBK;
         const int kb = min(NBK, n - k0);
         blk_stage_diag<T>(L, lda, k0, kb, S);
-        for (int c0 = 0; c0 < nrhs; c0 += KB_BLK_RC) {
-            const int rc = min(KB_BLK_RC, nrhs - c0);
-            const int below = n - (k0 + kb);
-            // y_p -= sum_{i below} L(i, k0+p) x_i on the matrix cores: output rows
-            // p in two 16-row halves of the block, N = the rc right-hand sides,
-            // K = the rows below in steps of 4; wave w takes half (w & 1) and every
-            // other K step (w >> 1), and the two partial sums meet in LDS
-            {
-                const int ph = wave & 1, kh = wave >> 1;
-                const int pa = 16 * ph + li;
-                typename TileV<T>::v4 acc = {T(0), T(0), T(0), T(0)};
-                for (int r4 = 4 * kh; r4 < below; r4 += 8) {
-                    const int r = k0 + kb + r4 + lg;
-                    const bool ain = r < n && pa < kb;
-                    const T av = gld(L + (ain ? r + (long)(k0 + pa) * lda : 0));
-                    const T a = ain ? av : T(0);
-                    const bool bin = r < n && li < rc;
-                    const T b = bin ? gld(rhs_elem(B, ldb, side_left, bin ? r : 0, bin ? c0 + li : c0)) : T(0);
-                    acc = mfma16x16x4(a, b, acc);
-                }
-                T* sR = S.sB;   // partial sums of waves 2 and 3: 2 x 4 x 64
-                if (kh == 1) {
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) sR[(ph * 4 + q) * 64 + lane] = acc[q];
-                }
-                __syncthreads();
-                if (kh == 0 && below > 0) {
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const int po = 16 * ph + blk_drow<T>(q, lg);
-                        if (po < kb && li < rc) {
-                            T* bp = rhs_elem(B, ldb, side_left, k0 + po, c0 + li);
-                            gst(bp, gld(bp) - (acc[q] + sR[(ph * 4 + q) * 64 + lane]));
-                        }
-                    }
-                }
-                __syncthreads();
-            }
-            {
-                const int c = 4 * wave + lg;
-                const bool act = c < rc;
-                T y[2], iv[2];
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int r = 16 * h + li;
-                    const bool in = act && r < kb;
-                    y[h] = in ? gld(rhs_elem(B, ldb, side_left, in ? k0 + r : 0, in ? c0 + c : 0)) : T(0);
-                    iv[h] = S.sInv[r];
-                }
-#pragma unroll
-                for (int j = NBK - 1; j >= 0; --j) {
-                    const T t = y[j >> 4] * S.sInv[j];      // lane j: x(j)
-#pragma unroll
-                    for (int h = 0; h < 2; ++h) {
-                        const int r = 16 * h + li;
-                        if (16 * h < j) {
-                            const T l = (r < j) ? S.sD[j * (NBK + 1) + r] : T(0);
-                            fnmac_rowk<T>(y[h], t, l, j);
-                        }
-                    }
-                }
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int r = 16 * h + li;
-                    if (act && r < kb) gst(rhs_elem(B, ldb, side_left, k0 + r, c0 + c), y[h] * iv[h]);
-                }
-            }
+        for (int c0 = 0; c0 < nrhs; c0 += RC) {
+            const int rc = min(RC, nrhs - c0);
+            const int ncb = (rc + 15) / 16;
+            blk_diag_subst_n<T, false>(ncb, B, ldb, side_left, k0, kb, c0, rc, S, sY);
+            __syncthreads();
+            blk_update_rows_n<T, true>(ncb, L, lda, B, ldb, side_left, k0, kb, 0, k0, c0, rc, sY);
             __syncthreads();
         }
     }

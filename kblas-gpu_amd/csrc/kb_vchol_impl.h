@@ -1,4 +1,4 @@
-// kb_vchol_impl.h -- non-uniform batched Cholesky family in one persistent launch.
+// kb_vchol_impl.h -- non-uniform batched Cholesky family over a size-sorted queue.
 //
 // The reference has no non-uniform POTRF/POSV; its non-uniform TRSM/SYRK go
 // through MAGMA with a host round trip for the maximum sizes
@@ -6,18 +6,24 @@
 //   1. a device counting sort orders the matrix ids by order n, largest
 //      first (clear -> histogram -> scan -> scatter; no host synchronisation,
 //      no allocation, fills done by kernels so the sequence can be captured in
-//      a graph);
-//   2. one persistent kernel pulls tickets from that queue with an atomic
-//      counter.  Matrices above the register range are factored by the whole
-//      256-thread workgroup (blocked algorithm); once the queue reaches the
-//      register range every wavefront takes its own tickets and runs the
-//      register-resident routine on one matrix.
+//      a graph); the scan also records how many orders exceed the register
+//      range;
+//   2. a persistent kernel of 256-thread workgroups pulls the tickets of those
+//      large matrices from the front of the queue (one workgroup per matrix,
+//      blocked algorithm, largest first);
+//   3. a second persistent kernel, queued behind it, gives every wavefront its
+//      own tickets from the rest of the queue (register-resident routine, one
+//      matrix per wavefront).  The two phases are separate kernels so each is
+//      compiled and occupied for its own register and LDS needs.
 #pragma once
 #include "kb_blk_impl.h"
 #include "kb_reg_impl.h"
 
 constexpr int KB_VQ_BINS = 4096;   // orders >= 4095 share the last bin
 constexpr int KB_VQ_HDR = 2 * KB_VQ_BINS + 64;   // ints: hist, offsets, counters
+// counter[0]: next ticket of the workgroup phase; counter[1]: number of orders
+// above the register range (tickets [0, counter[1])); counter[2]: next ticket
+// of the wavefront phase (starts at counter[1])
 
 struct VQueue {
     int* hist;
@@ -87,6 +93,10 @@ __global__ void __launch_bounds__(1024) k_vq_scan(void* ws, int batch)
     for (int j = 0; j < PER; ++j) {
         const int b = KB_VQ_BINS - 1 - (t * PER + j);
         q.offs[b] = run;
+        if (b == KB_REG_MAX_N) {   // run = number of orders above the register range
+            q.counter[1] = run;
+            q.counter[2] = run;
+        }
         run += q.hist[b];
     }
 }
@@ -101,16 +111,6 @@ __global__ void k_vq_scatter(void* ws, const int* vm, const int* vn, int side_le
 }
 
 }  // namespace
-
-// Shared memory of the persistent kernel: the blocked workgroup routine or,
-// in the register phase, one float LDS area per wavefront (only the FP32 and
-// mixed routines use it; the FP64 ones read their transposed factor back from
-// memory, and leaving the area out lets more workgroups share a CU).
-template <typename T, bool LT>
-union VSmem {
-    BlkSmem<T> blk;
-    float lt[4][LT ? 64 * 65 : 1];
-};
 
 template <typename T, int MODE>
 KB_DEV void v_small(long mat, int n, int nrhs, int side_left, BatchArg<T>& A, BatchArg<T>& B,
@@ -136,9 +136,133 @@ KB_DEV void v_small(long mat, int n, int nrhs, int side_left, BatchArg<T>& A, Ba
     }
 }
 
+// Workgroup phase: the matrices above the register range, one workgroup each.
 template <typename T, int MODE>
 __global__ void __launch_bounds__(KB_BLK_WG) __attribute__((amdgpu_waves_per_eu(2)))
-k_vchol(const int* vm, const int* vn, int side_left, BatchArg<T> A, BatchArg<T> B, BatchArg<T> X,
+k_vchol_blk(const int* vm, const int* vn, int side_left, BatchArg<T> A, BatchArg<T> B, BatchArg<T> X,
+            int* info, int* iter, int batch, void* ws)
+{
+    __shared__ BlkSmem<T> S;
+    __shared__ int s_ticket;
+    VQueue q = vq_view(ws, batch);
+    const int nbig = q.counter[1];
+    for (;;) {
+        if (threadIdx.x == 0) s_ticket = atomicAdd(q.counter, 1);
+        __syncthreads();
+        const int ticket = s_ticket;
+        __syncthreads();
+        if (ticket >= nbig) return;
+        const long mat = q.order[ticket];
+        const int n = vq_order(vm, vn, side_left, mat);
+        const int nrhs = vq_nrhs(vm, vn, side_left, mat);
+        if constexpr (MODE == KB_MODE_SPOSV) {
+            // Orders above the register range are solved in FP64 directly
+            // (LAPACK dsposv's ITER = -1: full precision for implementation
+            // reasons); A receives L as in the dsposv fallback.
+            T* Ap = A.get(mat);
+            const int lda = A.ldim(mat);
+            T* Bp = B.get(mat);
+            T* Xp = X.get(mat);
+            const int ldb = B.ldim(mat), ldx = X.ldim(mat);
+            for (int e = threadIdx.x; e < n * nrhs; e += KB_BLK_WG)
+                gst(rhs_elem(Xp, ldx, side_left, e % n, e / n), gld(rhs_elem(Bp, ldb, side_left, e % n, e / n)));
+            __syncthreads();
+            const int inf = blk_potrf<T>(Ap, lda, n, S);
+            if (threadIdx.x == 0) { if (info) info[mat] = inf; if (iter) iter[mat] = -1; }
+            if (inf == 0) blk_solve<T>(Ap, lda, n, Xp, ldx, side_left, nrhs, S);
+        } else {
+            T* Ap = A.get(mat);
+            const int lda = A.ldim(mat);
+            int inf = 0;
+            if constexpr (MODE != KB_MODE_POTRS) {
+                inf = blk_potrf<T>(Ap, lda, n, S);
+                if (threadIdx.x == 0 && info) info[mat] = inf;
+            }
+            if constexpr (MODE != KB_MODE_POTRF) {
+                if (inf == 0) blk_solve<T>(Ap, lda, n, B.get(mat), B.ldim(mat), side_left, nrhs, S);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Wavefront phase: the rest of the queue, one matrix per wavefront.  Only the
+// FP32 and mixed routines use the per-wavefront LDS area (their transposed
+// factor); the FP64 ones read it back from memory.
+template <typename T, int MODE>
+__global__ void __launch_bounds__(KB_BLK_WG)
+k_vchol_small(const int* vm, const int* vn, int side_left, BatchArg<T> A, BatchArg<T> B, BatchArg<T> X,
+              int* info, int* iter, int batch, void* ws)
+{
+    constexpr bool LT = sizeof(T) == 4 || MODE == KB_MODE_SPOSV;
+    __shared__ float lts[4][LT ? 64 * 65 : 1];
+    VQueue q = vq_view(ws, batch);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    float* lt = lts[wave];
+    for (;;) {
+        int t = 0;
+        if (lane == 0) t = atomicAdd(q.counter + 2, 1);
+        const int ticket = __builtin_amdgcn_readfirstlane(t);
+        if (ticket >= batch) return;
+        const long mat = q.order[ticket];
+        const int n = vq_order(vm, vn, side_left, mat);
+        const int nrhs = vq_nrhs(vm, vn, side_left, mat);
+        v_small<T, MODE>(mat, n, nrhs, side_left, A, B, X, info, iter, lt, lane);
+    }
+}
+
+template <typename K>
+static int vchol_grid(K kernel, int batch)
+{
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+        cus = 256;
+    // persistent grid: as many workgroups as can be resident at once
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, KB_BLK_WG, 0) != hipSuccess || per_cu < 1)
+        per_cu = 2;
+    const int grid = cus * per_cu;
+    return grid > batch ? batch : grid;
+}
+
+template <typename T, int MODE>
+static int launch_vchol(const int* vm, const int* vn, int side_left, kb_mat_t A, kb_mat_t B, kb_mat_t X,
+                        int* info, int* iter, int batch, void* ws, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_vq_clear, dim3(8), dim3(1024), 0, s, ws);
+    const int gb = (batch + 255) / 256 < 1024 ? (batch + 255) / 256 : 1024;
+    hipLaunchKernelGGL(k_vq_hist, dim3(gb), dim3(256), 0, s, ws, vm, vn, side_left, batch);
+    hipLaunchKernelGGL(k_vq_scan, dim3(1), dim3(1024), 0, s, ws, batch);
+    hipLaunchKernelGGL(k_vq_scatter, dim3(gb), dim3(256), 0, s, ws, vm, vn, side_left, batch);
+    const BatchArg<T> a = to_arg<T>(A), b = to_arg<T>(B), x = to_arg<T>(X);
+    hipLaunchKernelGGL((k_vchol_blk<T, MODE>), dim3(vchol_grid(k_vchol_blk<T, MODE>, batch)), dim3(KB_BLK_WG), 0, s,
+                       vm, vn, side_left, a, b, x, info, iter, batch, ws);
+    hipLaunchKernelGGL((k_vchol_small<T, MODE>), dim3(vchol_grid(k_vchol_small<T, MODE>, (batch + 3) / 4)),
+                       dim3(KB_BLK_WG), 0, s, vm, vn, side_left, a, b, x, info, iter, batch, ws);
+    return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision (SPOSV): both phases in one persistent kernel of 256-thread
+// workgroups (workgroup phase first; once the queue reaches the register
+// range every wavefront takes its own tickets).  The two-kernel split above
+// hung on the GPU for this mode with its cause not yet found, so the mixed
+// mode keeps this form.
+// ---------------------------------------------------------------------------
+// Shared memory of the persistent kernel: the blocked workgroup routine or,
+// in the register phase, one float LDS area per wavefront (only the FP32 and
+// mixed routines use it; the FP64 ones read their transposed factor back from
+// memory, and leaving the area out lets more workgroups share a CU).
+template <typename T, bool LT>
+union VSmem {
+    BlkSmem<T> blk;
+    float lt[4][LT ? 64 * 65 : 1];
+};
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(KB_BLK_WG) __attribute__((amdgpu_waves_per_eu(2)))
+k_vchol_mixed(const int* vm, const int* vn, int side_left, BatchArg<T> A, BatchArg<T> B, BatchArg<T> X,
         int* info, int* iter, int batch, void* ws)
 {
     __shared__ VSmem<T, (sizeof(T) == 4 || MODE == KB_MODE_SPOSV)> S;
@@ -199,13 +323,15 @@ k_vchol(const int* vm, const int* vn, int side_left, BatchArg<T> A, BatchArg<T> 
         const long mat = q.order[ticket];
         const int n = vq_order(vm, vn, side_left, mat);
         const int nrhs = vq_nrhs(vm, vn, side_left, mat);
-        v_small<T, MODE>(mat, n, nrhs, side_left, A, B, X, info, iter, lt, lane);
+        int ln = lane;
+        asm volatile("" : "+v"(ln));   // lane-dependent values formed per matrix, not kept live across phases
+        v_small<T, MODE>(mat, n, nrhs, side_left, A, B, X, info, iter, lt, ln);
         ticket = -1;
     }
 }
 
 template <typename T, int MODE>
-static int launch_vchol(const int* vm, const int* vn, int side_left, kb_mat_t A, kb_mat_t B, kb_mat_t X,
+static int launch_vchol_mixed(const int* vm, const int* vn, int side_left, kb_mat_t A, kb_mat_t B, kb_mat_t X,
                         int* info, int* iter, int batch, void* ws, hipStream_t s)
 {
     hipLaunchKernelGGL(k_vq_clear, dim3(8), dim3(1024), 0, s, ws);
@@ -219,12 +345,12 @@ static int launch_vchol(const int* vm, const int* vn, int side_left, kb_mat_t A,
         cus = 256;
     // persistent grid: as many workgroups as can be resident at once
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_vchol<T, MODE>, KB_BLK_WG, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_vchol_mixed<T, MODE>, KB_BLK_WG, 0) != hipSuccess ||
         per_cu < 1)
         per_cu = 2;
     int grid = cus * per_cu;
     if (grid > batch) grid = batch;
-    hipLaunchKernelGGL((k_vchol<T, MODE>), dim3(grid), dim3(KB_BLK_WG), 0, s, vm, vn, side_left,
+    hipLaunchKernelGGL((k_vchol_mixed<T, MODE>), dim3(grid), dim3(KB_BLK_WG), 0, s, vm, vn, side_left,
                        to_arg<T>(A), to_arg<T>(B), to_arg<T>(X), info, iter, batch, ws);
     return (int)hipGetLastError();
 }
