@@ -6,11 +6,14 @@
 // recursion of separate launches (src/batch_triangular/Xpotrf_batch_drivers.cuh:91-131),
 // here inside one kernel so every stage of a matrix reuses the workgroup's
 // LDS and the matrix's cache lines:
-//   - the 32x32 diagonal block is factored in VGPRs by wave 0 (row per lane,
-//     v_readlane broadcasts), copied to LDS with its inverse diagonal;
+//   - the 32x32 diagonal block is factored on one DPP row of wave 0 and
+//     copied to LDS with its inverse diagonal;
 //   - the panel below is solved one row per thread against the LDS block;
-//   - the trailing lower triangle is updated in 64x64 tiles (4x4 per thread)
-//     from two LDS-staged 64x32 slices of the panel.
+//   - panels go in pairs: the second panel's columns are updated with the
+//     first, and the trailing lower triangle then receives one update of
+//     depth 64 (both panels) on the matrix cores, in 64x64 tiles from two
+//     LDS-staged 64x64 slices, so the trailing matrix is read and written
+//     once per 64 columns instead of once per 32.
 // The solve (POSV/POTRS) runs blocked forward and backward sweeps over up to
 // 64 right-hand sides per pass (so L is streamed once per sweep for up to 64
 // right-hand sides): the diagonal blocks by DPP-row substitutions, the
@@ -22,6 +25,7 @@
 #include "kb_rg_impl.h"
 
 constexpr int KB_BLK_NBK = 32;
+constexpr int KB_BLK_KW = 2 * KB_BLK_NBK;   // depth of a trailing update (two panels)
 constexpr int KB_BLK_WG = 256;
 constexpr int KB_BLK_TILE = 64;
 constexpr int KB_BLK_RC = 64;   // right-hand sides per pass of the solve
@@ -30,8 +34,8 @@ template <typename T>
 struct BlkSmem {
     T sD[KB_BLK_NBK * (KB_BLK_NBK + 1)];
     T sInv[KB_BLK_NBK];
-    T sA[KB_BLK_TILE * (KB_BLK_NBK + 1)];
-    T sB[KB_BLK_TILE * (KB_BLK_NBK + 1)];
+    T sA[KB_BLK_TILE * (KB_BLK_KW + 1)];
+    T sB[KB_BLK_TILE * (KB_BLK_KW + 1)];
     int flag;
 };
 
@@ -42,35 +46,37 @@ KB_DEV T& rhs_at(T* B, int ldb, int side_left, int i, int c)
     return side_left ? B[i + (long)c * ldb] : B[c + (long)i * ldb];
 }
 
-// C (m2 x m2, lower) -= L21 L21^T, L21 is m2 x kb (ld lda), on the matrix
-// cores.  C is walked in 64 x 64 tiles; the 64-row slices of L21 are staged in
-// LDS (slice I once per row of tiles).  Wave w computes rows 16w..16w+15 of a
-// tile as four 16 x 16 products D' = L21_J(16 x kb) L21_I(16 x kb)^T, i.e.
-// the transpose of the C block, whose accumulator layout puts consecutive C
-// rows in consecutive lanes (coalesced column-major read-modify-write):
+// C (m2 x m2, lower; only its first ncol columns) -= L21 L21^T, L21 is
+// m2 x kb (kb <= 64, ld lda), on the matrix cores.  C is walked in 64 x 64
+// tiles; the 64-row slices of L21 are staged in LDS (slice I once per row of
+// tiles).  Wave w computes rows 16w..16w+15 of a tile as four 16 x 16 products
+// D' = L21_J(16 x kb) L21_I(16 x kb)^T, i.e. the transpose of the C block,
+// whose accumulator layout puts consecutive C rows in consecutive lanes
+// (coalesced column-major read-modify-write):
 //   FP64: lane (g, i), register r -> C(16w + i, 16q + 4r + g)
 //   FP32: lane (g, i), register r -> C(16w + i, 16q + 4g + r)
 template <typename T>
-KB_DEV void blk_syrk_update(const T* L21, T* C, int lda, int m2, int kb, BlkSmem<T>& S)
+KB_DEV void blk_syrk_update(const T* L21, T* C, int lda, int m2, int kb, BlkSmem<T>& S, int ncol)
 {
-    constexpr int NBK = KB_BLK_NBK, TL = KB_BLK_TILE;
+    constexpr int KW = KB_BLK_KW, TL = KB_BLK_TILE;
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));   // lane-dependent values formed per call, not kept live across
     const int wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lg = lane >> 4;
     const int nt = (m2 + TL - 1) / TL;
+    const int kb4 = (kb + 3) & ~3;
     for (int I = 0; I < nt; ++I) {
-        for (int e = tid; e < TL * NBK; e += KB_BLK_WG) {
+        for (int e = tid; e < TL * KW; e += KB_BLK_WG) {
             const int r = e % TL, p = e / TL;
             const int gi = I * TL + r;
-            S.sA[r * (NBK + 1) + p] = (gi < m2 && p < kb) ? gld(L21 + gi + (long)p * lda) : T(0);
+            if (p < kb4) S.sA[r * (KW + 1) + p] = (gi < m2 && p < kb) ? gld(L21 + gi + (long)p * lda) : T(0);
         }
-        for (int J = 0; J <= I; ++J) {
+        for (int J = 0; J <= I && J * TL < ncol; ++J) {
             if (J != I) {
-                for (int e = tid; e < TL * NBK; e += KB_BLK_WG) {
+                for (int e = tid; e < TL * KW; e += KB_BLK_WG) {
                     const int r = e % TL, p = e / TL;
                     const int gj = J * TL + r;
-                    S.sB[r * (NBK + 1) + p] = (gj < m2 && p < kb) ? gld(L21 + gj + (long)p * lda) : T(0);
+                    if (p < kb4) S.sB[r * (KW + 1) + p] = (gj < m2 && p < kb) ? gld(L21 + gj + (long)p * lda) : T(0);
                 }
             }
             __syncthreads();
@@ -79,6 +85,7 @@ KB_DEV void blk_syrk_update(const T* L21, T* C, int lda, int m2, int kb, BlkSmem
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 if (J == I && q > wave) continue;          // above the diagonal
+                if (J * TL + 16 * q >= ncol) continue;     // beyond the columns asked for
                 int col[4];
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
@@ -87,19 +94,19 @@ KB_DEV void blk_syrk_update(const T* L21, T* C, int lda, int m2, int kb, BlkSmem
                 T cv[4];
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const bool in = gi < m2 && col[r] <= gi;
+                    const bool in = gi < m2 && col[r] <= gi && col[r] < ncol;
                     const T v = gld(C + (in ? gi + (long)col[r] * lda : 0));
                     cv[r] = in ? v : T(0);
                 }
                 typename TileV<T>::v4 acc = {T(0), T(0), T(0), T(0)};
                 for (int p0 = 0; p0 < kb; p0 += 4) {
-                    const T a = sJ[(16 * q + li) * (NBK + 1) + p0 + lg];
-                    const T b = S.sA[(16 * wave + li) * (NBK + 1) + p0 + lg];
+                    const T a = sJ[(16 * q + li) * (KW + 1) + p0 + lg];
+                    const T b = S.sA[(16 * wave + li) * (KW + 1) + p0 + lg];
                     acc = mfma16x16x4(a, b, acc);
                 }
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
-                    if (gi < m2 && col[r] <= gi) gst(C + gi + (long)col[r] * lda, cv[r] - acc[r]);
+                    if (gi < m2 && col[r] <= gi && col[r] < ncol) gst(C + gi + (long)col[r] * lda, cv[r] - acc[r]);
             }
             __syncthreads();
         }
@@ -111,7 +118,9 @@ template <typename T>
 KB_DEV void blk_stage_diag(const T* L, int lda, int k0, int kb, BlkSmem<T>& S)
 {
     constexpr int NBK = KB_BLK_NBK;
-    for (int e = threadIdx.x; e < NBK * NBK; e += KB_BLK_WG) {
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));   // lane-dependent values formed per call, not kept live across
+    for (int e = tid; e < NBK * NBK; e += KB_BLK_WG) {
         const int r = e % NBK, c = e / NBK;
         S.sD[r * (NBK + 1) + c] = (r < kb && c <= r) ? gld(L + (k0 + r) + (long)(k0 + c) * lda) : T(0);
     }
@@ -122,55 +131,86 @@ KB_DEV void blk_stage_diag(const T* L, int lda, int k0, int kb, BlkSmem<T>& S)
     __syncthreads();
 }
 
+// One 32-column panel at k0 (kb columns, already updated by every panel to
+// its left): its diagonal block factored by wave 0 (L_D also goes to LDS for
+// the panel solve), then the rows below solved one row per thread,
+// X L_D^T = A_panel.  Returns LAPACK info (0, or k0 + the failing column).
+template <typename T>
+KB_DEV int blk_panel(T* A, int lda, int n, int k0, int kb, BlkSmem<T>& S)
+{
+    constexpr int NBK = KB_BLK_NBK;
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));   // lane-dependent values formed per panel, not kept live across
+    const int wave = tid >> 6, lane = tid & 63;
+    if (wave == 0) {
+        // the 32 x 32 diagonal block on one DPP row of wave 0 (lane l: rows
+        // l and 16 + l), fused DPP FMAs; the other rows of the wave repeat
+        // the work without writing
+        T* D = A + k0 + (long)k0 * lda;
+        T a[2][NBK];
+        T dinv2[2] = {T(1), T(1)};
+        rg_load<T, NBK, false>(a, D, lda, kb, lane & 15);
+        const int inf = rg_potrf<T, NBK, false, false>(a, dinv2, kb, lane & 15, D, lda, lane < 16, S.sD, S.sInv);
+        if (lane == 0) S.flag = inf ? k0 + inf : 0;
+    }
+    __syncthreads();
+    const int f = S.flag;
+    if (f) return f;
+    for (int i = k0 + kb + tid; i < n; i += KB_BLK_WG) {
+        T x[NBK];
+        // walk the row with a running pointer (made opaque each step), so the
+        // 32 column addresses are not precomputed and kept live across rows
+        T* p = A + i + (long)k0 * lda;
+#pragma unroll
+        for (int c = 0; c < NBK; ++c) {
+            x[c] = (c < kb) ? gld(p) : T(0);
+            p += lda;
+            asm volatile("" : "+v"(p));
+        }
+#pragma unroll
+        for (int j = 0; j < NBK; ++j) {
+            if (j < kb) {
+                x[j] = x[j] * S.sInv[j];
+#pragma unroll
+                for (int c = j + 1; c < NBK; ++c) x[c] = kb_fma<T>(-x[j], S.sD[c * (NBK + 1) + j], x[c]);
+            }
+            // keep each step's LDS reads with the step (hoisting them all would
+            // spill the 32-element row)
+            asm volatile("" ::: "memory");
+        }
+        p = A + i + (long)k0 * lda;
+#pragma unroll
+        for (int c = 0; c < NBK; ++c) {
+            if (c < kb) gst(p, x[c]);
+            p += lda;
+            asm volatile("" : "+v"(p));
+        }
+    }
+    __syncthreads();
+    return 0;
+}
+
 // In-place lower Cholesky of an n x n matrix by the whole workgroup.
 // Returns LAPACK info (identical in every thread).
 template <typename T>
 KB_DEV int blk_potrf(T* A, int lda, int n, BlkSmem<T>& S)
 {
     constexpr int NBK = KB_BLK_NBK;
-    for (int k0 = 0; k0 < n; k0 += NBK) {
-        int tid = threadIdx.x;
-        asm volatile("" : "+v"(tid));   // lane-dependent values formed per panel, not kept live across
-        const int wave = tid >> 6, lane = tid & 63;
-        const int kb = min(NBK, n - k0);
-        if (wave == 0) {
-            // the 32 x 32 diagonal block on one DPP row of wave 0 (lane l: rows
-            // l and 16 + l), fused DPP FMAs; the other rows of the wave repeat
-            // the work without writing
-            T* D = A + k0 + (long)k0 * lda;
-            T a[2][NBK];
-            T dinv2[2] = {T(1), T(1)};
-            rg_load<T, NBK, false>(a, D, lda, kb, lane & 15);
-            // L_D also goes to LDS (sD, sInv) for the panel solve below
-            const int inf = rg_potrf<T, NBK, false, false>(a, dinv2, kb, lane & 15, D, lda, lane < 16, S.sD, S.sInv);
-            if (lane == 0) S.flag = inf ? k0 + inf : 0;
-        }
-        __syncthreads();
-        const int f = S.flag;
+    for (int k0 = 0; k0 < n; k0 += 2 * NBK) {
+        const int kb0 = min(NBK, n - k0);
+        int f = blk_panel<T>(A, lda, n, k0, kb0, S);
         if (f) return f;
-        // panel below the diagonal block: X L_D^T = A_panel, one row per thread
-        for (int i = k0 + kb + tid; i < n; i += KB_BLK_WG) {
-            T x[NBK];
-#pragma unroll
-            for (int c = 0; c < NBK; ++c) x[c] = (c < kb) ? gld(A + i + (long)(k0 + c) * lda) : T(0);
-#pragma unroll
-            for (int j = 0; j < NBK; ++j) {
-                if (j < kb) {
-                    x[j] = x[j] * S.sInv[j];
-#pragma unroll
-                    for (int c = j + 1; c < NBK; ++c) x[c] = kb_fma<T>(-x[j], S.sD[c * (NBK + 1) + j], x[c]);
-                }
-                // keep each step's LDS reads with the step (hoisting them all would
-                // spill the 32-element row)
-                asm volatile("" ::: "memory");
-            }
-#pragma unroll
-            for (int c = 0; c < NBK; ++c)
-                if (c < kb) gst(A + i + (long)(k0 + c) * lda, x[c]);
-        }
+        const int k1 = k0 + kb0;
+        if (k1 >= n) break;
+        const int kb1 = min(NBK, n - k1);
+        // the second panel's columns, updated with the first
+        blk_syrk_update<T>(A + k1 + (long)k0 * lda, A + k1 + (long)k1 * lda, lda, n - k1, kb0, S, kb1);
         __syncthreads();
-        const int t0 = k0 + kb;
-        if (t0 < n) blk_syrk_update<T>(A + t0 + (long)k0 * lda, A + t0 + (long)t0 * lda, lda, n - t0, kb, S);
+        f = blk_panel<T>(A, lda, n, k1, kb1, S);
+        if (f) return f;
+        // the trailing matrix, with both panels at once (depth kb0 + kb1)
+        const int t0 = k1 + kb1;
+        if (t0 < n) blk_syrk_update<T>(A + t0 + (long)k0 * lda, A + t0 + (long)t0 * lda, lda, n - t0, kb0 + kb1, S, n - t0);
         __syncthreads();
     }
     return 0;
