@@ -6,8 +6,9 @@
 // recursion of separate launches (src/batch_triangular/Xpotrf_batch_drivers.cuh:91-131),
 // here inside one kernel so every stage of a matrix reuses the workgroup's
 // LDS and the matrix's cache lines:
-//   - the 32x32 diagonal block is factored on one DPP row of wave 0 and
-//     copied to LDS with its inverse diagonal;
+//   - the 32x32 diagonal block is factored by wave 0 in registers as three
+//     16x16 tiles (4-column panels on DPP rows, rank-4 updates on the matrix
+//     cores), written to memory and to LDS with its inverse diagonal;
 //   - the panel below is solved one row per thread against the LDS block;
 //   - panels go in pairs: the second panel's columns are updated with the
 //     first, and the trailing lower triangle then receives one update of
@@ -23,6 +24,7 @@
 #include "kb_launch.h"
 #include "kb_reg_impl.h"
 #include "kb_rg_impl.h"
+#include "kb_tile_impl.h"
 
 constexpr int KB_BLK_NBK = 32;
 constexpr int KB_BLK_KW = 2 * KB_BLK_NBK;   // depth of a trailing update (two panels)
@@ -34,6 +36,7 @@ template <typename T>
 struct BlkSmem {
     T sD[KB_BLK_NBK * (KB_BLK_NBK + 1)];
     T sInv[KB_BLK_NBK];
+    T tpan[4][16][4];   // panel transposes of the diagonal-block factorization
     T sA[KB_BLK_TILE * (KB_BLK_KW + 1)];
     T sB[KB_BLK_TILE * (KB_BLK_KW + 1)];
     int flag;
@@ -143,14 +146,41 @@ KB_DEV int blk_panel(T* A, int lda, int n, int k0, int kb, BlkSmem<T>& S)
     asm volatile("" : "+v"(tid));   // lane-dependent values formed per panel, not kept live across
     const int wave = tid >> 6, lane = tid & 63;
     if (wave == 0) {
-        // the 32 x 32 diagonal block on one DPP row of wave 0 (lane l: rows
-        // l and 16 + l), fused DPP FMAs; the other rows of the wave repeat
-        // the work without writing
+        // the 32 x 32 diagonal block as tiles (0,0), (1,0), (1,1): lane (i, g)
+        // holds row i and columns 4g..4g+3 of each; rows and columns at or
+        // beyond kb read as the identity
         T* D = A + k0 + (long)k0 * lda;
-        T a[2][NBK];
-        T dinv2[2] = {T(1), T(1)};
-        rg_load<T, NBK, false>(a, D, lda, kb, lane & 15);
-        const int inf = rg_potrf<T, NBK, false, false>(a, dinv2, kb, lane & 15, D, lda, lane < 16, S.sD, S.sInv);
+        const int i = lane & 15, g = lane >> 4;
+        typename TileV<T>::v4 td[3];
+#pragma unroll
+        for (int I = 0; I < 2; ++I)
+#pragma unroll
+            for (int J = 0; J <= I; ++J)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = 16 * I + i, col = 16 * J + 4 * g + r;
+                    const bool in = row < kb && col <= row;
+                    const T v = gld(D + (in ? row + (long)col * lda : 0));
+                    td[tix(I, J)][r] = in ? v : ((row == col) ? T(1) : T(0));
+                }
+        T invd[2] = {T(1), T(1)};
+        const int inf = tile_potrf<T, 2>(td, invd, kb, S.tpan, lane);
+        // L_D to memory (lower part), to LDS for the panel solve below, and
+        // 1/L(j, j) to sInv
+#pragma unroll
+        for (int I = 0; I < 2; ++I)
+#pragma unroll
+            for (int J = 0; J <= I; ++J)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = 16 * I + i, col = 16 * J + 4 * g + r;
+                    if (row < kb && col <= row) gst(D + row + (long)col * lda, td[tix(I, J)][r]);
+                    S.sD[row * (NBK + 1) + col] = (col <= row) ? td[tix(I, J)][r] : T(0);
+                }
+        if (g == 0) {
+            S.sInv[i] = invd[0];
+            S.sInv[16 + i] = invd[1];
+        }
         if (lane == 0) S.flag = inf ? k0 + inf : 0;
     }
     __syncthreads();

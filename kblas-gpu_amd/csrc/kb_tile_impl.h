@@ -84,15 +84,17 @@ KB_DEV void tile_store(const typename TileV<T>::v4 (&t)[10], T* A, int lda, int 
     }
 }
 
-// In-register Cholesky of the (padded) 64 x 64 matrix.  invd[K] receives
-// 1/L(16K+i, 16K+i) in every lane with (lane & 15) == i.  Returns LAPACK info.
-template <typename T>
-KB_DEV int tile_potrf(typename TileV<T>::v4 (&t)[10], T (&invd)[4], int n, T (&pan)[4][16][4], int lane)
+// In-register Cholesky of the (padded) 16NT x 16NT matrix (NT = 4: 64 x 64;
+// NT = 2: a 32 x 32 block).  invd[K] receives 1/L(16K+i, 16K+i) in every lane
+// with (lane & 15) == i.  Returns LAPACK info.
+template <typename T, int NT = 4>
+KB_DEV int tile_potrf(typename TileV<T>::v4 (&t)[NT * (NT + 1) / 2], T (&invd)[NT], int n, T (&pan)[4][16][4],
+                      int lane)
 {
     const int i = lane & 15, g = lane >> 4;
     int info = 0;
 #pragma unroll
-    for (int J = 0; J < 4; ++J) {
+    for (int J = 0; J < NT; ++J) {
 #pragma unroll
         for (int gp = 0; gp < 4; ++gp) {
             if (info) break;
@@ -110,7 +112,7 @@ KB_DEV int tile_potrf(typename TileV<T>::v4 (&t)[10], T (&invd)[4], int n, T (&p
                 if (i == 4 * gp + c) invd[J] = rs;
                 if (g == gp) {
 #pragma unroll
-                    for (int I = J; I < 4; ++I) {
+                    for (int I = J; I < NT; ++I) {
                         T v = t[tix(I, J)][c] * rs;
                         if (I == J && i == 4 * gp + c) v = s;
                         t[tix(I, J)][c] = v;
@@ -118,7 +120,7 @@ KB_DEV int tile_potrf(typename TileV<T>::v4 (&t)[10], T (&invd)[4], int n, T (&p
 #pragma unroll
                     for (int cp = c + 1; cp < 4; ++cp) {
 #pragma unroll
-                        for (int I = J; I < 4; ++I) {
+                        for (int I = J; I < NT; ++I) {
                             // t(I,J)[cp] -= t(I,J)[c] * t(J,J)[c] of lane 4 gp + cp
                             T acc = t[tix(I, J)][cp];
                             fnmac_rowk<T>(acc, t[tix(J, J)][c], t[tix(I, J)][c], 4 * gp + cp);
@@ -131,7 +133,7 @@ KB_DEV int tile_potrf(typename TileV<T>::v4 (&t)[10], T (&invd)[4], int n, T (&p
             // ---- transpose the panel: lane x + 16k receives P(X)[x][k] ----
             if (g == gp) {
 #pragma unroll
-                for (int X = J; X < 4; ++X)
+                for (int X = J; X < NT; ++X)
 #pragma unroll
                     for (int k = 0; k < 4; ++k) pan[X][i][k] = t[tix(X, J)][k];
             }
@@ -142,21 +144,21 @@ KB_DEV int tile_potrf(typename TileV<T>::v4 (&t)[10], T (&invd)[4], int n, T (&p
             // interleaves rows, D[i][j] at lane j + 16(i%4), register i/4, so
             // its A operand row i must carry tile column sig(i) = 4(i%4) + i/4.
             const int ci = (sizeof(T) == 8) ? 4 * (i & 3) + (i >> 2) : i;
-            T ts[4], ta[4];
+            T ts[NT], ta[NT];
 #pragma unroll
-            for (int X = 0; X < 4; ++X) {
+            for (int X = 0; X < NT; ++X) {
                 ts[X] = (X >= J) ? pan[X][i][g] : T(0);
                 ta[X] = (X >= J) ? pan[X][ci][g] : T(0);
             }
             wave_lds_sync();
             // ---- rank-4 trailing update on the matrix cores ----
 #pragma unroll
-            for (int Jp = J; Jp < 4; ++Jp) {
+            for (int Jp = J; Jp < NT; ++Jp) {
                 if (Jp == J && gp == 3) continue;             // tile column finished
                 T a = -ta[Jp];
                 if (Jp == J && ci < 4 * (gp + 1)) a = T(0);   // columns already factored
 #pragma unroll
-                for (int Ip = Jp; Ip < 4; ++Ip) t[tix(Ip, Jp)] = mfma16x16x4(a, ts[Ip], t[tix(Ip, Jp)]);
+                for (int Ip = Jp; Ip < NT; ++Ip) t[tix(Ip, Jp)] = mfma16x16x4(a, ts[Ip], t[tix(Ip, Jp)]);
             }
         }
         if (info) break;
