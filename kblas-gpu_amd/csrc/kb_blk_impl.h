@@ -12,9 +12,9 @@
 //   - the panel below is solved one row per thread against the LDS block;
 //   - panels go in pairs: the second panel's columns are updated with the
 //     first, and the trailing lower triangle then receives one update of
-//     depth 64 (both panels) on the matrix cores, in 64x64 tiles from two
-//     LDS-staged 64x64 slices, so the trailing matrix is read and written
-//     once per 64 columns instead of once per 32.
+//     depth 64 (both panels) on the matrix cores, in 64x64 tiles (one 64x64
+//     slice staged in LDS, the other operand in registers), so the trailing
+//     matrix is read and written once per 64 columns instead of once per 32.
 // The solve (POSV/POTRS) runs blocked forward and backward sweeps over up to
 // 64 right-hand sides per pass (so L is streamed once per sweep for up to 64
 // right-hand sides): the diagonal blocks by DPP-row substitutions, the
@@ -37,8 +37,7 @@ struct BlkSmem {
     T sD[KB_BLK_NBK * (KB_BLK_NBK + 1)];
     T sInv[KB_BLK_NBK];
     T tpan[4][16][4];   // panel transposes of the diagonal-block factorization
-    T sA[KB_BLK_TILE * (KB_BLK_KW + 1)];
-    T sB[KB_BLK_TILE * (KB_BLK_KW + 1)];
+    T sB[KB_BLK_TILE * (KB_BLK_KW + 1)];   // a 64-row slice of the panels; the solve's staging
     int flag;
 };
 
@@ -51,13 +50,14 @@ KB_DEV T& rhs_at(T* B, int ldb, int side_left, int i, int c)
 
 // C (m2 x m2, lower; only its first ncol columns) -= L21 L21^T, L21 is
 // m2 x kb (kb <= 64, ld lda), on the matrix cores.  C is walked in 64 x 64
-// tiles; the 64-row slices of L21 are staged in LDS (slice I once per row of
-// tiles).  Wave w computes rows 16w..16w+15 of a tile as four 16 x 16 products
+// tiles.  Wave w computes rows 16w..16w+15 of a tile as four 16 x 16 products
 // D' = L21_J(16 x kb) L21_I(16 x kb)^T, i.e. the transpose of the C block,
 // whose accumulator layout puts consecutive C rows in consecutive lanes
 // (coalesced column-major read-modify-write):
 //   FP64: lane (g, i), register r -> C(16w + i, 16q + 4r + g)
 //   FP32: lane (g, i), register r -> C(16w + i, 16q + 4g + r)
+// The wave's own 16 rows of slice I (the B operand of every K step) stay in
+// registers for the whole row of tiles; slice J is staged in LDS.
 template <typename T>
 KB_DEV void blk_syrk_update(const T* L21, T* C, int lda, int m2, int kb, BlkSmem<T>& S, int ncol)
 {
@@ -69,22 +69,22 @@ KB_DEV void blk_syrk_update(const T* L21, T* C, int lda, int m2, int kb, BlkSmem
     const int nt = (m2 + TL - 1) / TL;
     const int kb4 = (kb + 3) & ~3;
     for (int I = 0; I < nt; ++I) {
-        for (int e = tid; e < TL * KW; e += KB_BLK_WG) {
-            const int r = e % TL, p = e / TL;
-            const int gi = I * TL + r;
-            if (p < kb4) S.sA[r * (KW + 1) + p] = (gi < m2 && p < kb) ? gld(L21 + gi + (long)p * lda) : T(0);
+        const int gi = I * TL + 16 * wave + li;
+        T bI[KW / 4];
+#pragma unroll
+        for (int u = 0; u < KW / 4; ++u) {
+            const int p = 4 * u + lg;
+            const bool in = gi < m2 && p < kb;
+            const T v = gld(L21 + (in ? gi + (long)p * lda : 0));
+            bI[u] = in ? v : T(0);
         }
         for (int J = 0; J <= I && J * TL < ncol; ++J) {
-            if (J != I) {
-                for (int e = tid; e < TL * KW; e += KB_BLK_WG) {
-                    const int r = e % TL, p = e / TL;
-                    const int gj = J * TL + r;
-                    if (p < kb4) S.sB[r * (KW + 1) + p] = (gj < m2 && p < kb) ? gld(L21 + gj + (long)p * lda) : T(0);
-                }
+            for (int e = tid; e < TL * KW; e += KB_BLK_WG) {
+                const int r = e % TL, p = e / TL;
+                const int gj = J * TL + r;
+                if (p < kb4) S.sB[r * (KW + 1) + p] = (gj < m2 && p < kb) ? gld(L21 + gj + (long)p * lda) : T(0);
             }
             __syncthreads();
-            const T* sJ = (J == I) ? S.sA : S.sB;
-            const int gi = I * TL + 16 * wave + li;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 if (J == I && q > wave) continue;          // above the diagonal
@@ -102,10 +102,12 @@ KB_DEV void blk_syrk_update(const T* L21, T* C, int lda, int m2, int kb, BlkSmem
                     cv[r] = in ? v : T(0);
                 }
                 typename TileV<T>::v4 acc = {T(0), T(0), T(0), T(0)};
-                for (int p0 = 0; p0 < kb; p0 += 4) {
-                    const T a = sJ[(16 * q + li) * (KW + 1) + p0 + lg];
-                    const T b = S.sA[(16 * wave + li) * (KW + 1) + p0 + lg];
-                    acc = mfma16x16x4(a, b, acc);
+#pragma unroll
+                for (int u = 0; u < KW / 4; ++u) {
+                    if (4 * u < kb) {
+                        const T a = S.sB[(16 * q + li) * (KW + 1) + 4 * u + lg];
+                        acc = mfma16x16x4(a, bI[u], acc);
+                    }
                 }
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
@@ -415,7 +417,7 @@ template <typename T>
 KB_DEV void blk_solve(const T* L, int lda, int n, T* B, int ldb, int side_left, int nrhs, BlkSmem<T>& S)
 {
     constexpr int NBK = KB_BLK_NBK, RC = KB_BLK_RC;
-    T* sY = S.sA;   // RC x (NBK + 1) staging of the current block's solution
+    T* sY = S.sB;   // RC x (NBK + 1) staging of the current block's solution
     // ---------------- forward: L Y = B ----------------
     for (int k0 = 0; k0 < n; k0 += NBK) {
         const int kb = min(NBK, n - k0);
