@@ -118,20 +118,40 @@ KB_DEV void blk_syrk_update(const T* L21, T* C, int lda, int m2, int kb, BlkSmem
     }
 }
 
-// Load the diagonal block L(k0.., k0..) of the factor into LDS with its inverse diagonal.
+// The diagonal block L(k0.., k0..) of the factor in two steps, so the solve
+// can issue the loads of the next block before working on the current one:
+// blk_diag_fetch reads this thread's elements (e = tid + 256 m) into
+// registers, blk_diag_put writes them to LDS with the inverse diagonal.
+constexpr int KB_BLK_DPT = KB_BLK_NBK * KB_BLK_NBK / KB_BLK_WG;   // elements per thread
+
 template <typename T>
-KB_DEV void blk_stage_diag(const T* L, int lda, int k0, int kb, BlkSmem<T>& S)
+KB_DEV void blk_diag_fetch(T (&v)[KB_BLK_DPT], const T* L, int lda, int k0, int kb)
 {
     constexpr int NBK = KB_BLK_NBK;
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));   // lane-dependent values formed per call, not kept live across
-    for (int e = tid; e < NBK * NBK; e += KB_BLK_WG) {
+#pragma unroll
+    for (int m = 0; m < KB_BLK_DPT; ++m) {
+        const int e = tid + KB_BLK_WG * m;
         const int r = e % NBK, c = e / NBK;
-        S.sD[r * (NBK + 1) + c] = (r < kb && c <= r) ? gld(L + (k0 + r) + (long)(k0 + c) * lda) : T(0);
+        const bool in = r < kb && c <= r;
+        const T x = gld(L + (in ? (k0 + r) + (long)(k0 + c) * lda : 0));
+        v[m] = in ? x : T(0);
     }
-    if (threadIdx.x < NBK) {
-        const int r = threadIdx.x;
-        S.sInv[r] = (r < kb) ? T(1) / gld(L + (k0 + r) + (long)(k0 + r) * lda) : T(1);
+}
+
+template <typename T>
+KB_DEV void blk_diag_put(const T (&v)[KB_BLK_DPT], int kb, BlkSmem<T>& S)
+{
+    constexpr int NBK = KB_BLK_NBK;
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+#pragma unroll
+    for (int m = 0; m < KB_BLK_DPT; ++m) {
+        const int e = tid + KB_BLK_WG * m;
+        const int r = e % NBK, c = e / NBK;
+        S.sD[r * (NBK + 1) + c] = v[m];
+        if (r == c) S.sInv[r] = (r < kb) ? T(1) / v[m] : T(1);
     }
     __syncthreads();
 }
@@ -419,9 +439,12 @@ KB_DEV void blk_solve(const T* L, int lda, int n, T* B, int ldb, int side_left, 
     constexpr int NBK = KB_BLK_NBK, RC = KB_BLK_RC;
     T* sY = S.sB;   // RC x (NBK + 1) staging of the current block's solution
     // ---------------- forward: L Y = B ----------------
+    T dnext[KB_BLK_DPT];
+    blk_diag_fetch<T>(dnext, L, lda, 0, min(NBK, n));
     for (int k0 = 0; k0 < n; k0 += NBK) {
         const int kb = min(NBK, n - k0);
-        blk_stage_diag<T>(L, lda, k0, kb, S);
+        blk_diag_put<T>(dnext, kb, S);
+        if (k0 + NBK < n) blk_diag_fetch<T>(dnext, L, lda, k0 + NBK, min(NBK, n - k0 - NBK));
         for (int c0 = 0; c0 < nrhs; c0 += RC) {
             const int rc = min(RC, nrhs - c0);
             const int ncb = (rc + 15) / 16;
@@ -433,10 +456,12 @@ KB_DEV void blk_solve(const T* L, int lda, int n, T* B, int ldb, int side_left, 
     }
     // ---------------- backward: L^T X = Y ----------------
     const int nblk = (n + NBK - 1) / NBK;
+    blk_diag_fetch<T>(dnext, L, lda, (nblk - 1) * NBK, n - (nblk - 1) * NBK);
     for (int bk = nblk - 1; bk >= 0; --bk) {
         const int k0 = bk * NBK;
         const int kb = min(NBK, n - k0);
-        blk_stage_diag<T>(L, lda, k0, kb, S);
+        blk_diag_put<T>(dnext, kb, S);
+        if (bk > 0) blk_diag_fetch<T>(dnext, L, lda, k0 - NBK, NBK);
         for (int c0 = 0; c0 < nrhs; c0 += RC) {
             const int rc = min(RC, nrhs - c0);
             const int ncb = (rc + 15) / 16;
