@@ -86,41 +86,133 @@ KB_DEV double mix_anorm(const kb_d4 (&ta)[10], const kb_d4 (&tu)[10], int n, int
     return row16_max(anrm);
 }
 
+// One substitution step j of a unit triangular sweep as a single instruction
+// block: the NO off-diagonal products of step j-1 (plain FMAs of t, which
+// holds lane j-1's final value, into the accumulators a), the broadcast
+// t := y(lane j) and the diagonal update y -= t * ld.  The FMAs (or an s_nop)
+// supply the two wait states the DPP read of y needs after the previous
+// block's write of y; the accumulators take positive sums.
+template <int J>
+KB_DEV void mix_step0(float& y, float& t, float ld)
+{
+    asm("s_nop 1\n\tv_mov_b32_dpp %1, %0 row_newbcast:%3 row_mask:0xf bank_mask:0xf\n\tv_fmac_f32_e64 %0, -%1, %2"
+        : "+v"(y), "+v"(t)
+        : "v"(ld), "n"(J));
+}
+template <int J>
+KB_DEV void mix_step1(float& y, float& t, float ld, float& a1, float l1)
+{
+    asm("v_fmac_f32 %2, %4, %1\n\ts_nop 0\n\tv_mov_b32_dpp %1, %0 row_newbcast:%5 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_e64 %0, -%1, %3"
+        : "+v"(y), "+v"(t), "+v"(a1)
+        : "v"(ld), "v"(l1), "n"(J));
+}
+template <int J>
+KB_DEV void mix_step2(float& y, float& t, float ld, float& a1, float l1, float& a2, float l2)
+{
+    asm("v_fmac_f32 %2, %5, %1\n\tv_fmac_f32 %3, %6, %1\n\tv_mov_b32_dpp %1, %0 row_newbcast:%7 row_mask:0xf "
+        "bank_mask:0xf\n\tv_fmac_f32_e64 %0, -%1, %4"
+        : "+v"(y), "+v"(t), "+v"(a1), "+v"(a2)
+        : "v"(ld), "v"(l1), "v"(l2), "n"(J));
+}
+template <int J>
+KB_DEV void mix_step3(float& y, float& t, float ld, float& a1, float l1, float& a2, float l2, float& a3, float l3)
+{
+    asm("v_fmac_f32 %2, %6, %1\n\tv_fmac_f32 %3, %7, %1\n\tv_fmac_f32 %4, %8, %1\n\tv_mov_b32_dpp %1, %0 "
+        "row_newbcast:%9 row_mask:0xf bank_mask:0xf\n\tv_fmac_f32_e64 %0, -%1, %5"
+        : "+v"(y), "+v"(t), "+v"(a1), "+v"(a2), "+v"(a3)
+        : "v"(ld), "v"(l1), "v"(l2), "v"(l3), "n"(J));
+}
+#define KB_MS0(i) case i: mix_step0<i>(y, t, ld); return;
+#define KB_MS1(i) case i: mix_step1<i>(y, t, ld, a[0], l[0]); return;
+#define KB_MS2(i) case i: mix_step2<i>(y, t, ld, a[0], l[0], a[1], l[1]); return;
+#define KB_MS3(i) case i: mix_step3<i>(y, t, ld, a[0], l[0], a[1], l[1], a[2], l[2]); return;
+// j becomes a constant once the sweeps are unrolled; the switch folds
+template <int NO>
+KB_DEV void mix_step(int j, float& y, float& t, float ld, float (&a)[3], const float (&l)[3])
+{
+    if constexpr (NO == 0) { switch (j & 15) { KB_REP16(KB_MS0, 0) } }
+    else if constexpr (NO == 1) { switch (j & 15) { KB_REP16(KB_MS1, 0) } }
+    else if constexpr (NO == 2) { switch (j & 15) { KB_REP16(KB_MS2, 0) } }
+    else { switch (j & 15) { KB_REP16(KB_MS3, 0) } }
+}
+#undef KB_MS0
+#undef KB_MS1
+#undef KB_MS2
+#undef KB_MS3
+
+// Forward sweep over block K: unit substitution with the diagonal tile,
+// interleaved with the products of the off-diagonal tiles (I, K), I > K,
+// accumulated into acc[I]; then the D^-2 scaling.
+template <int K>
+KB_DEV void mix_fwd_block(float (&y)[4], float (&acc)[4], const MixLds& S, int lane)
+{
+    constexpr int NO = 3 - K;
+    const int i = lane & 15;
+    const float(*t0)[17] = S.m.l32[tix(K, K)];
+    float yk = y[K] - acc[K];
+    float t = 0.f;
+    float a[3] = {0.f, 0.f, 0.f};
+    float l[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        if (j == 0) {
+            mix_step<0>(0, yk, t, t0[i][0], a, l);
+        } else {
+#pragma unroll
+            for (int o = 0; o < NO; ++o) l[o] = S.m.l32[tix(K + 1 + o, K)][i][j - 1];
+            mix_step<NO>(j, yk, t, t0[i][j], a, l);
+        }
+    }
+#pragma unroll
+    for (int o = 0; o < NO; ++o) acc[K + 1 + o] += a[o] + S.m.l32[tix(K + 1 + o, K)][i][15] * t;
+    y[K] = yk * S.m.d2[K][lane];
+}
+
+// Backward sweep over block K: unit substitution with the transposed
+// diagonal tile, interleaved with the products of the transposed tiles
+// (K, I), I < K, accumulated into acc[I].
+template <int K>
+KB_DEV void mix_bwd_block(float (&y)[4], float (&acc)[4], const MixLds& S, int lane)
+{
+    constexpr int NO = K;
+    const int i = lane & 15;
+    const float(*t0)[17] = S.m.l32[tix(K, K)];
+    float yk = y[K] - acc[K];
+    float t = 0.f;
+    float a[3] = {0.f, 0.f, 0.f};
+    float l[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int j = 15; j >= 0; --j) {
+        if (j == 15) {
+            mix_step<0>(15, yk, t, t0[15][i], a, l);
+        } else {
+#pragma unroll
+            for (int o = 0; o < NO; ++o) l[o] = S.m.l32[tix(K, o)][j + 1][i];
+            mix_step<NO>(j, yk, t, t0[j][i], a, l);
+        }
+    }
+#pragma unroll
+    for (int o = 0; o < NO; ++o) acc[o] += a[o] + S.m.l32[tix(K, o)][0][i] * t;
+    y[K] = yk;
+}
+
 // z := (Lu D^2 Lu^T)^-1 z in FP32 for the group's four right-hand sides
 // (lane 16c + i holds row 16K + i of right-hand side c in y[K]): a unit forward
-// sweep, the D^-2 scaling, a unit backward sweep.  Step j of a diagonal-tile
-// substitution is interleaved with the off-diagonal products that use the
-// value it has made final (lane j), so four dependency chains run side by side.
+// sweep, the D^-2 scaling, a unit backward sweep.
 KB_DEV void mix_solve32(float (&y)[4], const MixLds& S, int lane)
 {
-    const int i = lane & 15;
     float acc[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int K = 0; K < 4; ++K) {
-        const float(*t)[17] = S.m.l32[tix(K, K)];
-        float yk = y[K] + acc[K];
-#pragma unroll
-        for (int j = 0; j < 16; ++j) {
-#pragma unroll
-            for (int I = K + 1; I < 4; ++I) fnmac_rowk<float>(acc[I], yk, S.m.l32[tix(I, K)][i][j], j);
-            fnmac_rowk<float>(yk, yk, t[i][j], j);
-        }
-        y[K] = yk * S.m.d2[K][lane];
-    }
+    mix_fwd_block<0>(y, acc, S, lane);
+    mix_fwd_block<1>(y, acc, S, lane);
+    mix_fwd_block<2>(y, acc, S, lane);
+    mix_fwd_block<3>(y, acc, S, lane);
 #pragma unroll
     for (int K = 0; K < 4; ++K) acc[K] = 0.f;
-#pragma unroll
-    for (int K = 3; K >= 0; --K) {
-        const float(*t)[17] = S.m.l32[tix(K, K)];
-        float yk = y[K] + acc[K];
-#pragma unroll
-        for (int j = 15; j >= 0; --j) {
-#pragma unroll
-            for (int I = 0; I < K; ++I) fnmac_rowk<float>(acc[I], yk, S.m.l32[tix(K, I)][j][i], j);
-            fnmac_rowk<float>(yk, yk, t[j][i], j);
-        }
-        y[K] = yk;
-    }
+    mix_bwd_block<3>(y, acc, S, lane);
+    mix_bwd_block<2>(y, acc, S, lane);
+    mix_bwd_block<1>(y, acc, S, lane);
+    mix_bwd_block<0>(y, acc, S, lane);
 }
 
 // r := b - A x in FP64.  The solution blocks go through LDS (xs[J][row][c]);
