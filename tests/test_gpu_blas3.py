@@ -211,3 +211,67 @@ def test_dpotrf_blocked_info_and_right_side_solve(h, n):
         assert inf[i] == 0
         _, Xo, _ = po.posv("R", po.col(A, i), po.col(B, i))
         assert rel(po.col(X, i), Xo) <= 64 * n * EPS
+
+
+@pytest.mark.parametrize("n,nrhs", [(96, 40), (129, 64), (193, 70)])
+@pytest.mark.parametrize("side", ["L", "R"])
+def test_dposv_blocked_wide_rhs(h, n, nrhs, side):
+    # up to 64 right-hand sides per pass of the blocked solve: three and four
+    # column blocks of 16, and two passes (64 + 6); n = 129 ends with a
+    # one-column panel, n = 96 with a lone panel after a pair
+    batch = 2
+    A, _ = po.gen_spd_cond_batch(21, n, batch, 2.0, 5.0)
+    B = po.gen_rhs_batch(22, n, nrhs, batch) if side == "L" else po.gen_rhs_batch(22, nrhs, n, batch)
+    dA, dB = kt.to_dev(A), kt.to_dev(B)
+    info = kt.to_dev(np.full(batch, -7, np.int32))
+    rows, cols = B.shape[2], B.shape[1]
+    st = h.lib.kblasDposv_batch_strided(h.h, side.encode(), b"L", rows, cols, kt.P(dA), n, n * n, kt.P(dB), rows,
+                                        rows * cols, batch, kt.P(info))
+    assert st == pykblas.KBLAS_Success
+    assert np.all(kt.to_host(info) == 0)
+    L, X = kt.to_host(dA), kt.to_host(dB)
+    for i in range(batch):
+        Ai = po.col(A, i)
+        assert po.potrf_berr(Ai, po.col(L, i)) <= 4 * n * EPS
+        assert po.solve_berr(side, Ai, po.col(X, i), po.col(B, i)) <= 4 * n * EPS
+
+
+@pytest.mark.parametrize("n,nrhs", [(100, 3), (160, 50)])
+def test_dpotrs_blocked_uses_given_factor(h, n, nrhs):
+    batch = 3
+    A = po.gen_spd_shift_batch(23, n, batch)
+    Lh = np.zeros_like(A)
+    for i in range(batch):
+        Lh[i] = np.tril(po.potrf(po.col(A, i))[0]).T
+    B = po.gen_rhs_batch(24, n, nrhs, batch)
+    Lin = Lh.copy()
+    for c in range(n):
+        Lin[:, c, :c] = np.nan
+    dL, dB = kt.to_dev(Lin), kt.to_dev(B)
+    st = h.lib.kblasDpotrs_batch_strided(h.h, b"L", b"L", n, nrhs, kt.P(dL), n, n * n, kt.P(dB), n, n * nrhs,
+                                         batch)
+    assert st == pykblas.KBLAS_Success
+    X = kt.to_host(dB)
+    for i in range(batch):
+        Xo = po.potrs("L", po.col(Lh, i), po.col(B, i))
+        assert np.max(np.abs(po.col(X, i) - Xo)) <= 64 * n * EPS * max(np.max(np.abs(Xo)), 1)
+
+
+@pytest.mark.parametrize("fail_col", [40, 100, 130])
+def test_dpotrf_blocked_info_in_second_panel(h, fail_col):
+    # panels go in pairs: columns 32..63 and 96..127 are the second panel of a
+    # pair, 128.. the first of the third pair
+    n, batch = 150, 3
+    A = po.gen_spd_shift_batch(25, n, batch)
+    A[1, fail_col, fail_col] = -2.0
+    dA = kt.to_dev(A)
+    info = kt.to_dev(np.full(batch, -7, np.int32))
+    st = h.lib.kblasDpotrf_batch_strided(h.h, b"L", n, kt.P(dA), n, n * n, batch, kt.P(info))
+    assert st == pykblas.KBLAS_Success
+    inf = kt.to_host(info)
+    L = kt.to_host(dA)
+    _, io = po.potrf(po.col(A, 1))
+    assert inf[1] == io and 0 < io <= fail_col + 1
+    for i in (0, 2):
+        assert inf[i] == 0
+        assert po.potrf_berr(po.col(A, i), po.col(L, i)) <= 4 * n * EPS
