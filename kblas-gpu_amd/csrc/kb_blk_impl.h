@@ -242,32 +242,6 @@ KB_DEV int blk_panel(T* A, int lda, int n, int k0, int kb, BlkSmem<T>& S)
     return 0;
 }
 
-// In-place lower Cholesky of an n x n matrix by the whole workgroup.
-// Returns LAPACK info (identical in every thread).
-template <typename T>
-KB_DEV int blk_potrf(T* A, int lda, int n, BlkSmem<T>& S)
-{
-    constexpr int NBK = KB_BLK_NBK;
-    for (int k0 = 0; k0 < n; k0 += 2 * NBK) {
-        const int kb0 = min(NBK, n - k0);
-        int f = blk_panel<T>(A, lda, n, k0, kb0, S);
-        if (f) return f;
-        const int k1 = k0 + kb0;
-        if (k1 >= n) break;
-        const int kb1 = min(NBK, n - k1);
-        // the second panel's columns, updated with the first
-        blk_syrk_update<T>(A + k1 + (long)k0 * lda, A + k1 + (long)k1 * lda, lda, n - k1, kb0, S, kb1);
-        __syncthreads();
-        f = blk_panel<T>(A, lda, n, k1, kb1, S);
-        if (f) return f;
-        // the trailing matrix, with both panels at once (depth kb0 + kb1)
-        const int t0 = k1 + kb1;
-        if (t0 < n) blk_syrk_update<T>(A + t0 + (long)k0 * lda, A + t0 + (long)t0 * lda, lda, n - t0, kb0 + kb1, S, n - t0);
-        __syncthreads();
-    }
-    return 0;
-}
-
 // Row of a 16x16 matrix-core accumulator held in register q by lanes 16g..16g+15
 // (the column is the lane's position in its row of 16): FP64 interleaves the
 // rows, FP32 blocks them.
@@ -429,32 +403,49 @@ KB_DEV void blk_update_rows_n(int ncb, const T* L, int lda, T* B, int ldb, int s
     }
 }
 
-// Solve (L L^T) X = B for nrhs vectors stored by rhs_at(); L lower in A.
-// Both sweeps are right-looking over 32-row blocks: substitution with the
-// diagonal block, then the update of the rows not yet solved (below it in
-// the forward sweep, above it in the backward one).
+// One block step of the forward sweep L Y = B: substitution with the
+// diagonal block staged in sD / sInv, then the rows below updated with the
+// block column of L (all right-hand sides, in passes of up to 64).
 template <typename T>
-KB_DEV void blk_solve(const T* L, int lda, int n, T* B, int ldb, int side_left, int nrhs, BlkSmem<T>& S)
+KB_DEV void blk_fwd_step(const T* L, int lda, int n, int k0, int kb, T* B, int ldb, int side_left, int nrhs,
+                         BlkSmem<T>& S)
 {
-    constexpr int NBK = KB_BLK_NBK, RC = KB_BLK_RC;
-    T* sY = S.sB;   // RC x (NBK + 1) staging of the current block's solution
-    // ---------------- forward: L Y = B ----------------
+    constexpr int RC = KB_BLK_RC;
+    T* sY = S.sB;   // RC x (NBK + 1) staging of the block's solution
+    for (int c0 = 0; c0 < nrhs; c0 += RC) {
+        const int rc = min(RC, nrhs - c0);
+        const int ncb = (rc + 15) / 16;
+        blk_diag_subst_n<T, true>(ncb, B, ldb, side_left, k0, kb, c0, rc, S, sY);
+        __syncthreads();
+        blk_update_rows_n<T, false>(ncb, L, lda, B, ldb, side_left, k0, kb, k0 + kb, n, c0, rc, sY);
+        __syncthreads();
+    }
+}
+
+// Forward sweep L Y = B over 32-row blocks (right-looking), the next
+// diagonal block fetched while the current one is used.
+template <typename T>
+KB_DEV void blk_solve_fwd(const T* L, int lda, int n, T* B, int ldb, int side_left, int nrhs, BlkSmem<T>& S)
+{
+    constexpr int NBK = KB_BLK_NBK;
     T dnext[KB_BLK_DPT];
     blk_diag_fetch<T>(dnext, L, lda, 0, min(NBK, n));
     for (int k0 = 0; k0 < n; k0 += NBK) {
         const int kb = min(NBK, n - k0);
         blk_diag_put<T>(dnext, kb, S);
         if (k0 + NBK < n) blk_diag_fetch<T>(dnext, L, lda, k0 + NBK, min(NBK, n - k0 - NBK));
-        for (int c0 = 0; c0 < nrhs; c0 += RC) {
-            const int rc = min(RC, nrhs - c0);
-            const int ncb = (rc + 15) / 16;
-            blk_diag_subst_n<T, true>(ncb, B, ldb, side_left, k0, kb, c0, rc, S, sY);
-            __syncthreads();
-            blk_update_rows_n<T, false>(ncb, L, lda, B, ldb, side_left, k0, kb, k0 + kb, n, c0, rc, sY);
-            __syncthreads();
-        }
+        blk_fwd_step<T>(L, lda, n, k0, kb, B, ldb, side_left, nrhs, S);
     }
-    // ---------------- backward: L^T X = Y ----------------
+}
+
+// Backward sweep L^T X = Y over 32-row blocks (right-looking: after each
+// block's substitution the rows above it are updated).
+template <typename T>
+KB_DEV void blk_solve_bwd(const T* L, int lda, int n, T* B, int ldb, int side_left, int nrhs, BlkSmem<T>& S)
+{
+    constexpr int NBK = KB_BLK_NBK, RC = KB_BLK_RC;
+    T* sY = S.sB;
+    T dnext[KB_BLK_DPT];
     const int nblk = (n + NBK - 1) / NBK;
     blk_diag_fetch<T>(dnext, L, lda, (nblk - 1) * NBK, n - (nblk - 1) * NBK);
     for (int bk = nblk - 1; bk >= 0; --bk) {
@@ -473,6 +464,47 @@ KB_DEV void blk_solve(const T* L, int lda, int n, T* B, int ldb, int side_left, 
     }
 }
 
+// Solve (L L^T) X = B for nrhs vectors stored by rhs_at(); L lower in A.
+template <typename T>
+KB_DEV void blk_solve(const T* L, int lda, int n, T* B, int ldb, int side_left, int nrhs, BlkSmem<T>& S)
+{
+    blk_solve_fwd<T>(L, lda, n, B, ldb, side_left, nrhs, S);
+    blk_solve_bwd<T>(L, lda, n, B, ldb, side_left, nrhs, S);
+}
+
+// In-place lower Cholesky of an n x n matrix by the whole workgroup.
+// Returns LAPACK info (identical in every thread).  With B given (POSV), the
+// forward sweep L Y = B runs inside the factorization: each panel's block step
+// follows the panel, while its diagonal block is still in LDS and the panel
+// in cache; a factorization that fails at panel k leaves B forward-solved
+// through the panels before k.
+template <typename T>
+KB_DEV int blk_potrf(T* A, int lda, int n, BlkSmem<T>& S, T* B = nullptr, int ldb = 0, int side_left = 0,
+                     int nrhs = 0)
+{
+    constexpr int NBK = KB_BLK_NBK;
+    for (int k0 = 0; k0 < n; k0 += 2 * NBK) {
+        const int kb0 = min(NBK, n - k0);
+        int f = blk_panel<T>(A, lda, n, k0, kb0, S);
+        if (f) return f;
+        if (B) blk_fwd_step<T>(A, lda, n, k0, kb0, B, ldb, side_left, nrhs, S);
+        const int k1 = k0 + kb0;
+        if (k1 >= n) break;
+        const int kb1 = min(NBK, n - k1);
+        // the second panel's columns, updated with the first
+        blk_syrk_update<T>(A + k1 + (long)k0 * lda, A + k1 + (long)k1 * lda, lda, n - k1, kb0, S, kb1);
+        __syncthreads();
+        f = blk_panel<T>(A, lda, n, k1, kb1, S);
+        if (f) return f;
+        if (B) blk_fwd_step<T>(A, lda, n, k1, kb1, B, ldb, side_left, nrhs, S);
+        // the trailing matrix, with both panels at once (depth kb0 + kb1)
+        const int t0 = k1 + kb1;
+        if (t0 < n) blk_syrk_update<T>(A + t0 + (long)k0 * lda, A + t0 + (long)t0 * lda, lda, n - t0, kb0 + kb1, S, n - t0);
+        __syncthreads();
+    }
+    return 0;
+}
+
 template <typename T, int MODE>
 __global__ void __launch_bounds__(KB_BLK_WG) __attribute__((amdgpu_waves_per_eu(2)))
 k_blk_chol(int n, int nrhs, int side_left, BatchArg<T> A, BatchArg<T> B, int* info, int batch)
@@ -482,12 +514,17 @@ k_blk_chol(int n, int nrhs, int side_left, BatchArg<T> A, BatchArg<T> B, int* in
         T* Ap = A.get(mat);
         const int lda = A.ldim(mat);
         int inf = 0;
-        if constexpr (MODE != KB_MODE_POTRS) {
+        if constexpr (MODE == KB_MODE_POTRF) {
             inf = blk_potrf<T>(Ap, lda, n, S);
             if (threadIdx.x == 0 && info) info[mat] = inf;
-        }
-        if constexpr (MODE != KB_MODE_POTRF) {
-            if (inf == 0) blk_solve<T>(Ap, lda, n, B.get(mat), B.ldim(mat), side_left, nrhs, S);
+        } else if constexpr (MODE == KB_MODE_POSV) {
+            T* Bp = B.get(mat);
+            const int ldb = B.ldim(mat);
+            inf = blk_potrf<T>(Ap, lda, n, S, Bp, ldb, side_left, nrhs);
+            if (threadIdx.x == 0 && info) info[mat] = inf;
+            if (inf == 0) blk_solve_bwd<T>(Ap, lda, n, Bp, ldb, side_left, nrhs, S);
+        } else {
+            blk_solve<T>(Ap, lda, n, B.get(mat), B.ldim(mat), side_left, nrhs, S);
         }
         __syncthreads();
     }

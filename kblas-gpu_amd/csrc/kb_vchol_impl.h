@@ -167,19 +167,23 @@ k_vchol_blk(const int* vm, const int* vn, int side_left, BatchArg<T> A, BatchArg
             for (int e = threadIdx.x; e < n * nrhs; e += KB_BLK_WG)
                 gst(rhs_elem(Xp, ldx, side_left, e % n, e / n), gld(rhs_elem(Bp, ldb, side_left, e % n, e / n)));
             __syncthreads();
-            const int inf = blk_potrf<T>(Ap, lda, n, S);
+            const int inf = blk_potrf<T>(Ap, lda, n, S, Xp, ldx, side_left, nrhs);
             if (threadIdx.x == 0) { if (info) info[mat] = inf; if (iter) iter[mat] = -1; }
-            if (inf == 0) blk_solve<T>(Ap, lda, n, Xp, ldx, side_left, nrhs, S);
+            if (inf == 0) blk_solve_bwd<T>(Ap, lda, n, Xp, ldx, side_left, nrhs, S);
         } else {
             T* Ap = A.get(mat);
             const int lda = A.ldim(mat);
-            int inf = 0;
-            if constexpr (MODE != KB_MODE_POTRS) {
-                inf = blk_potrf<T>(Ap, lda, n, S);
+            if constexpr (MODE == KB_MODE_POTRF) {
+                const int inf = blk_potrf<T>(Ap, lda, n, S);
                 if (threadIdx.x == 0 && info) info[mat] = inf;
-            }
-            if constexpr (MODE != KB_MODE_POTRF) {
-                if (inf == 0) blk_solve<T>(Ap, lda, n, B.get(mat), B.ldim(mat), side_left, nrhs, S);
+            } else if constexpr (MODE == KB_MODE_POSV) {
+                T* Bp = B.get(mat);
+                const int ldb = B.ldim(mat);
+                const int inf = blk_potrf<T>(Ap, lda, n, S, Bp, ldb, side_left, nrhs);
+                if (threadIdx.x == 0 && info) info[mat] = inf;
+                if (inf == 0) blk_solve_bwd<T>(Ap, lda, n, Bp, ldb, side_left, nrhs, S);
+            } else {
+                blk_solve<T>(Ap, lda, n, B.get(mat), B.ldim(mat), side_left, nrhs, S);
             }
         }
         __syncthreads();
@@ -293,19 +297,23 @@ k_vchol_mixed(const int* vm, const int* vn, int side_left, BatchArg<T> A, BatchA
             for (int e = threadIdx.x; e < n * nrhs; e += KB_BLK_WG)
                 gst(rhs_elem(Xp, ldx, side_left, e % n, e / n), gld(rhs_elem(Bp, ldb, side_left, e % n, e / n)));
             __syncthreads();
-            const int inf = blk_potrf<T>(Ap, lda, n, S.blk);
+            const int inf = blk_potrf<T>(Ap, lda, n, S.blk, Xp, ldx, side_left, nrhs);
             if (threadIdx.x == 0) { if (info) info[mat] = inf; if (iter) iter[mat] = -1; }
-            if (inf == 0) blk_solve<T>(Ap, lda, n, Xp, ldx, side_left, nrhs, S.blk);
+            if (inf == 0) blk_solve_bwd<T>(Ap, lda, n, Xp, ldx, side_left, nrhs, S.blk);
         } else {
             T* Ap = A.get(mat);
             const int lda = A.ldim(mat);
-            int inf = 0;
-            if constexpr (MODE != KB_MODE_POTRS) {
-                inf = blk_potrf<T>(Ap, lda, n, S.blk);
+            if constexpr (MODE == KB_MODE_POTRF) {
+                const int inf = blk_potrf<T>(Ap, lda, n, S.blk);
                 if (threadIdx.x == 0 && info) info[mat] = inf;
-            }
-            if constexpr (MODE != KB_MODE_POTRF) {
-                if (inf == 0) blk_solve<T>(Ap, lda, n, B.get(mat), B.ldim(mat), side_left, nrhs, S.blk);
+            } else if constexpr (MODE == KB_MODE_POSV) {
+                T* Bp = B.get(mat);
+                const int ldb = B.ldim(mat);
+                const int inf = blk_potrf<T>(Ap, lda, n, S.blk, Bp, ldb, side_left, nrhs);
+                if (threadIdx.x == 0 && info) info[mat] = inf;
+                if (inf == 0) blk_solve_bwd<T>(Ap, lda, n, Bp, ldb, side_left, nrhs, S.blk);
+            } else {
+                blk_solve<T>(Ap, lda, n, B.get(mat), B.ldim(mat), side_left, nrhs, S.blk);
             }
         }
         __syncthreads();
