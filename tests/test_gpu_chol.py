@@ -219,3 +219,25 @@ def test_spotrf_blocked_rows_single_precision(h, n):
     for i in range(batch):
         assert po.potrf_berr(po.col(A, i), po.col(L, i)) <= 4 * n * kt.EPS32
         assert np.all(np.isnan(po.col(L, i)[np.triu_indices(n, 1)]))
+
+
+@pytest.mark.parametrize("n", [70, 129, 200])
+def test_sposv_blocked_single_precision(h, n):
+    # n > 64: the blocked workgroup kernel (FP32 instance): register-tile
+    # diagonal blocks, paired panels, fused forward sweep
+    batch = 5
+    A = po.gen_spd_shift_batch(16, n, batch)
+    B = po.gen_rhs_batch(17, 3, n, batch)
+    dA = kt.to_dev(nan_upper(A), np.float32)
+    dB = kt.to_dev(B, np.float32)
+    info = kt.to_dev(np.full(batch, -7, np.int32))
+    st = h.lib.kblasSposv_batch_strided(h.h, b"R", b"L", 3, n, kt.P(dA), n, n * n, kt.P(dB), 3, 3 * n, batch,
+                                        kt.P(info))
+    assert st == pykblas.KBLAS_Success
+    assert np.all(kt.to_host(info) == 0)
+    L, X = kt.to_host(dA).astype(np.float64), kt.to_host(dB).astype(np.float64)
+    for i in range(batch):
+        Ai = po.col(A, i)
+        assert po.potrf_berr(Ai, po.col(L, i)) <= 4 * n * kt.EPS32
+        assert po.solve_berr("R", Ai, po.col(X, i), po.col(B, i)) <= 8 * n * kt.EPS32
+        assert np.all(np.isnan(po.col(L, i)[np.triu_indices(n, 1)]))

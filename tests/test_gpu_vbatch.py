@@ -133,3 +133,33 @@ def test_vbatch_requires_workspace():
         assert st == pykblas.KBLAS_InsufficientWorkspace
     finally:
         hh.close()
+
+
+def test_sposv_vbatch_single_precision(h):
+    # FP32 non-uniform: both phases (orders above and below 64)
+    rng = np.random.default_rng(41)
+    batch = 24
+    ns = rng.integers(2, 180, batch).astype(np.int32)
+    ns[:3] = [65, 64, 1]
+    nrhs = 2
+    torch = kt.torch_dev()
+    As, Bs = [], []
+    for i, n in enumerate(ns):
+        As.append(po.gen_spd_shift(51, i, int(n)))
+        Bs.append(po.gen_rhs(52, i, int(n), nrhs))
+    dA = [kt.to_dev(np.asfortranarray(A).T.copy(), np.float32) for A in As]
+    dB = [kt.to_dev(np.asfortranarray(B).T.copy(), np.float32) for B in Bs]
+    pA = torch.tensor([t.data_ptr() for t in dA], dtype=torch.int64, device="cuda")
+    pB = torch.tensor([t.data_ptr() for t in dB], dtype=torch.int64, device="cuda")
+    alloc_ws(h, 180, batch)
+    dn = kt.to_dev(ns)
+    dr = kt.to_dev(np.full(batch, nrhs, np.int32))
+    info = kt.to_dev(np.full(batch, -7, np.int32))
+    st = h.lib.kblasSposv_vbatch(h.h, b"L", b"L", kt.P(dn), kt.P(dr), 0, 0, kt.P(pA), kt.P(dn), kt.P(pB), kt.P(dn),
+                                 batch, kt.P(info))
+    assert st == pykblas.KBLAS_Success
+    assert np.all(kt.to_host(info) == 0)
+    for i in range(batch):
+        n = int(ns[i])
+        X = kt.to_host(dB[i]).T.astype(np.float64)
+        assert po.solve_berr("L", As[i], X, Bs[i]) <= 8 * n * kt.EPS32, (i, n)
