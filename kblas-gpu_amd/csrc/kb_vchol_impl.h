@@ -248,12 +248,38 @@ static int launch_vchol(const int* vm, const int* vn, int side_left, kb_mat_t A,
 }
 
 // ---------------------------------------------------------------------------
-// Mixed precision (SPOSV): both phases in one persistent kernel of 256-thread
-// workgroups (workgroup phase first; once the queue reaches the register
-// range every wavefront takes its own tickets).  The two-kernel split above
-// hung on the GPU for this mode with its cause not yet found, so the mixed
-// mode keeps this form.
+// Mixed precision (SPOSV).  Orders above the register range take LAPACK
+// dsposv's full-precision path (ITER = -1): X := B (k_vsposv_big_prep), then
+// the FP64 POSV workgroup kernel above on (A, X).  The rest of the queue runs
+// the mixed register routine in k_vchol_mixed: a persistent kernel of
+// 256-thread workgroups that can also take the large orders itself (workgroup
+// phase first; once the queue reaches the register range every wavefront
+// takes its own tickets); with small_only set it skips that phase and its
+// wavefronts draw from the tickets after the large orders.  (Splitting this
+// mode into k_vchol_blk<SPOSV> + k_vchol_small<SPOSV> hung on the GPU, cause
+// not found, so neither is built for it.)
 // ---------------------------------------------------------------------------
+// X := B and ITER := -1 for the orders above the register range.
+template <typename T>
+__global__ void __launch_bounds__(KB_BLK_WG)
+k_vsposv_big_prep(const int* vm, const int* vn, int side_left, BatchArg<T> B, BatchArg<T> X, int* iter,
+                  int batch, void* ws)
+{
+    VQueue q = vq_view(ws, batch);
+    const int nbig = q.counter[1];
+    for (int t = blockIdx.x; t < nbig; t += gridDim.x) {
+        const long mat = q.order[t];
+        const int n = vq_order(vm, vn, side_left, mat);
+        const int nrhs = vq_nrhs(vm, vn, side_left, mat);
+        const T* Bp = B.get(mat);
+        T* Xp = X.get(mat);
+        const int ldb = B.ldim(mat), ldx = X.ldim(mat);
+        for (int e = threadIdx.x; e < n * nrhs; e += KB_BLK_WG)
+            gst(rhs_elem(Xp, ldx, side_left, e % n, e / n),
+                gld(rhs_elem(const_cast<T*>(Bp), ldb, side_left, e % n, e / n)));
+        if (threadIdx.x == 0 && iter) iter[mat] = -1;
+    }
+}
 // Shared memory of the persistent kernel: the blocked workgroup routine or,
 // in the register phase, one float LDS area per wavefront (only the FP32 and
 // mixed routines use it; the FP64 ones read their transposed factor back from
@@ -267,15 +293,15 @@ union VSmem {
 template <typename T, int MODE>
 __global__ void __launch_bounds__(KB_BLK_WG) __attribute__((amdgpu_waves_per_eu(2)))
 k_vchol_mixed(const int* vm, const int* vn, int side_left, BatchArg<T> A, BatchArg<T> B, BatchArg<T> X,
-        int* info, int* iter, int batch, void* ws)
+        int* info, int* iter, int batch, void* ws, int small_only)
 {
     __shared__ VSmem<T, (sizeof(T) == 4 || MODE == KB_MODE_SPOSV)> S;
     __shared__ int s_ticket;
     VQueue q = vq_view(ws, batch);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     // ---- workgroup phase: matrices above the register range ----
-    int ticket;
-    for (;;) {
+    int ticket = -1;
+    while (!small_only) {
         if (threadIdx.x == 0) s_ticket = atomicAdd(q.counter, 1);
         __syncthreads();
         ticket = s_ticket;
@@ -318,13 +344,15 @@ k_vchol_mixed(const int* vm, const int* vn, int side_left, BatchArg<T> A, BatchA
         }
         __syncthreads();
     }
-    // ---- wavefront phase: wave 0 keeps the ticket it holds, the others draw ----
+    // ---- wavefront phase: wave 0 keeps the ticket it holds, the others draw
+    // (from counter[2], the tickets after the large orders, with small_only) ----
     float* lt = S.lt[wave];
     if (wave != 0) ticket = -1;
+    int* ctr = q.counter + (small_only ? 2 : 0);
     for (;;) {
         if (ticket < 0) {
             int t = 0;
-            if (lane == 0) t = atomicAdd(q.counter, 1);
+            if (lane == 0) t = atomicAdd(ctr, 1);
             ticket = __builtin_amdgcn_readfirstlane(t);
         }
         if (ticket >= batch) return;
@@ -347,18 +375,14 @@ static int launch_vchol_mixed(const int* vm, const int* vn, int side_left, kb_ma
     hipLaunchKernelGGL(k_vq_hist, dim3(gb), dim3(256), 0, s, ws, vm, vn, side_left, batch);
     hipLaunchKernelGGL(k_vq_scan, dim3(1), dim3(1024), 0, s, ws, batch);
     hipLaunchKernelGGL(k_vq_scatter, dim3(gb), dim3(256), 0, s, ws, vm, vn, side_left, batch);
-    int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
-        cus = 256;
-    // persistent grid: as many workgroups as can be resident at once
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_vchol_mixed<T, MODE>, KB_BLK_WG, 0) != hipSuccess ||
-        per_cu < 1)
-        per_cu = 2;
-    int grid = cus * per_cu;
-    if (grid > batch) grid = batch;
-    hipLaunchKernelGGL((k_vchol_mixed<T, MODE>), dim3(grid), dim3(KB_BLK_WG), 0, s, vm, vn, side_left,
-                       to_arg<T>(A), to_arg<T>(B), to_arg<T>(X), info, iter, batch, ws);
+    const BatchArg<T> a = to_arg<T>(A), b = to_arg<T>(B), x = to_arg<T>(X);
+    // orders above the register range: X := B, then the FP64 POSV workgroup kernel on (A, X)
+    hipLaunchKernelGGL((k_vsposv_big_prep<T>), dim3(vchol_grid(k_vsposv_big_prep<T>, batch)), dim3(KB_BLK_WG), 0, s,
+                       vm, vn, side_left, b, x, iter, batch, ws);
+    hipLaunchKernelGGL((k_vchol_blk<T, KB_MODE_POSV>), dim3(vchol_grid(k_vchol_blk<T, KB_MODE_POSV>, batch)),
+                       dim3(KB_BLK_WG), 0, s, vm, vn, side_left, a, x, x, info, iter, batch, ws);
+    // the rest: the mixed register routine, one matrix per wavefront
+    hipLaunchKernelGGL((k_vchol_mixed<T, MODE>), dim3(vchol_grid(k_vchol_mixed<T, MODE>, batch)), dim3(KB_BLK_WG), 0,
+                       s, vm, vn, side_left, a, b, x, info, iter, batch, ws, 1);
     return (int)hipGetLastError();
 }
