@@ -18,8 +18,7 @@ k_sposv_full_prep(int n, int nrhs, int side_left, BatchArg<double> B, BatchArg<d
         const double* Bp = B.get(mat);
         double* Xp = X.get(mat);
         const int ldb = B.ldim(mat), ldx = X.ldim(mat);
-        for (int e = threadIdx.x; e < n * nrhs; e += blockDim.x)
-            gst(rhs_elem(Xp, ldx, side_left, e % n, e / n), gld(rhs_elem(const_cast<double*>(Bp), ldb, side_left, e % n, e / n)));
+        copy_rhs<double>(Bp, ldb, Xp, ldx, side_left, n, nrhs, threadIdx.x, blockDim.x);
         if (threadIdx.x == 0 && iter) iter[mat] = -1;
     }
 }

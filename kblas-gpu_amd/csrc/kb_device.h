@@ -71,6 +71,19 @@ KB_DEV void gst(T* p, T v)
     *(__attribute__((address_space(1))) T*)p = v;
 }
 
+// X := B for an n-row block of nrhs right-hand sides, thread t of nt; the
+// element order follows the storage (rows fastest for side_left, right-hand
+// sides fastest otherwise) so consecutive threads touch consecutive addresses.
+template <typename T>
+KB_DEV void copy_rhs(const T* Bp, int ldb, T* Xp, int ldx, int side_left, int n, int nrhs, int t, int nt)
+{
+    const int fast = side_left ? n : nrhs;
+    for (int e = t; e < n * nrhs; e += nt) {
+        const int r = side_left ? e % fast : e / fast, c = side_left ? e / fast : e % fast;
+        gst(rhs_elem(Xp, ldx, side_left, r, c), gld(rhs_elem(const_cast<T*>(Bp), ldb, side_left, r, c)));
+    }
+}
+
 // Make this wavefront's global stores visible to its own later global loads
 // (by other lanes of the same wavefront).  Workgroup scope suffices: the
 // stores and loads go through the same CU's vector L1, so this is a wait for

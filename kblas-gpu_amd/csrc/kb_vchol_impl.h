@@ -164,8 +164,7 @@ k_vchol_blk(const int* vm, const int* vn, int side_left, BatchArg<T> A, BatchArg
             T* Bp = B.get(mat);
             T* Xp = X.get(mat);
             const int ldb = B.ldim(mat), ldx = X.ldim(mat);
-            for (int e = threadIdx.x; e < n * nrhs; e += KB_BLK_WG)
-                gst(rhs_elem(Xp, ldx, side_left, e % n, e / n), gld(rhs_elem(Bp, ldb, side_left, e % n, e / n)));
+            copy_rhs<T>(Bp, ldb, Xp, ldx, side_left, n, nrhs, threadIdx.x, KB_BLK_WG);
             __syncthreads();
             const int inf = blk_potrf<T>(Ap, lda, n, S, Xp, ldx, side_left, nrhs);
             if (threadIdx.x == 0) { if (info) info[mat] = inf; if (iter) iter[mat] = -1; }
@@ -274,9 +273,7 @@ k_vsposv_big_prep(const int* vm, const int* vn, int side_left, BatchArg<T> B, Ba
         const T* Bp = B.get(mat);
         T* Xp = X.get(mat);
         const int ldb = B.ldim(mat), ldx = X.ldim(mat);
-        for (int e = threadIdx.x; e < n * nrhs; e += KB_BLK_WG)
-            gst(rhs_elem(Xp, ldx, side_left, e % n, e / n),
-                gld(rhs_elem(const_cast<T*>(Bp), ldb, side_left, e % n, e / n)));
+        copy_rhs<T>(Bp, ldb, Xp, ldx, side_left, n, nrhs, threadIdx.x, KB_BLK_WG);
         if (threadIdx.x == 0 && iter) iter[mat] = -1;
     }
 }
@@ -320,8 +317,7 @@ k_vchol_mixed(const int* vm, const int* vn, int side_left, BatchArg<T> A, BatchA
             T* Bp = B.get(mat);
             T* Xp = X.get(mat);
             const int ldb = B.ldim(mat), ldx = X.ldim(mat);
-            for (int e = threadIdx.x; e < n * nrhs; e += KB_BLK_WG)
-                gst(rhs_elem(Xp, ldx, side_left, e % n, e / n), gld(rhs_elem(Bp, ldb, side_left, e % n, e / n)));
+            copy_rhs<T>(Bp, ldb, Xp, ldx, side_left, n, nrhs, threadIdx.x, KB_BLK_WG);
             __syncthreads();
             const int inf = blk_potrf<T>(Ap, lda, n, S.blk, Xp, ldx, side_left, nrhs);
             if (threadIdx.x == 0) { if (info) info[mat] = inf; if (iter) iter[mat] = -1; }
