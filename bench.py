@@ -44,6 +44,9 @@ CONFIGS = {
                workload="dsposv_vbatch n~U{128..512} x 2000, 64 RHS, cond in [1e2,1e4], non-uniform mixed POSV"),
     "c4": dict(op="sposv", n=64, batch=50000, nrhs=4, gen="cond",
                workload="dsposv_batch_strided n=64 x 50000, 4 RHS, cond in [1e2,1e6], mixed FP32+IR vs FP64"),
+    # inverse family (SPD inverse), sized like c1; not a headline configuration
+    "poti": dict(op="poti", n=32, batch=10000, nrhs=0, gen="shift", seed=1240,
+                 workload="dpoti_batch_strided n=32 x 10000 (A := A^-1, POTRF then POTRI), A=B*B^T+32I"),
 }
 
 
@@ -255,10 +258,10 @@ def main():
     print("bench: config %s (%s), generating inputs" % (args.config, cfg["workload"]), file=sys.stderr, flush=True)
     vb = None
     if cfg["op"] in ("vposv", "vsposv"):
-        vb = gen_vbatch(torch, cfg, 1234 + int(args.config[1]))
+        vb = gen_vbatch(torch, cfg, cfg.get("seed", 1234 + int(args.config[1])))
         A0, B0, kappa = None, None, None
     else:
-        A0, B0, kappa = gen_inputs(torch, cfg, 1234 + int(args.config[1]))
+        A0, B0, kappa = gen_inputs(torch, cfg, cfg.get("seed") or 1234 + int(args.config[1]))
     torch.cuda.synchronize()
     h = pykblas.Handle(torch.cuda.current_stream().cuda_stream)
     lib = h.lib
@@ -362,6 +365,32 @@ def main():
         dtype = "f64"
         X_sample = None
         del As
+    elif cfg["op"] == "poti":
+        As = [A0.clone() for _ in range(K + W)]
+
+        def call(i):
+            st = lib.kblasDpoti_batch_strided(h.h, b"L", n, As[i].data_ptr(), n, NA, bc, info.data_ptr())
+            assert st == pykblas.KBLAS_Success, st
+        t, wall = time_steps(torch, call, K, W)
+        assert int((info != 0).sum().item()) == 0
+        X_sample = As[0]
+        # POTRI alone, on the factors
+        Ls = [A0.clone() for _ in range(K + W)]
+        for i in range(K + W):
+            assert lib.kblasDpotrf_batch_strided(h.h, b"L", n, Ls[i].data_ptr(), n, NA, bc,
+                                                 info.data_ptr()) == pykblas.KBLAS_Success
+
+        def call_potri(i):
+            st = lib.kblasDpotri_batch_strided(h.h, b"L", n, Ls[i].data_ptr(), n, NA, bc, info.data_ptr())
+            assert st == pykblas.KBLAS_Success, st
+        tp, _ = time_steps(torch, call_potri, K, W)
+        out["potri"] = {"value": bc * K / tp, "ms_per_step": 1000 * tp / K,
+                        "achieved_GBs": 8 * n * (n + 1) * bc * K / tp / 1e9}
+        per_unit_bytes = 8 * n * (n + 1)
+        value = bc * K / t
+        metric, unit = "inversions/s (batched DPOTI n=%d)" % n, "inversions/s"
+        dtype = "f64"
+        del Ls
     elif cfg["op"] == "posv":
         As = [A0.clone() for _ in range(K + W)]
         Bs = [B0.clone() for _ in range(K + W)]
@@ -443,7 +472,29 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    if not args.no_cpu and cfg["op"] in ("vposv", "vsposv"):
+    if not args.no_cpu and cfg["op"] == "poti":
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as po
+        bs = min(args.cpu_sample or 4000, bc)
+        A_s = A0[:bs].cpu().numpy()
+        Xg = X_sample[:bs].cpu().numpy()
+        print("bench: CPU baseline on %d matrices, 1 thread" % bs, file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        worst = 0.0
+        for i in range(bs):
+            L, inf = po.potrf(A_s[i])
+            Xo, inf2 = po.potri(L)
+            if i < 200:   # accuracy of the GPU inverse on a sub-sample: |X A - I| / (|X| |A|)
+                Xi = np.tril(Xg[i].T)
+                Xf = Xi + np.tril(Xi, -1).T
+                worst = max(worst, float(np.abs(Xf @ A_s[i] - np.eye(n)).max()
+                                         / (np.abs(Xf).max() * np.abs(A_s[i]).max() * n)))
+        tc = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": bs / tc, "unit": unit, "cores": 1, "kind": "port",
+                                "sample": "%d of the %d matrices, FP64 oracle POTRF + POTRI restatement, "
+                                          "one thread (includes the residual check of 200)" % (bs, bc)}
+        line["inverse_residual_max_sample"] = worst
+    elif not args.no_cpu and cfg["op"] in ("vposv", "vsposv"):
         line["cpu_baseline"] = cpu_baseline_vbatch(cfg, vb, K, unit)
     elif not args.no_cpu:
         cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)

@@ -232,6 +232,126 @@ int kblasDtrsm_vbatch(kblasHandle_t handle, char side, char uplo, char trans, ch
                        int* m, int* n, int max_m, int max_n, double alpha,
                        double** A, int* lda, double** B, int* ldb, int batchCount);
 
+
+/* ---- inverse family and TRMM (lower triangle) --------------------------------
+ * Served: uplo = Lower (Upper returns KBLAS_NotImplemented, as in the reference);
+ * TRTRI diag NonUnit and Unit; TRMM side Left and Right, trans N/T, diag
+ * NonUnit and Unit (the reference serves Left / NonUnit only,
+ * src/batch_triangular/Xtrmm_batch.cu:63-105).  Orders up to 1024 for the
+ * inverse family (KBLAS_NotImplemented above).  info_array: 0, or for
+ * TRTRI / POTRI the 1-based index of the first exactly-zero diagonal element
+ * (LAPACK xTRTRI; A is then left unchanged), for POTI POTRF's info (the
+ * inversion is skipped for that matrix).  No workspace is needed. */
+/* include/kblas_batch.h:1100,1113 */
+void kblas_trmm_batch_wsquery(kblasHandle_t handle, char side, int m, int n, int batchCount);
+void kblas_trmm_batch_strided_wsquery(kblasHandle_t handle, char side, int m, int n, int batchCount);
+/* include/kblas_batch.h:1611,1622 */
+void kblas_lauum_batch_wsquery(kblasHandle_t handle, const int n, int batchCount);
+void kblas_lauum_batch_strided_wsquery(kblasHandle_t handle, const int n, int batchCount);
+/* include/kblas_batch.h:1840,1851 */
+void kblas_trtri_batch_wsquery(kblasHandle_t handle, const int n, int batchCount);
+void kblas_trtri_batch_strided_wsquery(kblasHandle_t handle, const int n, int batchCount);
+/* include/kblas_batch.h:2321,2332 */
+void kblas_potri_batch_wsquery(kblasHandle_t handle, const int n, int batchCount);
+void kblas_potri_batch_strided_wsquery(kblasHandle_t handle, const int n, int batchCount);
+/* include/kblas_batch.h:2547,2558 */
+void kblas_poti_batch_wsquery(kblasHandle_t handle, const int n, int batchCount);
+void kblas_poti_batch_strided_wsquery(kblasHandle_t handle, const int n, int batchCount);
+
+/* include/kblas_batch.h:1186-1366 (TRMM: C := alpha op(A) B / alpha B op(A), in place in B) */
+int kblasStrmm_batch(kblasHandle_t handle, char side, char uplo, char trans, char diag, const int m,
+                     const int n, const float alpha, const float** A_array, int lda, float** B_array, int ldb,
+                     int batchCount);
+int kblasStrmm_batch_strided(kblasHandle_t handle, char side, char uplo, char trans, char diag, const int m,
+                             const int n, const float alpha, const float* A, int lda, long strideA, float* B,
+                             int ldb, long strideB, int batchCount);
+int kblasDtrmm_batch(kblasHandle_t handle, char side, char uplo, char trans, char diag, const int m,
+                     const int n, const double alpha, const double** A_array, int lda, double** B_array, int ldb,
+                     int batchCount);
+int kblasDtrmm_batch_strided(kblasHandle_t handle, char side, char uplo, char trans, char diag, const int m,
+                             const int n, const double alpha, const double* A, int lda, long strideA, double* B,
+                             int ldb, long strideB, int batchCount);
+int kblasCtrmm_batch(kblasHandle_t handle, char side, char uplo, char trans, char diag, const int m,
+                     const int n, const hipFloatComplex alpha, const hipFloatComplex** A_array, int lda, hipFloatComplex** B_array, int ldb,
+                     int batchCount);
+int kblasCtrmm_batch_strided(kblasHandle_t handle, char side, char uplo, char trans, char diag, const int m,
+                             const int n, const hipFloatComplex alpha, const hipFloatComplex* A, int lda, long strideA, hipFloatComplex* B,
+                             int ldb, long strideB, int batchCount);
+int kblasZtrmm_batch(kblasHandle_t handle, char side, char uplo, char trans, char diag, const int m,
+                     const int n, const hipDoubleComplex alpha, const hipDoubleComplex** A_array, int lda, hipDoubleComplex** B_array, int ldb,
+                     int batchCount);
+int kblasZtrmm_batch_strided(kblasHandle_t handle, char side, char uplo, char trans, char diag, const int m,
+                             const int n, const hipDoubleComplex alpha, const hipDoubleComplex* A, int lda, long strideA, hipDoubleComplex* B,
+                             int ldb, long strideB, int batchCount);
+/* include/kblas_batch.h:1686-1826 (LAUUM: A := L^T L) */
+int kblasSlauum_batch(kblasHandle_t handle, char uplo, const int n, float** A_array, int lda, int batchCount,
+                      int* info_array);
+int kblasSlauum_batch_strided(kblasHandle_t handle, char uplo, const int n, float* A, int lda, long strideA,
+                              int batchCount, int* info_array);
+int kblasDlauum_batch(kblasHandle_t handle, char uplo, const int n, double** A_array, int lda, int batchCount,
+                      int* info_array);
+int kblasDlauum_batch_strided(kblasHandle_t handle, char uplo, const int n, double* A, int lda, long strideA,
+                              int batchCount, int* info_array);
+int kblasClauum_batch(kblasHandle_t handle, char uplo, const int n, hipFloatComplex** A_array, int lda, int batchCount,
+                      int* info_array);
+int kblasClauum_batch_strided(kblasHandle_t handle, char uplo, const int n, hipFloatComplex* A, int lda, long strideA,
+                              int batchCount, int* info_array);
+int kblasZlauum_batch(kblasHandle_t handle, char uplo, const int n, hipDoubleComplex** A_array, int lda, int batchCount,
+                      int* info_array);
+int kblasZlauum_batch_strided(kblasHandle_t handle, char uplo, const int n, hipDoubleComplex* A, int lda, long strideA,
+                              int batchCount, int* info_array);
+/* include/kblas_batch.h:1915-2061 (TRTRI: A := L^-1) */
+int kblasStrtri_batch(kblasHandle_t handle, char uplo, char diag, const int n, float** A_array, int lda, int batchCount,
+                      int* info_array);
+int kblasStrtri_batch_strided(kblasHandle_t handle, char uplo, char diag, const int n, float* A, int lda, long strideA,
+                              int batchCount, int* info_array);
+int kblasDtrtri_batch(kblasHandle_t handle, char uplo, char diag, const int n, double** A_array, int lda, int batchCount,
+                      int* info_array);
+int kblasDtrtri_batch_strided(kblasHandle_t handle, char uplo, char diag, const int n, double* A, int lda, long strideA,
+                              int batchCount, int* info_array);
+int kblasCtrtri_batch(kblasHandle_t handle, char uplo, char diag, const int n, hipFloatComplex** A_array, int lda, int batchCount,
+                      int* info_array);
+int kblasCtrtri_batch_strided(kblasHandle_t handle, char uplo, char diag, const int n, hipFloatComplex* A, int lda, long strideA,
+                              int batchCount, int* info_array);
+int kblasZtrtri_batch(kblasHandle_t handle, char uplo, char diag, const int n, hipDoubleComplex** A_array, int lda, int batchCount,
+                      int* info_array);
+int kblasZtrtri_batch_strided(kblasHandle_t handle, char uplo, char diag, const int n, hipDoubleComplex* A, int lda, long strideA,
+                              int batchCount, int* info_array);
+/* include/kblas_batch.h:2396-2533 (POTRI: A := (L L^T)^-1 from the factor L) */
+int kblasSpotri_batch(kblasHandle_t handle, char uplo, const int n, float** A_array, int lda, int batchCount,
+                      int* info_array);
+int kblasSpotri_batch_strided(kblasHandle_t handle, char uplo, const int n, float* A, int lda, long strideA,
+                              int batchCount, int* info_array);
+int kblasDpotri_batch(kblasHandle_t handle, char uplo, const int n, double** A_array, int lda, int batchCount,
+                      int* info_array);
+int kblasDpotri_batch_strided(kblasHandle_t handle, char uplo, const int n, double* A, int lda, long strideA,
+                              int batchCount, int* info_array);
+int kblasCpotri_batch(kblasHandle_t handle, char uplo, const int n, hipFloatComplex** A_array, int lda, int batchCount,
+                      int* info_array);
+int kblasCpotri_batch_strided(kblasHandle_t handle, char uplo, const int n, hipFloatComplex* A, int lda, long strideA,
+                              int batchCount, int* info_array);
+int kblasZpotri_batch(kblasHandle_t handle, char uplo, const int n, hipDoubleComplex** A_array, int lda, int batchCount,
+                      int* info_array);
+int kblasZpotri_batch_strided(kblasHandle_t handle, char uplo, const int n, hipDoubleComplex* A, int lda, long strideA,
+                              int batchCount, int* info_array);
+/* include/kblas_batch.h:2622-2756 (POTI: A := A^-1 of an SPD A, POTRF then POTRI) */
+int kblasSpoti_batch(kblasHandle_t handle, char uplo, const int n, float** A_array, int lda, int batchCount,
+                      int* info_array);
+int kblasSpoti_batch_strided(kblasHandle_t handle, char uplo, const int n, float* A, int lda, long strideA,
+                              int batchCount, int* info_array);
+int kblasDpoti_batch(kblasHandle_t handle, char uplo, const int n, double** A_array, int lda, int batchCount,
+                      int* info_array);
+int kblasDpoti_batch_strided(kblasHandle_t handle, char uplo, const int n, double* A, int lda, long strideA,
+                              int batchCount, int* info_array);
+int kblasCpoti_batch(kblasHandle_t handle, char uplo, const int n, hipFloatComplex** A_array, int lda, int batchCount,
+                      int* info_array);
+int kblasCpoti_batch_strided(kblasHandle_t handle, char uplo, const int n, hipFloatComplex* A, int lda, long strideA,
+                              int batchCount, int* info_array);
+int kblasZpoti_batch(kblasHandle_t handle, char uplo, const int n, hipDoubleComplex** A_array, int lda, int batchCount,
+                      int* info_array);
+int kblasZpoti_batch_strided(kblasHandle_t handle, char uplo, const int n, hipDoubleComplex* A, int lda, long strideA,
+                              int batchCount, int* info_array);
+
 #ifdef __cplusplus
 }  /* extern "C" */
 #endif
@@ -403,6 +523,97 @@ inline int kblas_syrk_batch(kblasHandle_t h, char uplo, char trans, int* m, int*
 inline int kblas_trsm_batch(kblasHandle_t h, char side, char uplo, char trans, char diag, int* m, int* n,
                             int max_m, int max_n, double alpha, double** A, int* lda, double** B, int* ldb, int bc)
 { return kblasDtrsm_vbatch(h, side, uplo, trans, diag, m, n, max_m, max_n, alpha, A, lda, B, ldb, bc); }
+
+/* ---- C++ overloads of the inverse family and TRMM (include/kblas_batch.h:1115-1184,
+ * 1624-1684, 1853-1913, 2334-2394, 2560-2620) */
+inline int kblas_trmm_batch(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n,
+                            const float alpha, const float** A, int lda, float** B, int ldb, int bc)
+{ return kblasStrmm_batch(h, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, bc); }
+inline int kblas_trmm_batch(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n,
+                            const float alpha, const float* A, int lda, long sA, float* B, int ldb, long sB, int bc)
+{ return kblasStrmm_batch_strided(h, side, uplo, trans, diag, m, n, alpha, A, lda, sA, B, ldb, sB, bc); }
+inline int kblas_trmm_batch(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n,
+                            const double alpha, const double** A, int lda, double** B, int ldb, int bc)
+{ return kblasDtrmm_batch(h, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, bc); }
+inline int kblas_trmm_batch(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n,
+                            const double alpha, const double* A, int lda, long sA, double* B, int ldb, long sB, int bc)
+{ return kblasDtrmm_batch_strided(h, side, uplo, trans, diag, m, n, alpha, A, lda, sA, B, ldb, sB, bc); }
+inline int kblas_trmm_batch(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n,
+                            const hipFloatComplex alpha, const hipFloatComplex** A, int lda, hipFloatComplex** B, int ldb, int bc)
+{ return kblasCtrmm_batch(h, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, bc); }
+inline int kblas_trmm_batch(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n,
+                            const hipFloatComplex alpha, const hipFloatComplex* A, int lda, long sA, hipFloatComplex* B, int ldb, long sB, int bc)
+{ return kblasCtrmm_batch_strided(h, side, uplo, trans, diag, m, n, alpha, A, lda, sA, B, ldb, sB, bc); }
+inline int kblas_trmm_batch(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n,
+                            const hipDoubleComplex alpha, const hipDoubleComplex** A, int lda, hipDoubleComplex** B, int ldb, int bc)
+{ return kblasZtrmm_batch(h, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, bc); }
+inline int kblas_trmm_batch(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n,
+                            const hipDoubleComplex alpha, const hipDoubleComplex* A, int lda, long sA, hipDoubleComplex* B, int ldb, long sB, int bc)
+{ return kblasZtrmm_batch_strided(h, side, uplo, trans, diag, m, n, alpha, A, lda, sA, B, ldb, sB, bc); }
+inline int kblas_lauum_batch(kblasHandle_t h, char uplo, const int n, float** A, int lda, int bc, int* info)
+{ return kblasSlauum_batch(h, uplo, n, A, lda, bc, info); }
+inline int kblas_lauum_batch(kblasHandle_t h, char uplo, const int n, float* A, int lda, long sA, int bc, int* info)
+{ return kblasSlauum_batch_strided(h, uplo, n, A, lda, sA, bc, info); }
+inline int kblas_lauum_batch(kblasHandle_t h, char uplo, const int n, double** A, int lda, int bc, int* info)
+{ return kblasDlauum_batch(h, uplo, n, A, lda, bc, info); }
+inline int kblas_lauum_batch(kblasHandle_t h, char uplo, const int n, double* A, int lda, long sA, int bc, int* info)
+{ return kblasDlauum_batch_strided(h, uplo, n, A, lda, sA, bc, info); }
+inline int kblas_lauum_batch(kblasHandle_t h, char uplo, const int n, hipFloatComplex** A, int lda, int bc, int* info)
+{ return kblasClauum_batch(h, uplo, n, A, lda, bc, info); }
+inline int kblas_lauum_batch(kblasHandle_t h, char uplo, const int n, hipFloatComplex* A, int lda, long sA, int bc, int* info)
+{ return kblasClauum_batch_strided(h, uplo, n, A, lda, sA, bc, info); }
+inline int kblas_lauum_batch(kblasHandle_t h, char uplo, const int n, hipDoubleComplex** A, int lda, int bc, int* info)
+{ return kblasZlauum_batch(h, uplo, n, A, lda, bc, info); }
+inline int kblas_lauum_batch(kblasHandle_t h, char uplo, const int n, hipDoubleComplex* A, int lda, long sA, int bc, int* info)
+{ return kblasZlauum_batch_strided(h, uplo, n, A, lda, sA, bc, info); }
+inline int kblas_trtri_batch(kblasHandle_t h, char uplo, char diag, const int n, float** A, int lda, int bc, int* info)
+{ return kblasStrtri_batch(h, uplo, diag, n, A, lda, bc, info); }
+inline int kblas_trtri_batch(kblasHandle_t h, char uplo, char diag, const int n, float* A, int lda, long sA, int bc, int* info)
+{ return kblasStrtri_batch_strided(h, uplo, diag, n, A, lda, sA, bc, info); }
+inline int kblas_trtri_batch(kblasHandle_t h, char uplo, char diag, const int n, double** A, int lda, int bc, int* info)
+{ return kblasDtrtri_batch(h, uplo, diag, n, A, lda, bc, info); }
+inline int kblas_trtri_batch(kblasHandle_t h, char uplo, char diag, const int n, double* A, int lda, long sA, int bc, int* info)
+{ return kblasDtrtri_batch_strided(h, uplo, diag, n, A, lda, sA, bc, info); }
+inline int kblas_trtri_batch(kblasHandle_t h, char uplo, char diag, const int n, hipFloatComplex** A, int lda, int bc, int* info)
+{ return kblasCtrtri_batch(h, uplo, diag, n, A, lda, bc, info); }
+inline int kblas_trtri_batch(kblasHandle_t h, char uplo, char diag, const int n, hipFloatComplex* A, int lda, long sA, int bc, int* info)
+{ return kblasCtrtri_batch_strided(h, uplo, diag, n, A, lda, sA, bc, info); }
+inline int kblas_trtri_batch(kblasHandle_t h, char uplo, char diag, const int n, hipDoubleComplex** A, int lda, int bc, int* info)
+{ return kblasZtrtri_batch(h, uplo, diag, n, A, lda, bc, info); }
+inline int kblas_trtri_batch(kblasHandle_t h, char uplo, char diag, const int n, hipDoubleComplex* A, int lda, long sA, int bc, int* info)
+{ return kblasZtrtri_batch_strided(h, uplo, diag, n, A, lda, sA, bc, info); }
+inline int kblas_potri_batch(kblasHandle_t h, char uplo, const int n, float** A, int lda, int bc, int* info)
+{ return kblasSpotri_batch(h, uplo, n, A, lda, bc, info); }
+inline int kblas_potri_batch(kblasHandle_t h, char uplo, const int n, float* A, int lda, long sA, int bc, int* info)
+{ return kblasSpotri_batch_strided(h, uplo, n, A, lda, sA, bc, info); }
+inline int kblas_potri_batch(kblasHandle_t h, char uplo, const int n, double** A, int lda, int bc, int* info)
+{ return kblasDpotri_batch(h, uplo, n, A, lda, bc, info); }
+inline int kblas_potri_batch(kblasHandle_t h, char uplo, const int n, double* A, int lda, long sA, int bc, int* info)
+{ return kblasDpotri_batch_strided(h, uplo, n, A, lda, sA, bc, info); }
+inline int kblas_potri_batch(kblasHandle_t h, char uplo, const int n, hipFloatComplex** A, int lda, int bc, int* info)
+{ return kblasCpotri_batch(h, uplo, n, A, lda, bc, info); }
+inline int kblas_potri_batch(kblasHandle_t h, char uplo, const int n, hipFloatComplex* A, int lda, long sA, int bc, int* info)
+{ return kblasCpotri_batch_strided(h, uplo, n, A, lda, sA, bc, info); }
+inline int kblas_potri_batch(kblasHandle_t h, char uplo, const int n, hipDoubleComplex** A, int lda, int bc, int* info)
+{ return kblasZpotri_batch(h, uplo, n, A, lda, bc, info); }
+inline int kblas_potri_batch(kblasHandle_t h, char uplo, const int n, hipDoubleComplex* A, int lda, long sA, int bc, int* info)
+{ return kblasZpotri_batch_strided(h, uplo, n, A, lda, sA, bc, info); }
+inline int kblas_poti_batch(kblasHandle_t h, char uplo, const int n, float** A, int lda, int bc, int* info)
+{ return kblasSpoti_batch(h, uplo, n, A, lda, bc, info); }
+inline int kblas_poti_batch(kblasHandle_t h, char uplo, const int n, float* A, int lda, long sA, int bc, int* info)
+{ return kblasSpoti_batch_strided(h, uplo, n, A, lda, sA, bc, info); }
+inline int kblas_poti_batch(kblasHandle_t h, char uplo, const int n, double** A, int lda, int bc, int* info)
+{ return kblasDpoti_batch(h, uplo, n, A, lda, bc, info); }
+inline int kblas_poti_batch(kblasHandle_t h, char uplo, const int n, double* A, int lda, long sA, int bc, int* info)
+{ return kblasDpoti_batch_strided(h, uplo, n, A, lda, sA, bc, info); }
+inline int kblas_poti_batch(kblasHandle_t h, char uplo, const int n, hipFloatComplex** A, int lda, int bc, int* info)
+{ return kblasCpoti_batch(h, uplo, n, A, lda, bc, info); }
+inline int kblas_poti_batch(kblasHandle_t h, char uplo, const int n, hipFloatComplex* A, int lda, long sA, int bc, int* info)
+{ return kblasCpoti_batch_strided(h, uplo, n, A, lda, sA, bc, info); }
+inline int kblas_poti_batch(kblasHandle_t h, char uplo, const int n, hipDoubleComplex** A, int lda, int bc, int* info)
+{ return kblasZpoti_batch(h, uplo, n, A, lda, bc, info); }
+inline int kblas_poti_batch(kblasHandle_t h, char uplo, const int n, hipDoubleComplex* A, int lda, long sA, int bc, int* info)
+{ return kblasZpoti_batch_strided(h, uplo, n, A, lda, sA, bc, info); }
 #endif /* __cplusplus */
 
 #endif /* KBLAS_BATCH_H_ */

@@ -1,4 +1,5 @@
-/* kb_api.c -- the kblas{S,D}{potrf,potrs,posv,trsm,syrk,gemm}_batch* C entry points.
+/* kb_api.c -- the kblas{S,D}{potrf,potrs,posv,trsm,syrk,gemm}_batch* C entry points,
+ * and those of the inverse family (trtri, lauum, potri, poti) and TRMM.
  *
  * Front-end behaviour mirrors the reference front-ends
  * (src/batch_triangular/X{potrf,potrs,posv,trsm,syrk,gemm}_batch.cu): workspace
@@ -429,6 +430,98 @@ int kblasDsposv_vbatch(kblasHandle_t h, char side, char uplo, int* m, int* n, in
                        int* info, int* iter)
 { (void)max_m; return vchol(h, 'm', KB_MODE_SPOSV, side, uplo, m, n, max_n, A, lda, (void*)B, ldb, X, ldx, bc, info, iter); }
 
+/* ============ inverse family: TRTRI / LAUUM / POTRI / POTI ============== */
+/* Front-ends of src/batch_triangular/X{trtri,lauum,potri,poti}_batch.cu: Upper
+ * returns KBLAS_NotImplemented (Xpotri_batch_drivers.cuh:38-41,
+ * Xlauum_batch_drivers.cuh:39-42, Xpoti_batch_drivers.cuh:39-42).  POTI is
+ * POTRF then POTRI (Xpoti_batch_drivers.cuh:80-110); matrices whose
+ * factorization failed keep POTRF's info and factor.  Orders above
+ * KB_INV_MAX_N return KBLAS_NotImplemented. */
+enum { INV_TRTRI, INV_LAUUM, INV_POTRI, INV_POTI };
+static int inv_core(kblasHandle_t h, char prec, int what, char uplo, char diag, int n, kb_mat_t A,
+                    long strideA, int strided, int bc, int* info)
+{
+    KB_CHECK_HANDLE(h);
+    int u = upl(uplo), dg = what == INV_TRTRI ? diagof(diag) : 0;
+    if (u < 0 || dg < 0 || n < 0 || bc < 0 || A.ld < IMAX(1, n)) return KBLAS_Error_WrongInput;
+    if (u == 1) return KBLAS_NotImplemented;
+    if (bc == 0) return KBLAS_Success;
+    if (strided ? (!A.base || strideA < (long)A.ld * n) : !A.ptrs) return KBLAS_Error_WrongInput;
+    if (n == 0) { zero_info(h, info, bc); return KBLAS_Success; }
+    if (n > KB_INV_MAX_N) return KBLAS_NotImplemented;
+    if (what == INV_POTI) {
+        int st = n <= KB_REG_MAX_N
+                     ? kb_launch_reg_chol(prec, KB_MODE_POTRF, n, 0, 0, A, A, info, bc, h->stream)
+                     : kb_launch_blk_chol(prec, KB_MODE_POTRF, n, 0, 0, A, A, info, bc, h->stream);
+        if (st) return launched(st);
+        return launched(kb_launch_inv(prec, KB_INV_OP_POTRI, 0, n, A, info, info, bc, h->stream));
+    }
+    int op = what == INV_TRTRI ? KB_INV_OP_TRTRI : what == INV_LAUUM ? KB_INV_OP_LAUUM : KB_INV_OP_POTRI;
+    return launched(kb_launch_inv(prec, op, dg, n, A, info, NULL, bc, h->stream));
+}
+
+#define KB_INV_ENTRY(P, PREC, T, NAME, WHAT)                                                                  \
+    int kblas##P##NAME##_batch(kblasHandle_t h, char uplo, const int n, T** A, int lda, int bc, int* info)   \
+    { return inv_core(h, PREC, WHAT, uplo, 'N', n, mat_ptrs(A, lda), 0, 0, bc, info); }                      \
+    int kblas##P##NAME##_batch_strided(kblasHandle_t h, char uplo, const int n, T* A, int lda, long sA,       \
+                                       int bc, int* info)                                                     \
+    { return inv_core(h, PREC, WHAT, uplo, 'N', n, mat_strided(A, lda, sA), sA, 1, bc, info); }
+KB_INV_ENTRY(D, 'd', double, lauum, INV_LAUUM)
+KB_INV_ENTRY(S, 's', float, lauum, INV_LAUUM)
+KB_INV_ENTRY(D, 'd', double, potri, INV_POTRI)
+KB_INV_ENTRY(S, 's', float, potri, INV_POTRI)
+KB_INV_ENTRY(D, 'd', double, poti, INV_POTI)
+KB_INV_ENTRY(S, 's', float, poti, INV_POTI)
+#undef KB_INV_ENTRY
+
+int kblasDtrtri_batch(kblasHandle_t h, char uplo, char diag, const int n, double** A, int lda, int bc, int* info)
+{ return inv_core(h, 'd', INV_TRTRI, uplo, diag, n, mat_ptrs(A, lda), 0, 0, bc, info); }
+int kblasStrtri_batch(kblasHandle_t h, char uplo, char diag, const int n, float** A, int lda, int bc, int* info)
+{ return inv_core(h, 's', INV_TRTRI, uplo, diag, n, mat_ptrs(A, lda), 0, 0, bc, info); }
+int kblasDtrtri_batch_strided(kblasHandle_t h, char uplo, char diag, const int n, double* A, int lda, long sA,
+                              int bc, int* info)
+{ return inv_core(h, 'd', INV_TRTRI, uplo, diag, n, mat_strided(A, lda, sA), sA, 1, bc, info); }
+int kblasStrtri_batch_strided(kblasHandle_t h, char uplo, char diag, const int n, float* A, int lda, long sA,
+                              int bc, int* info)
+{ return inv_core(h, 's', INV_TRTRI, uplo, diag, n, mat_strided(A, lda, sA), sA, 1, bc, info); }
+
+/* ================================ TRMM ==================================== */
+/* src/batch_triangular/Xtrmm_batch.cu:63-105 serves Left / Lower / NonUnit
+ * only; here Right and Unit are served too, Upper returns NotImplemented. */
+static int trmm_core(kblasHandle_t h, char prec, char side, char uplo, char trans, char diag,
+                     int m, int n, double alpha, kb_mat_t A, long sA, kb_mat_t B, long sB,
+                     int strided, int bc)
+{
+    KB_CHECK_HANDLE(h);
+    int sd = sideof(side), u = upl(uplo), t = transof(trans), dg = diagof(diag);
+    if (sd < 0 || u < 0 || t < 0 || dg < 0 || m < 0 || n < 0 || bc < 0) return KBLAS_Error_WrongInput;
+    int ka = sd == 0 ? m : n;
+    if (A.ld < IMAX(1, ka) || B.ld < IMAX(1, m)) return KBLAS_Error_WrongInput;
+    if (u == 1) return KBLAS_NotImplemented;
+    if (bc == 0 || m == 0 || n == 0) return KBLAS_Success;
+    if (strided) {
+        if (!A.base || !B.base || sA < 0 || sB < (long)B.ld * n) return KBLAS_Error_WrongInput;
+    } else if (!A.ptrs || !B.ptrs) {
+        return KBLAS_Error_WrongInput;
+    }
+    return launched(kb_launch_trmm(prec, sd == 0, t, dg, m, n, alpha, A, B, bc, h->stream));
+}
+
+int kblasDtrmm_batch(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n,
+                     const double alpha, const double** A, int lda, double** B, int ldb, int bc)
+{ return trmm_core(h, 'd', side, uplo, trans, diag, m, n, alpha, mat_ptrs(A, lda), 0, mat_ptrs(B, ldb), 0, 0, bc); }
+int kblasStrmm_batch(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n,
+                     const float alpha, const float** A, int lda, float** B, int ldb, int bc)
+{ return trmm_core(h, 's', side, uplo, trans, diag, m, n, alpha, mat_ptrs(A, lda), 0, mat_ptrs(B, ldb), 0, 0, bc); }
+int kblasDtrmm_batch_strided(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m,
+                             const int n, const double alpha, const double* A, int lda, long sA, double* B,
+                             int ldb, long sB, int bc)
+{ return trmm_core(h, 'd', side, uplo, trans, diag, m, n, alpha, mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB, 1, bc); }
+int kblasStrmm_batch_strided(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m,
+                             const int n, const float alpha, const float* A, int lda, long sA, float* B,
+                             int ldb, long sB, int bc)
+{ return trmm_core(h, 's', side, uplo, trans, diag, m, n, alpha, mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB, 1, bc); }
+
 /* ====================== workspace queries (all zero) ===================== */
 /* The uniform paths run without any workspace: no pointer arrays are built
  * (the reference needs them for its cuBLAS GEMM steps,
@@ -447,6 +540,17 @@ void kblas_potrs_batch_wsquery(kblasHandle_t h, const int m, const int n, int bc
 void kblas_potrs_batch_strided_wsquery(kblasHandle_t h, const int m, const int n, int bc) { (void)h; (void)m; (void)n; (void)bc; }
 void kblas_posv_batch_wsquery(kblasHandle_t h, char side, const int m, const int n, int bc) { (void)h; (void)side; (void)m; (void)n; (void)bc; }
 void kblas_posv_batch_strided_wsquery(kblasHandle_t h, char side, const int m, const int n, int bc) { (void)h; (void)side; (void)m; (void)n; (void)bc; }
+
+void kblas_trtri_batch_wsquery(kblasHandle_t h, const int n, int bc) { (void)h; (void)n; (void)bc; }
+void kblas_trtri_batch_strided_wsquery(kblasHandle_t h, const int n, int bc) { (void)h; (void)n; (void)bc; }
+void kblas_lauum_batch_wsquery(kblasHandle_t h, const int n, int bc) { (void)h; (void)n; (void)bc; }
+void kblas_lauum_batch_strided_wsquery(kblasHandle_t h, const int n, int bc) { (void)h; (void)n; (void)bc; }
+void kblas_potri_batch_wsquery(kblasHandle_t h, const int n, int bc) { (void)h; (void)n; (void)bc; }
+void kblas_potri_batch_strided_wsquery(kblasHandle_t h, const int n, int bc) { (void)h; (void)n; (void)bc; }
+void kblas_poti_batch_wsquery(kblasHandle_t h, const int n, int bc) { (void)h; (void)n; (void)bc; }
+void kblas_poti_batch_strided_wsquery(kblasHandle_t h, const int n, int bc) { (void)h; (void)n; (void)bc; }
+void kblas_trmm_batch_wsquery(kblasHandle_t h, char side, int m, int n, int bc) { (void)h; (void)side; (void)m; (void)n; (void)bc; }
+void kblas_trmm_batch_strided_wsquery(kblasHandle_t h, char side, int m, int n, int bc) { (void)h; (void)side; (void)m; (void)n; (void)bc; }
 
 /* ============ complex precisions: link-compatible, not implemented ======= */
 #define KB_NI return KBLAS_NotImplemented
@@ -474,3 +578,23 @@ int kblasCposv_batch(kblasHandle_t h, char side, char uplo, const int m, const i
 int kblasZposv_batch(kblasHandle_t h, char side, char uplo, const int m, const int n, hipDoubleComplex** A, int lda, hipDoubleComplex** B, int ldb, int bc, int* info) { KB_NI; }
 int kblasCposv_batch_strided(kblasHandle_t h, char side, char uplo, const int m, const int n, hipFloatComplex* A, int lda, long sA, hipFloatComplex* B, int ldb, long sB, int bc, int* info) { KB_NI; }
 int kblasZposv_batch_strided(kblasHandle_t h, char side, char uplo, const int m, const int n, hipDoubleComplex* A, int lda, long sA, hipDoubleComplex* B, int ldb, long sB, int bc, int* info) { KB_NI; }
+int kblasClauum_batch(kblasHandle_t h, char uplo, const int n, hipFloatComplex** A, int lda, int bc, int* info) { KB_NI; }
+int kblasClauum_batch_strided(kblasHandle_t h, char uplo, const int n, hipFloatComplex* A, int lda, long sA, int bc, int* info) { KB_NI; }
+int kblasCpotri_batch(kblasHandle_t h, char uplo, const int n, hipFloatComplex** A, int lda, int bc, int* info) { KB_NI; }
+int kblasCpotri_batch_strided(kblasHandle_t h, char uplo, const int n, hipFloatComplex* A, int lda, long sA, int bc, int* info) { KB_NI; }
+int kblasCpoti_batch(kblasHandle_t h, char uplo, const int n, hipFloatComplex** A, int lda, int bc, int* info) { KB_NI; }
+int kblasCpoti_batch_strided(kblasHandle_t h, char uplo, const int n, hipFloatComplex* A, int lda, long sA, int bc, int* info) { KB_NI; }
+int kblasCtrtri_batch(kblasHandle_t h, char uplo, char diag, const int n, hipFloatComplex** A, int lda, int bc, int* info) { KB_NI; }
+int kblasCtrtri_batch_strided(kblasHandle_t h, char uplo, char diag, const int n, hipFloatComplex* A, int lda, long sA, int bc, int* info) { KB_NI; }
+int kblasCtrmm_batch(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n, const hipFloatComplex alpha, const hipFloatComplex** A, int lda, hipFloatComplex** B, int ldb, int bc) { KB_NI; }
+int kblasCtrmm_batch_strided(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n, const hipFloatComplex alpha, const hipFloatComplex* A, int lda, long sA, hipFloatComplex* B, int ldb, long sB, int bc) { KB_NI; }
+int kblasZlauum_batch(kblasHandle_t h, char uplo, const int n, hipDoubleComplex** A, int lda, int bc, int* info) { KB_NI; }
+int kblasZlauum_batch_strided(kblasHandle_t h, char uplo, const int n, hipDoubleComplex* A, int lda, long sA, int bc, int* info) { KB_NI; }
+int kblasZpotri_batch(kblasHandle_t h, char uplo, const int n, hipDoubleComplex** A, int lda, int bc, int* info) { KB_NI; }
+int kblasZpotri_batch_strided(kblasHandle_t h, char uplo, const int n, hipDoubleComplex* A, int lda, long sA, int bc, int* info) { KB_NI; }
+int kblasZpoti_batch(kblasHandle_t h, char uplo, const int n, hipDoubleComplex** A, int lda, int bc, int* info) { KB_NI; }
+int kblasZpoti_batch_strided(kblasHandle_t h, char uplo, const int n, hipDoubleComplex* A, int lda, long sA, int bc, int* info) { KB_NI; }
+int kblasZtrtri_batch(kblasHandle_t h, char uplo, char diag, const int n, hipDoubleComplex** A, int lda, int bc, int* info) { KB_NI; }
+int kblasZtrtri_batch_strided(kblasHandle_t h, char uplo, char diag, const int n, hipDoubleComplex* A, int lda, long sA, int bc, int* info) { KB_NI; }
+int kblasZtrmm_batch(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n, const hipDoubleComplex alpha, const hipDoubleComplex** A, int lda, hipDoubleComplex** B, int ldb, int bc) { KB_NI; }
+int kblasZtrmm_batch_strided(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n, const hipDoubleComplex alpha, const hipDoubleComplex* A, int lda, long sA, hipDoubleComplex* B, int ldb, long sB, int bc) { KB_NI; }

@@ -1,0 +1,246 @@
+// kb_inv.hip -- batched triangular inverse family and TRMM (lower triangle).
+//
+//   TRTRI  A := L^-1                 src/batch_triangular/Xtrtri_batch_kernels.cuh:37-79
+//   LAUUM  A := L^T L (lower part)   src/batch_triangular/Xlauum_batch_kernels.cuh:112-146
+//   POTRI  A := (L L^T)^-1 = LAUUM(TRTRI(L))
+//                                    src/batch_triangular/Xpotri_batch_drivers.cuh:78-110
+//   TRMM   B := alpha op(L) B  or  alpha B op(L)
+//                                    src/batch_triangular/Xtrmm_batch_kernels.cuh:37-175
+//
+// One workgroup per matrix, one thread per row (TRTRI) or column (LAUUM) of
+// the matrix, so n <= KB_INV_MAX_N.  The matrix is staged in LDS when its
+// n x n block fits in KB_INV_LDS bytes and is worked on in place in HBM
+// (through the CU's cache) otherwise.  The element recurrences and their
+// summation orders are the reference register kernels' (TRTRI row by row:
+// X(r,j) = -(sum_{j<k<=r} X(r,k) A(k,j)) / A(j,j) with j descending;
+// LAUUM row i of the result: sum_{k>=i} A(k,c) A(k,i), k ascending), which
+// lets each step read only values that no thread writes in that step, so
+// one barrier per step orders the reads before the writes.
+//
+// TRMM: the triangle (when it fits) and a panel of B (whole columns for the
+// left side, whole rows for the right side) are copied to LDS, then every
+// thread forms output elements from the copies and writes them to B, so the
+// in-place update needs no ordering between elements.
+#include "kb_device.h"
+#include "kb_launch.h"
+
+#define KB_INV_LDS (60 * 1024)    // k_inv: dynamic part (a few static bytes beside it)
+#define KB_TRMM_LDS (64 * 1024)
+
+// Singularity check of a non-unit triangle (LAPACK xTRTRI: the first exactly
+// zero diagonal element, 1-based; nothing is computed when one exists).
+template <typename T>
+KB_DEV int inv_zero_pivot(const T* M, int ld, int n, int r, int* sflag)
+{
+    if (r == 0) *sflag = n + 1;
+    __syncthreads();
+    if (r < n && M[r + (long)r * ld] == T(0)) atomicMin(sflag, r + 1);
+    __syncthreads();
+    const int z = *sflag;
+    return z <= n ? z : 0;
+}
+
+// In-place inverse of the lower triangle, thread r owns row r.
+template <typename T>
+KB_DEV void inv_trtri(T* M, int ld, int n, int unit, int r)
+{
+    for (int j = n - 1; j >= 0; --j) {
+        const T a = unit ? T(1) : M[j + (long)j * ld];
+        T s = T(0);
+        if (r > j && r < n) {
+            const T* xr = M + r;              // row r of X (columns > j done)
+            const T* aj = M + (long)j * ld;   // column j of A (still original)
+            for (int k = j + 1; k < r; ++k) s = fma(xr[(long)k * ld], aj[k], s);
+            s = unit ? s + aj[r] : fma(xr[(long)r * ld], aj[r], s);
+        }
+        __syncthreads();
+        if (r == j && !unit) M[j + (long)j * ld] = T(1) / a;
+        else if (r > j && r < n) M[r + (long)j * ld] = -s / a;
+    }
+    __syncthreads();
+}
+
+// In-place L^T L of the lower triangle, thread c owns column c.
+template <typename T>
+KB_DEV void inv_lauum(T* M, int ld, int n, int c)
+{
+    for (int i = 0; i < n; ++i) {
+        T s = T(0);
+        if (c <= i) {
+            const T* ac = M + (long)c * ld;
+            const T* ai = M + (long)i * ld;
+            for (int k = i; k < n; ++k) s = fma(ac[k], ai[k], s);
+        }
+        __syncthreads();
+        if (c <= i) M[i + (long)c * ld] = s;
+    }
+    __syncthreads();
+}
+
+template <typename T>
+KB_DEV int inv_run(int op, int unit, T* M, int ld, int n, int r, int* sflag)
+{
+    int info = 0;
+    if (op != KB_INV_OP_LAUUM) {
+        if (!unit) info = inv_zero_pivot<T>(M, ld, n, r, sflag);
+        if (info) return info;
+        inv_trtri<T>(M, ld, n, unit, r);
+    }
+    if (op != KB_INV_OP_TRTRI) inv_lauum<T>(M, ld, n, r);
+    return info;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024)
+k_inv(int op, int unit, int n, BatchArg<T> A, int* info, const int* skip, int staged, int batch)
+{
+    extern __shared__ __align__(16) unsigned char kb_inv_lds[];
+    __shared__ int sflag;
+    const long b = blockIdx.x;
+    if (b >= batch) return;
+    if (skip && skip[b] != 0) return;          // POTI: factorization failed, info already set
+    T* G = A.get(b);
+    const int lda = A.ldim(b);
+    const int r = threadIdx.x, nt = blockDim.x;
+    int inf;
+    if (staged) {
+        T* S = reinterpret_cast<T*>(kb_inv_lds);
+        const int ls = n + 1;   // odd leading dimension: column-wise reads spread over the banks
+        for (long e = r; e < (long)n * n; e += nt) {
+            const int i = (int)(e % n), j = (int)(e / n);
+            if (i >= j) S[i + (long)j * ls] = G[i + (long)j * lda];
+        }
+        __syncthreads();
+        inf = inv_run<T>(op, unit, S, ls, n, r, &sflag);
+        if (inf == 0)
+            for (long e = r; e < (long)n * n; e += nt) {
+                const int i = (int)(e % n), j = (int)(e / n);
+                if (i >= j && !(unit && i == j)) G[i + (long)j * lda] = S[i + (long)j * ls];
+            }
+    } else {
+        inf = inv_run<T>(op, unit, G, lda, n, r, &sflag);
+    }
+    if (info && r == 0) info[b] = inf;
+}
+
+// ---------------------------------------------------------------- TRMM ----
+// Element (r, c) of alpha op(L) B (left, L of order m) or alpha B op(L)
+// (right, L of order n), read from the panel copy P (ldp rows) of B.
+template <typename T>
+KB_DEV T trmm_elem(int side_left, int trans, int unit, const T* L, int ldl, int ord, const T* P, int ldp,
+                   int r, int c)
+{
+    T s = T(0);
+    if (side_left) {
+        const T* pc = P + (long)c * ldp;
+        if (!trans) {   // sum_{k<=r} L(r,k) B(k,c)
+            for (int k = 0; k < r; ++k) s = fma(L[r + (long)k * ldl], pc[k], s);
+            s = unit ? s + pc[r] : fma(L[r + (long)r * ldl], pc[r], s);
+        } else {        // sum_{k>=r} L(k,r) B(k,c)
+            const T* lr = L + (long)r * ldl;
+            s = unit ? pc[r] : lr[r] * pc[r];
+            for (int k = r + 1; k < ord; ++k) s = fma(lr[k], pc[k], s);
+        }
+    } else {
+        if (!trans) {   // sum_{k>=c} B(r,k) L(k,c)
+            const T* lc = L + (long)c * ldl;
+            s = unit ? P[r + (long)c * ldp] : lc[c] * P[r + (long)c * ldp];
+            for (int k = c + 1; k < ord; ++k) s = fma(P[r + (long)k * ldp], lc[k], s);
+        } else {        // sum_{k<=c} B(r,k) L(c,k)
+            for (int k = 0; k < c; ++k) s = fma(P[r + (long)k * ldp], L[c + (long)k * ldl], s);
+            s = unit ? s + P[r + (long)c * ldp] : fma(P[r + (long)c * ldp], L[c + (long)c * ldl], s);
+        }
+    }
+    return s;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_trmm(int side_left, int trans, int unit, int m, int n, T alpha, BatchArg<T> A, BatchArg<T> B, int stageA,
+       int panel, int batch)
+{
+    extern __shared__ __align__(16) unsigned char kb_inv_lds[];
+    const long b = blockIdx.x;
+    if (b >= batch) return;
+    const int t = threadIdx.x, nt = blockDim.x;
+    const int ord = side_left ? m : n;
+    const T* GA = A.get(b);
+    T* GB = B.get(b);
+    const int lda = A.ldim(b), ldb = B.ldim(b);
+    T* SA = reinterpret_cast<T*>(kb_inv_lds);
+    T* SP = SA + (stageA ? (long)ord * (ord + 1) : 0);
+    const T* L = GA;
+    int ldl = lda;
+    if (stageA) {
+        for (long e = t; e < (long)ord * ord; e += nt) {
+            const int i = (int)(e % ord), j = (int)(e / ord);
+            if (i >= j) SA[i + (long)j * (ord + 1)] = GA[i + (long)j * lda];
+        }
+        L = SA;
+        ldl = ord + 1;
+    }
+    // Panel: columns [p0, p0 + w) of B (left) or rows [p0, p0 + w) (right).
+    const int total = side_left ? n : m;
+    for (int p0 = 0; p0 < total; p0 += panel) {
+        const int w = total - p0 < panel ? total - p0 : panel;
+        const int prow = side_left ? m : w, pcol = side_left ? w : n;
+        __syncthreads();   // previous panel's reads done (and the triangle staged)
+        for (long e = t; e < (long)prow * pcol; e += nt) {
+            const int i = (int)(e % prow), j = (int)(e / prow);
+            SP[e] = side_left ? GB[i + (long)(p0 + j) * ldb] : GB[p0 + i + (long)j * ldb];
+        }
+        __syncthreads();
+        for (long e = t; e < (long)prow * pcol; e += nt) {
+            const int i = (int)(e % prow), j = (int)(e / prow);
+            T v = T(0);
+            if (alpha != T(0)) v = alpha * trmm_elem<T>(side_left, trans, unit, L, ldl, ord, SP, prow, i, j);
+            if (side_left) GB[i + (long)(p0 + j) * ldb] = v;
+            else GB[p0 + i + (long)j * ldb] = v;
+        }
+    }
+}
+
+template <typename T>
+static int launch_inv(int op, int unit, int n, kb_mat_t A, int* info, const int* skip, int batch, hipStream_t s)
+{
+    if (n > KB_INV_MAX_N) return (int)hipErrorNotSupported;
+    const size_t need = (size_t)n * (n + 1) * sizeof(T);
+    const int staged = need <= KB_INV_LDS;
+    const int threads = ((n + 63) / 64) * 64;
+    hipLaunchKernelGGL(k_inv<T>, dim3(batch), dim3(threads), staged ? need : 0, s, op, unit, n, to_arg<T>(A),
+                       info, skip, staged, batch);
+    return (int)hipGetLastError();
+}
+
+template <typename T>
+static int launch_trmm(int side_left, int trans, int unit, int m, int n, double alpha, kb_mat_t A, kb_mat_t B,
+                       int batch, hipStream_t s)
+{
+    const int ord = side_left ? m : n;
+    const size_t tri = (size_t)ord * (ord + 1) * sizeof(T);
+    const int stageA = tri <= KB_TRMM_LDS * 5 / 8;
+    const size_t left = KB_TRMM_LDS - (stageA ? tri : 0);
+    const size_t line = (size_t)(side_left ? m : n) * sizeof(T);   // one column (left) / row (right) of B
+    if (line > left) return (int)hipErrorNotSupported;
+    const int total = side_left ? n : m;
+    long panel = (long)(left / line);
+    if (panel > total) panel = total;
+    const size_t bytes = (stageA ? tri : 0) + (size_t)panel * line;
+    hipLaunchKernelGGL(k_trmm<T>, dim3(batch), dim3(256), bytes, s, side_left, trans, unit, m, n, (T)alpha,
+                       to_arg<T>(A), to_arg<T>(B), stageA, (int)panel, batch);
+    return (int)hipGetLastError();
+}
+
+extern "C" int kb_launch_inv(char prec, int op, int unit, int n, kb_mat_t A, int* info, const int* skip, int batch,
+                             hipStream_t s)
+{
+    return prec == 'd' ? launch_inv<double>(op, unit, n, A, info, skip, batch, s)
+                       : launch_inv<float>(op, unit, n, A, info, skip, batch, s);
+}
+
+extern "C" int kb_launch_trmm(char prec, int side_left, int trans, int unit, int m, int n, double alpha,
+                              kb_mat_t A, kb_mat_t B, int batch, hipStream_t s)
+{
+    return prec == 'd' ? launch_trmm<double>(side_left, trans, unit, m, n, alpha, A, B, batch, s)
+                       : launch_trmm<float>(side_left, trans, unit, m, n, alpha, A, B, batch, s);
+}

@@ -70,6 +70,23 @@ int kb_launch_vchol(char prec, int mode, const int* vm, const int* vn, int side_
                     kb_mat_t A, kb_mat_t B, kb_mat_t X, int* info, int* iter, int batch,
                     int max_n, void* ws, hipStream_t s);
 
+/* Batched inverse family on the lower triangle, one workgroup per matrix,
+ * n <= KB_INV_MAX_N (hipErrorNotSupported above).  op: KB_INV_OP_TRTRI
+ * (A := L^-1, unit: diagonal taken as 1 and left unread/unwritten),
+ * KB_INV_OP_LAUUM (A := L^T L), KB_INV_OP_POTRI (A := (L L^T)^-1).
+ * info[b]: first exactly-zero diagonal element of a non-unit triangle
+ * (1-based, A left unchanged) or 0.  skip (optional): matrices with
+ * skip[b] != 0 are left alone, info included (POTI after a failed POTRF). */
+#define KB_INV_MAX_N 1024
+enum { KB_INV_OP_TRTRI = 0, KB_INV_OP_LAUUM = 1, KB_INV_OP_POTRI = 2 };
+int kb_launch_inv(char prec, int op, int unit, int n, kb_mat_t A, int* info, const int* skip, int batch,
+                  hipStream_t s);
+/* Batched lower TRMM: B := alpha op(L) B (side_left, L of order m) or
+ * B := alpha B op(L) (L of order n); B is m x n.  Orders up to 8192 (FP32) /
+ * 4096 (FP64) per staged line of B (hipErrorNotSupported above). */
+int kb_launch_trmm(char prec, int side_left, int trans, int unit, int m, int n, double alpha,
+                   kb_mat_t A, kb_mat_t B, int batch, hipStream_t s);
+
 /* Fill count ints with v (a kernel, so it is safe inside captured graphs). */
 int kb_launch_fill_int(int* p, int v, long count, hipStream_t s);
 

@@ -58,6 +58,16 @@ _SIG = {
     "kblas_gemm_batch_nonuniform_wsquery": (None, [P]),
     "kblas_potrf_vbatch_wsquery": (None, [P, c_int, c_int]),
     "kblas_posv_vbatch_wsquery": (None, [P, c_int, c_int]),
+    "kblas_trmm_batch_wsquery": (None, [P, c_char, c_int, c_int, c_int]),
+    "kblas_trmm_batch_strided_wsquery": (None, [P, c_char, c_int, c_int, c_int]),
+    "kblas_trtri_batch_wsquery": (None, [P, c_int, c_int]),
+    "kblas_trtri_batch_strided_wsquery": (None, [P, c_int, c_int]),
+    "kblas_lauum_batch_wsquery": (None, [P, c_int, c_int]),
+    "kblas_lauum_batch_strided_wsquery": (None, [P, c_int, c_int]),
+    "kblas_potri_batch_wsquery": (None, [P, c_int, c_int]),
+    "kblas_potri_batch_strided_wsquery": (None, [P, c_int, c_int]),
+    "kblas_poti_batch_wsquery": (None, [P, c_int, c_int]),
+    "kblas_poti_batch_strided_wsquery": (None, [P, c_int, c_int]),
 }
 _TSIG = {
     "potrf_batch": (c_int, [P, c_char, c_int, P, c_int, c_int, P]),
@@ -77,6 +87,16 @@ _TSIG = {
     "trsm_vbatch": (c_int, [P, c_char, c_char, c_char, c_char, P, P, c_int, c_int, "T", P, P, P, P, c_int]),
     "potrf_vbatch": (c_int, [P, c_char, P, c_int, P, P, c_int, P]),
     "posv_vbatch": (c_int, [P, c_char, c_char, P, P, c_int, c_int, P, P, P, P, c_int, P]),
+    "trmm_batch": (c_int, [P, c_char, c_char, c_char, c_char, c_int, c_int, "T", P, c_int, P, c_int, c_int]),
+    "trmm_batch_strided": (c_int, [P, c_char, c_char, c_char, c_char, c_int, c_int, "T", P, c_int, c_long, P, c_int, c_long, c_int]),
+    "trtri_batch": (c_int, [P, c_char, c_char, c_int, P, c_int, c_int, P]),
+    "trtri_batch_strided": (c_int, [P, c_char, c_char, c_int, P, c_int, c_long, c_int, P]),
+    "lauum_batch": (c_int, [P, c_char, c_int, P, c_int, c_int, P]),
+    "lauum_batch_strided": (c_int, [P, c_char, c_int, P, c_int, c_long, c_int, P]),
+    "potri_batch": (c_int, [P, c_char, c_int, P, c_int, c_int, P]),
+    "potri_batch_strided": (c_int, [P, c_char, c_int, P, c_int, c_long, c_int, P]),
+    "poti_batch": (c_int, [P, c_char, c_int, P, c_int, c_int, P]),
+    "poti_batch_strided": (c_int, [P, c_char, c_int, P, c_int, c_long, c_int, P]),
 }
 _DONLY = {
     "kblasDpotrs_vbatch": (c_int, [P, c_char, c_char, P, P, c_int, c_int, P, P, P, P, c_int]),

@@ -784,3 +784,91 @@ int orc_lapack_posv_batch(char side, int m, int n, double* A, int lda, long stri
     }
     return 1;
 }
+
+/* ======================= inverse family and TRMM ======================== */
+/* TRTRI, lower: the row recurrence of the reference register kernel
+ * (src/batch_triangular/Xtrtri_batch_kernels.cuh:37-79):
+ *   X(r,j) = -(sum_{j<k<=r} X(r,k) A(k,j)) / A(j,j),  X(j,j) = 1 / A(j,j),
+ * columns j descending, k ascending.  The reference uses it for n <= 16 and
+ * splits larger orders with TRSMs (Xtrtri_batch_drivers.cuh:31-75), the same
+ * quantity in another association order.  A non-unit triangle with an exactly
+ * zero diagonal element returns its 1-based index and is left unchanged
+ * (LAPACK dtrtri). */
+int orc_trtri(char diag, int n, double* A, int lda)
+{
+    const int unit = diag == 'U' || diag == 'u';
+    if (!unit)
+        for (int j = 0; j < n; ++j)
+            if (A[j + (long)j * lda] == 0.0) return j + 1;
+    for (int j = n - 1; j >= 0; --j) {
+        const double a = unit ? 1.0 : A[j + (long)j * lda];
+        /* rows descending: row r's sum reads column j only at rows <= r */
+        for (int r = n - 1; r > j; --r) {
+            double s = 0.0;
+            for (int k = j + 1; k < r; ++k) s = fma(A[r + (long)k * lda], A[k + (long)j * lda], s);
+            s = unit ? s + A[r + (long)j * lda] : fma(A[r + (long)r * lda], A[r + (long)j * lda], s);
+            A[r + (long)j * lda] = -s / a;
+        }
+        if (!unit) A[j + (long)j * lda] = 1.0 / a;
+    }
+    return 0;
+}
+
+/* LAUUM, lower: A := L^T L on the lower triangle, row i of the result
+ * (LtL)(i,c) = sum_{k>=i} L(k,c) L(k,i), k ascending, rows ascending
+ * (src/batch_triangular/Xlauum_batch_kernels.cuh:112-146). */
+void orc_lauum(int n, double* A, int lda)
+{
+    for (int i = 0; i < n; ++i)
+        for (int c = 0; c <= i; ++c) {   /* (i,c) is read by no later column of this row */
+            double s = 0.0;
+            for (int k = i; k < n; ++k) s = fma(A[k + (long)c * lda], A[k + (long)i * lda], s);
+            A[i + (long)c * lda] = s;
+        }
+}
+
+/* POTRI, lower: TRTRI then LAUUM (src/batch_triangular/Xpotri_batch_drivers.cuh:78-110). */
+int orc_potri(int n, double* A, int lda)
+{
+    int info = orc_trtri('N', n, A, lda);
+    if (info) return info;
+    orc_lauum(n, A, lda);
+    return 0;
+}
+
+/* TRMM, lower: B := alpha op(L) B (side 'L', L of order m) or alpha B op(L)
+ * (side 'R', L of order n); each element an ascending-k dot product over the
+ * triangle's nonzeros, scaled by alpha last; alpha = 0 gives B = 0 (BLAS).
+ * src/batch_triangular/Xtrmm_batch_kernels.cuh:37-175 (left side). */
+void orc_trmm(char side, char trans, char diag, int m, int n, double alpha, const double* A, int lda,
+              double* B, int ldb)
+{
+    const int left = side == 'L' || side == 'l', tr = !(trans == 'N' || trans == 'n');
+    const int unit = diag == 'U' || diag == 'u';
+    double* C = (double*)malloc(sizeof(double) * (size_t)m * n + 1);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < m; ++i) C[i + (long)j * m] = B[i + (long)j * ldb];
+#define L_(i, j) A[(i) + (long)(j) * lda]
+#define C_(i, j) C[(i) + (long)(j) * m]
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < m; ++i) {
+            double s = 0.0;
+            if (left && !tr) {
+                for (int k = 0; k < i; ++k) s = fma(L_(i, k), C_(k, j), s);
+                s = unit ? s + C_(i, j) : fma(L_(i, i), C_(i, j), s);
+            } else if (left) {
+                s = unit ? C_(i, j) : L_(i, i) * C_(i, j);
+                for (int k = i + 1; k < m; ++k) s = fma(L_(k, i), C_(k, j), s);
+            } else if (!tr) {
+                s = unit ? C_(i, j) : L_(j, j) * C_(i, j);
+                for (int k = j + 1; k < n; ++k) s = fma(C_(i, k), L_(k, j), s);
+            } else {
+                for (int k = 0; k < j; ++k) s = fma(C_(i, k), L_(j, k), s);
+                s = unit ? s + C_(i, j) : fma(C_(i, j), L_(j, j), s);
+            }
+            B[i + (long)j * ldb] = alpha == 0.0 ? 0.0 : alpha * s;
+        }
+#undef L_
+#undef C_
+    free(C);
+}

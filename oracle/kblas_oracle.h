@@ -15,6 +15,8 @@
  *   - SYRK diagonal-tile numerics      src/batch_triangular/Xsyrk_batch_kernels.cuh:40-86
  *   - POTRS Right/Lower sequence       src/batch_triangular/Xpotrs_batch_drivers.cuh:79-174
  *   - POSV = POTRF then POTRS          src/batch_triangular/Xposv_batch_drivers.cuh:80-117
+ *   - TRTRI / LAUUM / POTRI / TRMM     src/batch_triangular/X{trtri,lauum,trmm}_batch_kernels.cuh,
+ *                                       Xpotri_batch_drivers.cuh:78-110
  *   - FLOP model                       testing/flops.h:74-150
  *   - reference test data              testing/testing_helper.cu:361-403
  * Extensions with no reference counterpart (documented as such):
@@ -59,6 +61,13 @@ int  orc_reg_size(int n);            /* power of two test   */
 int  orc_closest_reg_size(int n);    /* largest pow2 < n    */
 /* Returns info (0 ok, j>0 first non-positive leading minor).                 */
 int  orc_potrf(int n, double* A, int lda);
+/* Inverse family and TRMM, lower triangle (info: first exactly-zero diagonal
+ * element, 1-based, matrix unchanged).                                      */
+int  orc_trtri(char diag, int n, double* A, int lda);
+void orc_lauum(int n, double* A, int lda);
+int  orc_potri(int n, double* A, int lda);
+void orc_trmm(char side, char trans, char diag, int m, int n, double alpha, const double* A, int lda,
+              double* B, int ldb);
 void orc_trsm(char side, char trans, int m, int n, double alpha,
               const double* A, int lda, double* B, int ldb);
 void orc_syrk(char trans, int m, int k, double alpha, const double* A, int lda,

@@ -36,6 +36,10 @@ _SIG = {
     "orc_syrk": (None, [_c, _i, _i, _d, _p, _i, _d, _p, _i]),
     "orc_gemm": (None, [_c, _c, _i, _i, _i, _d, _p, _i, _p, _i, _d, _p, _i]),
     "orc_potrs": (None, [_c, _i, _i, _p, _i, _p, _i]),
+    "orc_trtri": (_i, [_c, _i, _p, _i]),
+    "orc_lauum": (None, [_i, _p, _i]),
+    "orc_potri": (_i, [_i, _p, _i]),
+    "orc_trmm": (None, [_c, _c, _c, _i, _i, _d, _p, _i, _p, _i]),
     "orc_posv": (_i, [_c, _i, _i, _p, _i, _p, _i]),
     "orc_sposv": (_i, [_c, _i, _i, _p, _i, _p, _i, _p, _i, _p]),
     "orc_potrf_berr": (_d, [_i, _p, _i, _p, _i]),
@@ -127,6 +131,33 @@ def gemm(ta, tb, alpha, A, B, beta, C):
     k = A.shape[1] if ta == "N" else A.shape[0]
     lib().orc_gemm(ch(ta), ch(tb), m, n, k, alpha, ptr(A), A.shape[0], ptr(B), B.shape[0], beta, ptr(R), m)
     return R
+
+
+def trtri(diag, L):
+    """lower triangular inverse in place of a copy; returns (X, info)"""
+    X = np.array(L, dtype=np.float64, order="F", copy=True)
+    info = lib().orc_trtri(ch(diag), X.shape[0], ptr(X), X.shape[0])
+    return X, info
+
+
+def lauum(L):
+    X = np.array(L, dtype=np.float64, order="F", copy=True)
+    lib().orc_lauum(X.shape[0], ptr(X), X.shape[0])
+    return X
+
+
+def potri(L):
+    X = np.array(L, dtype=np.float64, order="F", copy=True)
+    info = lib().orc_potri(X.shape[0], ptr(X), X.shape[0])
+    return X, info
+
+
+def trmm(side, trans, diag, alpha, A, B):
+    X = np.array(B, dtype=np.float64, order="F", copy=True)
+    A = np.asfortranarray(A, dtype=np.float64)
+    m, n = X.shape
+    lib().orc_trmm(ch(side), ch(trans), ch(diag), m, n, alpha, ptr(A), A.shape[0], ptr(X), m)
+    return X
 
 
 def potrs(side, L, B):

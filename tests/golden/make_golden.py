@@ -10,7 +10,9 @@ expected values, but seeded and committed:
   - expected outputs from MKL dpotrf / dtrsm / dsyrk / dgemm / dpotrs, with
     POSV side 'R' computed exactly as testing/batch_triangular/test_Xposv_batch.cpp:313-326
     (dpotrf, then dtrsm(R,L,T,N) and dtrsm(R,L,N,N)).
-Run in the build container only (needs MKL): python tests/golden/make_golden.py
+The inverse family (dtrtri / dlauum / dpotri, and dtrmm) goes to a file of its
+own, lapack_golden_inv.npz, written by ``--inv`` from its own seeded generator.
+Run in the build container only (needs MKL): python tests/golden/make_golden.py [--inv]
 """
 import ctypes
 import os
@@ -169,5 +171,87 @@ def main():
     print("wrote", len(out), "arrays with", MKL_VERSION)
 
 
+def dtrtri(diag, L):
+    X = np.array(L, order="F", copy=True)
+    n = X.shape[0]; info = ci(0)
+    mkl.dtrtri_(byr(cc(b"L")), byr(cc(diag)), byr(ci(n)), P(X), byr(ci(n)), byr(info))
+    return X, info.value
+
+
+def dlauum(L):
+    X = np.array(L, order="F", copy=True)
+    n = X.shape[0]; info = ci(0)
+    mkl.dlauum_(byr(cc(b"L")), byr(ci(n)), P(X), byr(ci(n)), byr(info))
+    return X
+
+
+def dpotri(L):
+    X = np.array(L, order="F", copy=True)
+    n = X.shape[0]; info = ci(0)
+    mkl.dpotri_(byr(cc(b"L")), byr(ci(n)), P(X), byr(ci(n)), byr(info))
+    return X, info.value
+
+
+def dtrmm(side, trans, diag, alpha, A, B):
+    X = np.array(B, order="F", copy=True)
+    m, n = X.shape
+    mkl.dtrmm_(byr(cc(side)), byr(cc(b"L")), byr(cc(trans)), byr(cc(diag)), byr(ci(m)), byr(ci(n)),
+               byr(cd(alpha)), P(np.asfortranarray(A)), byr(ci(A.shape[0])), P(X), byr(ci(m)))
+    return X
+
+
+def inv_main():
+    """Inverse family: factors L of make_hpd matrices (well conditioned) and
+    unit-lower triangles with small off-diagonal entries."""
+    rng = np.random.default_rng(20190601)
+    out = {"mkl_version": np.array(MKL_VERSION)}
+    for n in (1, 2, 3, 5, 8, 16, 17, 32, 33, 64, 65):
+        A = make_hpd(rng, n)
+        L, info = dpotrf(A)
+        assert info == 0
+        L = np.asfortranarray(np.tril(L))
+        out["inv_A_%d" % n] = A
+        out["inv_L_%d" % n] = L
+        X, info = dtrtri(b"N", L)
+        assert info == 0
+        out["trtri_X_%d" % n] = np.tril(X)
+        out["lauum_X_%d" % n] = np.tril(dlauum(L))
+        X, info = dpotri(L)
+        assert info == 0
+        out["potri_X_%d" % n] = np.tril(X)
+        U = np.asfortranarray(np.tril(rng.uniform(-0.5, 0.5, (n, n)) / max(n, 1) ** 0.5, -1) +
+                              np.diag(rng.uniform(2.0, 3.0, n)))   # diagonal ignored with diag = U
+        X, info = dtrtri(b"U", U)
+        assert info == 0
+        out["trtriU_A_%d" % n] = U
+        out["trtriU_X_%d" % n] = np.tril(X, -1)
+    for n, p in ((6, 3), (20, 1), (40, 40)):
+        A = make_hpd(rng, n)
+        L, _ = dpotrf(A)
+        L = np.asfortranarray(np.tril(L))
+        L[p - 1, p - 1] = 0.0
+        _, info = dtrtri(b"N", L)
+        out["trtri_sing_L_%d" % n] = L
+        out["trtri_sing_info_%d" % n] = np.array(info)
+    for n in (5, 16, 40):
+        A = make_hpd(rng, n)
+        L, _ = dpotrf(A)
+        L = np.asfortranarray(np.tril(L))
+        for side in (b"L", b"R"):
+            for trans in (b"N", b"T"):
+                for diag in (b"N", b"U"):
+                    shape = (n, 6) if side == b"L" else (6, n)
+                    B = np.asfortranarray(rng.standard_normal(shape))
+                    key = "trmm_%s%s%s_%d" % (side.decode(), trans.decode(), diag.decode(), n)
+                    out[key + "_A"] = L
+                    out[key + "_B"] = B
+                    out[key + "_X"] = dtrmm(side, trans, diag, 0.75, L, B)
+    np.savez_compressed(os.path.join(HERE, "lapack_golden_inv.npz"), **out)
+    print("wrote", len(out), "arrays with", MKL_VERSION)
+
+
 if __name__ == "__main__":
-    main()
+    if "--inv" in sys.argv[1:]:
+        inv_main()
+    else:
+        main()
