@@ -302,6 +302,22 @@ template <> KB_DEV double kb_rsqrt<double>(double x)
 }
 template <> KB_DEV float kb_rsqrt<float>(float x) { return __builtin_amdgcn_rsqf(x); }
 
+// Reciprocal: hardware estimate and two Newton steps (FP64), within an ulp
+// or so of 1/x, against the ~10-instruction IEEE division sequence.
+template <typename T> KB_DEV T kb_rcp(T x);
+template <> KB_DEV double kb_rcp<double>(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+#pragma unroll
+    for (int it = 0; it < 2; ++it) r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+    return r;
+}
+template <> KB_DEV float kb_rcp<float>(float x)
+{
+    float r = __builtin_amdgcn_rcpf(x);
+    return __builtin_fmaf(r, __builtin_fmaf(-x, r, 1.0f), r);
+}
+
 template <typename T> KB_DEV T kb_sqrt(T x);
 template <> KB_DEV double kb_sqrt<double>(double x) { return __builtin_sqrt(x); }
 template <> KB_DEV float kb_sqrt<float>(float x) { return __builtin_sqrtf(x); }

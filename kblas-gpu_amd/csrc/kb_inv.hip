@@ -50,6 +50,7 @@ KB_DEV void inv_trtri(T* M, int ld, int n, int unit, int r)
         if (r > j && r < n) {
             const T* xr = M + r;              // row r of X (columns > j done)
             const T* aj = M + (long)j * ld;   // column j of A (still original)
+#pragma unroll 8
             for (int k = j + 1; k < r; ++k) s = fma(xr[(long)k * ld], aj[k], s);
             s = unit ? s + aj[r] : fma(xr[(long)r * ld], aj[r], s);
         }
@@ -69,6 +70,7 @@ KB_DEV void inv_lauum(T* M, int ld, int n, int c)
         if (c <= i) {
             const T* ac = M + (long)c * ld;
             const T* ai = M + (long)i * ld;
+#pragma unroll 8
             for (int k = i; k < n; ++k) s = fma(ac[k], ai[k], s);
         }
         __syncthreads();
@@ -121,6 +123,94 @@ k_inv(int op, int unit, int n, BatchArg<T> A, int* info, const int* skip, int st
         inf = inv_run<T>(op, unit, G, lda, n, r, &sflag);
     }
     if (info && r == 0) info[b] = inf;
+}
+
+// ------------------------------------------------- register form, n <= 32 ----
+// One wave holds 64 / NL matrices (NL = 16 or 32 lanes each).  The original
+// triangle is staged once in LDS (zero outside n x n) and only read after
+// that, by broadcast: TRTRI keeps row r of X = L^-1 in lane r's registers and
+// needs column j of A at step j; LAUUM keeps column c of X in lane c and needs
+// column i of X (written once to LDS) at step i.  No step waits for another
+// lane, so there is no barrier between steps.  TRTRI's sums run from the far
+// end (k descending), so the term that needs the previous step's result is
+// added last and successive steps' products overlap; the pivot's reciprocal
+// (refined hardware estimate) scales the sum instead of a division.
+template <typename T, int NL>
+__global__ void __launch_bounds__(64)
+k_inv_reg(int op, int unit, int n, BatchArg<T> A, int* info, const int* skip, int batch)
+{
+    constexpr int MPW = 64 / NL, LD = NL + 1;
+    __shared__ T tile[MPW][NL * LD];
+    const int lane = threadIdx.x, g = lane / NL, r = lane % NL;
+    const long b = (long)blockIdx.x * MPW + g;
+    const bool live = b < batch && !(skip && skip[b] != 0);
+    T* S = tile[g];
+    T* G = live ? A.get(b) : nullptr;
+    const int lda = live ? A.ldim(b) : 0;
+#pragma unroll
+    for (int j = 0; j < NL; ++j) {
+        T v = T(0);
+        if (live && r < n && j < n && r >= j) v = G[r + (long)j * lda];
+        S[r + j * LD] = v;
+    }
+    __syncthreads();
+    // first exactly-zero diagonal element of this lane group's matrix (non-unit TRTRI / POTRI)
+    const bool zero = op != KB_INV_OP_LAUUM && !unit && r < n && S[r + r * LD] == T(0);
+    const unsigned long long zm = (__ballot(zero) >> (g * NL)) & (NL == 64 ? ~0ull : ((1ull << NL) - 1));
+    const int inf = zm ? __builtin_ctzll(zm) + 1 : 0;
+    T x[NL];
+    if (op != KB_INV_OP_LAUUM) {
+#pragma unroll
+        for (int k = 0; k < NL; ++k) x[k] = (k < r) ? S[r + k * LD] : (k == r ? (unit ? T(1) : S[r + k * LD]) : T(0));
+#pragma unroll
+        for (int j = NL - 1; j >= 0; --j) {
+            // Steps past the order change no live row; the uniform branch also
+            // keeps each step's LDS reads in its own block (scheduled there).
+            if (j < n) {
+                const T a = unit ? T(1) : S[j + j * LD];
+                T s = T(0);
+#pragma unroll
+                for (int k = NL - 1; k > j; --k) s = fma(x[k], S[k + j * LD], s);   // x[k] = 0 above row r
+                const T ia = kb_rcp<T>(a);   // one reciprocal per step, shared by the group's lanes
+                const T nv = (r == j) ? (unit ? T(1) : ia) : -s * ia;
+                x[j] = (r >= j) ? nv : x[j];
+            }
+        }
+        if (op == KB_INV_OP_TRTRI) {
+            if (live && inf == 0 && r < n)
+#pragma unroll
+                for (int k = 0; k < NL; ++k)
+                    if (k < r || (k == r && !unit)) G[r + (long)k * lda] = x[k];
+            if (live && info && r == 0) info[b] = inf;
+            return;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < NL; ++k) S[r + k * LD] = (r < n) ? x[k] : T(0);   // X, rows beyond n zero
+        __syncthreads();
+    }
+    // LAUUM on the triangle in S: lane c holds column c, row i of the result is
+    // sum_{k>=i} X(k,c) X(k,i), k ascending.
+#pragma unroll
+    for (int k = 0; k < NL; ++k) x[k] = S[k + r * LD];
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+        if (i < n) {   // rows past the order are not stored
+            T s = T(0);
+#pragma unroll
+            for (int k = i; k < NL; ++k) s = fma(x[k], S[k + i * LD], s);
+            x[i] = s;   // row i is not read by later rows
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NL; ++i) S[i + r * LD] = x[i];   // transpose through LDS for row-wise stores
+    __syncthreads();
+    if (live && inf == 0 && r < n)
+#pragma unroll
+        for (int k = 0; k < NL; ++k)
+            if (k <= r) G[r + (long)k * lda] = S[r + k * LD];
+    if (live && info && r == 0) info[b] = inf;
 }
 
 // ---------------------------------------------------------------- TRMM ----
@@ -204,6 +294,16 @@ template <typename T>
 static int launch_inv(int op, int unit, int n, kb_mat_t A, int* info, const int* skip, int batch, hipStream_t s)
 {
     if (n > KB_INV_MAX_N) return (int)hipErrorNotSupported;
+    if (n <= 16) {
+        hipLaunchKernelGGL((k_inv_reg<T, 16>), dim3((batch + 3) / 4), dim3(64), 0, s, op, unit, n, to_arg<T>(A), info,
+                           skip, batch);
+        return (int)hipGetLastError();
+    }
+    if (n <= 32) {
+        hipLaunchKernelGGL((k_inv_reg<T, 32>), dim3((batch + 1) / 2), dim3(64), 0, s, op, unit, n, to_arg<T>(A), info,
+                           skip, batch);
+        return (int)hipGetLastError();
+    }
     const size_t need = (size_t)n * (n + 1) * sizeof(T);
     const int staged = need <= KB_INV_LDS;
     const int threads = ((n + 63) / 64) * 64;
