@@ -386,6 +386,17 @@ def main():
         tp, _ = time_steps(torch, call_potri, K, W)
         out["potri"] = {"value": bc * K / tp, "ms_per_step": 1000 * tp / K,
                         "achieved_GBs": 8 * n * (n + 1) * bc * K / tp / 1e9}
+        # TRMM B := L^T B (left, lower, transposed) with the factors and n right-hand sides
+        Bt = [torch.randn((bc, n, n), device="cuda", dtype=torch.float64) for _ in range(K + W)]
+
+        def call_trmm(i):
+            st = lib.kblasDtrmm_batch_strided(h.h, b"L", b"L", b"T", b"N", n, n, 1.0, Ls[i].data_ptr(), n, NA,
+                                              Bt[i].data_ptr(), n, NA, bc)
+            assert st == pykblas.KBLAS_Success, st
+        tt, _ = time_steps(torch, call_trmm, K, W)
+        out["trmm_LLTN"] = {"value": bc * K / tt, "unit": "products/s", "ms_per_step": 1000 * tt / K,
+                            "achieved_GBs": (4 * n * (n + 1) + 16 * n * n) * bc * K / tt / 1e9}
+        del Bt
         per_unit_bytes = 8 * n * (n + 1)
         value = bc * K / t
         metric, unit = "inversions/s (batched DPOTI n=%d)" % n, "inversions/s"

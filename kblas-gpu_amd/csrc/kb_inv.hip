@@ -7,8 +7,9 @@
 //   TRMM   B := alpha op(L) B  or  alpha B op(L)
 //                                    src/batch_triangular/Xtrmm_batch_kernels.cuh:37-175
 //
-// One workgroup per matrix, one thread per row (TRTRI) or column (LAUUM) of
-// the matrix, so n <= KB_INV_MAX_N.  The matrix is staged in LDS when its
+// Orders up to 32: k_inv_reg, several matrices per wave, registers (below).
+// Larger orders: k_inv, one workgroup per matrix, one thread per row (TRTRI)
+// or column (LAUUM) of the matrix, so n <= KB_INV_MAX_N.  The matrix is staged in LDS when its
 // n x n block fits in KB_INV_LDS bytes and is worked on in place in HBM
 // (through the CU's cache) otherwise.  The element recurrences and their
 // summation orders are the reference register kernels' (TRTRI row by row:

@@ -2,7 +2,8 @@
 and TRMM against the CPU oracle (pinned in test_oracle_inv.py).
 
 Bars: FP64 relative max-norm error within c * n * eps64 of the oracle (the
-kernels follow the oracle's recurrences and summation orders), FP32 within
+kernels follow the oracle's recurrences; for n <= 32 TRTRI's sums run in the
+other direction and use a refined reciprocal of the pivot), FP32 within
 c * n * eps32 of the FP64 oracle on the same FP32-rounded inputs; integer info
 and untouched memory (strict upper triangle, leading-dimension padding, a
 singular matrix, POTI after a failed factorization) bit-exact; strided and
