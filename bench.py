@@ -397,6 +397,19 @@ def main():
         out["trmm_LLTN"] = {"value": bc * K / tt, "unit": "products/s", "ms_per_step": 1000 * tt / K,
                             "achieved_GBs": (4 * n * (n + 1) + 16 * n * n) * bc * K / tt / 1e9}
         del Bt
+        # pivoted Cholesky (PSTRF) of the same SPD inputs
+        Ps = [A0.clone() for _ in range(K + W)]
+        pv = torch.zeros((bc, n), dtype=torch.int32, device="cuda")
+        rk = torch.zeros(bc, dtype=torch.int32, device="cuda")
+
+        def call_pstrf(i):
+            st = lib.kblasDpstrf_batch_strided(h.h, n, Ps[i].data_ptr(), n, NA, pv.data_ptr(), n, rk.data_ptr(), bc)
+            assert st == pykblas.KBLAS_Success, st
+        tq, _ = time_steps(torch, call_pstrf, K, W)
+        assert int((rk != n).sum().item()) == 0
+        out["pstrf"] = {"value": bc * K / tq, "unit": "factorizations/s", "ms_per_step": 1000 * tq / K,
+                        "achieved_GBs": (8 * n * (n + 1) + 4 * n) * bc * K / tq / 1e9}
+        del Ps
         per_unit_bytes = 8 * n * (n + 1)
         value = bc * K / t
         metric, unit = "inversions/s (batched DPOTI n=%d)" % n, "inversions/s"

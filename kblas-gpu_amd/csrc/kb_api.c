@@ -1,5 +1,5 @@
 /* kb_api.c -- the kblas{S,D}{potrf,potrs,posv,trsm,syrk,gemm}_batch* C entry points,
- * and those of the inverse family (trtri, lauum, potri, poti) and TRMM.
+ * and those of the inverse family (trtri, lauum, potri, poti), TRMM and PSTRF.
  *
  * Front-end behaviour mirrors the reference front-ends
  * (src/batch_triangular/X{potrf,potrs,posv,trsm,syrk,gemm}_batch.cu): workspace
@@ -15,6 +15,7 @@
 #include <string.h>
 
 #include "kb_internal.h"
+#include "batch_pstrf.h"
 
 #define IMAX(a, b) ((a) > (b) ? (a) : (b))
 
@@ -521,6 +522,34 @@ int kblasStrmm_batch_strided(kblasHandle_t h, char side, char uplo, char trans, 
                              const int n, const float alpha, const float* A, int lda, long sA, float* B,
                              int ldb, long sB, int bc)
 { return trmm_core(h, 's', side, uplo, trans, diag, m, n, alpha, mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB, 1, bc); }
+
+/* ================================ PSTRF ==================================== */
+/* src/batch_svd/batch_pstrf.cu:160-223: n > 64 returns KBLAS_Error_WrongInput;
+ * the strides are int, as declared there.  Added: negative sizes, small lda
+ * or strides and NULL operands return KBLAS_Error_WrongInput; num_ops == 0 and
+ * n == 0 return KBLAS_Success (ranks zeroed for n == 0). */
+static int pstrf_core(kblasHandle_t h, char prec, int n, kb_mat_t A, long sA, int strided, int** pivp, int* pivs,
+                      long spiv, int* ranks, int bc)
+{
+    KB_CHECK_HANDLE(h);
+    if (n < 0 || n > 64 || bc < 0 || A.ld < IMAX(1, n)) return KBLAS_Error_WrongInput;
+    if (bc == 0) return KBLAS_Success;
+    if (strided ? (!A.base || sA < (long)A.ld * n || !pivs || spiv < n) : (!A.ptrs || !pivp))
+        return KBLAS_Error_WrongInput;
+    if (n == 0) { if (ranks) kb_launch_fill_int(ranks, 0, bc, h->stream); return KBLAS_Success; }
+    return launched(kb_launch_pstrf(prec, n, A, (int* const*)pivp, pivs, spiv, ranks, bc, h->stream));
+}
+
+int kblasDpstrf_batch_strided(kblasHandle_t h, int n, double* A, int lda, int stride_a, int* piv, int stride_piv,
+                              int* ranks, int num_ops)
+{ return pstrf_core(h, 'd', n, mat_strided(A, lda, stride_a), stride_a, 1, NULL, piv, stride_piv, ranks, num_ops); }
+int kblasSpstrf_batch_strided(kblasHandle_t h, int n, float* A, int lda, int stride_a, int* piv, int stride_piv,
+                              int* ranks, int num_ops)
+{ return pstrf_core(h, 's', n, mat_strided(A, lda, stride_a), stride_a, 1, NULL, piv, stride_piv, ranks, num_ops); }
+int kblasDpstrf_batch(kblasHandle_t h, int n, double** A, int lda, int** piv, int* ranks, int num_ops)
+{ return pstrf_core(h, 'd', n, mat_ptrs(A, lda), 0, 0, piv, NULL, 0, ranks, num_ops); }
+int kblasSpstrf_batch(kblasHandle_t h, int n, float** A, int lda, int** piv, int* ranks, int num_ops)
+{ return pstrf_core(h, 's', n, mat_ptrs(A, lda), 0, 0, piv, NULL, 0, ranks, num_ops); }
 
 /* ====================== workspace queries (all zero) ===================== */
 /* The uniform paths run without any workspace: no pointer arrays are built

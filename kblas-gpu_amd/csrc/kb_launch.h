@@ -87,6 +87,11 @@ int kb_launch_inv(char prec, int op, int unit, int n, kb_mat_t A, int* info, con
 int kb_launch_trmm(char prec, int side_left, int trans, int unit, int m, int n, double alpha,
                    kb_mat_t A, kb_mat_t B, int batch, hipStream_t s);
 
+/* Batched pivoted Cholesky P^T A P = R^T R (upper), n <= 64 (hipErrorNotSupported
+ * above).  Pivots: pivp[b] (pointer array) or pivs + b * spiv (strided), 0-based. */
+int kb_launch_pstrf(char prec, int n, kb_mat_t A, int* const* pivp, int* pivs, long spiv, int* ranks,
+                    int batch, hipStream_t s);
+
 /* Fill count ints with v (a kernel, so it is safe inside captured graphs). */
 int kb_launch_fill_int(int* p, int v, long count, hipStream_t s);
 

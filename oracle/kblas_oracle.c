@@ -872,3 +872,76 @@ void orc_trmm(char side, char trans, char diag, int m, int n, double alpha, cons
 #undef C_
     free(C);
 }
+
+/* ================================ PSTRF ==================================== */
+/* Pivot search of src/batch_svd/batch_pstrf.cu:44-66 (getMaxDiagEntry +
+ * warp_max): 32 slots hold (value, index) with value -1 outside [k, n); for
+ * n > 32 slot t takes entry t + 32 when strictly larger; then a butterfly over
+ * masks 16, 8, 4, 2, 1 where a slot takes its partner's pair only when the
+ * partner's value is strictly larger; slot 0's pair wins. */
+static void pstrf_maxdiag(const double* S, int ld, int n, int k, double* val, int* idx)
+{
+    double v[32];
+    int key[32];
+    for (int t = 0; t < 32; ++t) {
+        v[t] = (t >= k && t < n) ? S[t + (long)t * ld] : -1.0;
+        key[t] = t;
+        if (n > 32) {
+            const int nx = t + 32;
+            if (nx >= k && nx < n && S[nx + (long)nx * ld] > v[t]) { v[t] = S[nx + (long)nx * ld]; key[t] = nx; }
+        }
+    }
+    for (int m = 16; m >= 1; m >>= 1) {
+        double nv[32];
+        int nk[32];
+        for (int t = 0; t < 32; ++t) {
+            const int o = t ^ m;
+            nv[t] = v[t]; nk[t] = key[t];
+            if (v[o] > v[t]) { nv[t] = v[o]; nk[t] = key[o]; }
+        }
+        for (int t = 0; t < 32; ++t) { v[t] = nv[t]; key[t] = nk[t]; }
+    }
+    *val = v[0];
+    *idx = key[0];
+}
+
+/* Batched pivoted Cholesky P^T A P = R^T R (src/batch_svd/batch_pstrf.cu:73-158):
+ * the upper triangle mirrored into the lower one, tol = max(diag) * n * eps
+ * (eps = 2^-52), right-looking steps with symmetric row/column swaps, sqrt,
+ * division of row k, trailing update a -= u * r as one fused multiply-add.
+ * On exit: upper triangle = R (past the rank: the unfactored remainder),
+ * strict lower = 0, piv 0-based, returns the rank.  n <= 64. */
+int orc_pstrf(int n, double* A, int lda, int* piv)
+{
+    const int ld = n > 0 ? n : 1;
+    double* S = (double*)calloc((size_t)ld * ld + 1, sizeof(double));
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) S[i + (long)j * ld] = i > j ? A[j + (long)i * lda] : A[i + (long)j * lda];
+    for (int i = 0; i < n; ++i) piv[i] = i;
+    double dmax;
+    int dummy;
+    pstrf_maxdiag(S, ld, n, 0, &dmax, &dummy);
+    const double tol = dmax * (double)n * 2.2204460492503131e-16;
+    int rank = n;
+    for (int k = 0; k < n; ++k) {
+        double val;
+        int p;
+        pstrf_maxdiag(S, ld, n, k, &val, &p);
+        if (val <= tol) { rank = k; break; }
+        if (p != k) {
+            int ti = piv[k]; piv[k] = piv[p]; piv[p] = ti;
+            for (int c = 0; c < n; ++c) { double t = S[k + (long)c * ld]; S[k + (long)c * ld] = S[p + (long)c * ld]; S[p + (long)c * ld] = t; }
+            for (int r = 0; r < n; ++r) { double t = S[r + (long)k * ld]; S[r + (long)k * ld] = S[r + (long)p * ld]; S[r + (long)p * ld] = t; }
+        }
+        S[k + (long)k * ld] = sqrt(S[k + (long)k * ld]);
+        for (int c = k + 1; c < n; ++c) S[k + (long)c * ld] = S[k + (long)c * ld] / S[k + (long)k * ld];
+        for (int r = k + 1; r < n; ++r) {
+            const double u = S[k + (long)r * ld];
+            for (int c = k + 1; c < n; ++c) S[r + (long)c * ld] = fma(-u, S[k + (long)c * ld], S[r + (long)c * ld]);
+        }
+    }
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) A[i + (long)j * lda] = i > j ? 0.0 : S[i + (long)j * ld];
+    free(S);
+    return rank;
+}

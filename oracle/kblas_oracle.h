@@ -17,6 +17,7 @@
  *   - POSV = POTRF then POTRS          src/batch_triangular/Xposv_batch_drivers.cuh:80-117
  *   - TRTRI / LAUUM / POTRI / TRMM     src/batch_triangular/X{trtri,lauum,trmm}_batch_kernels.cuh,
  *                                       Xpotri_batch_drivers.cuh:78-110
+ *   - PSTRF (pivoted Cholesky)         src/batch_svd/batch_pstrf.cu:44-158
  *   - FLOP model                       testing/flops.h:74-150
  *   - reference test data              testing/testing_helper.cu:361-403
  * Extensions with no reference counterpart (documented as such):
@@ -64,6 +65,8 @@ int  orc_potrf(int n, double* A, int lda);
 /* Inverse family and TRMM, lower triangle (info: first exactly-zero diagonal
  * element, 1-based, matrix unchanged).                                      */
 int  orc_trtri(char diag, int n, double* A, int lda);
+/* Pivoted Cholesky P^T A P = R^T R (upper), n <= 64; piv 0-based; returns the rank. */
+int  orc_pstrf(int n, double* A, int lda, int* piv);
 void orc_lauum(int n, double* A, int lda);
 int  orc_potri(int n, double* A, int lda);
 void orc_trmm(char side, char trans, char diag, int m, int n, double alpha, const double* A, int lda,

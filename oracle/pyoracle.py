@@ -37,6 +37,7 @@ _SIG = {
     "orc_gemm": (None, [_c, _c, _i, _i, _i, _d, _p, _i, _p, _i, _d, _p, _i]),
     "orc_potrs": (None, [_c, _i, _i, _p, _i, _p, _i]),
     "orc_trtri": (_i, [_c, _i, _p, _i]),
+    "orc_pstrf": (_i, [_i, _p, _i, _p]),
     "orc_lauum": (None, [_i, _p, _i]),
     "orc_potri": (_i, [_i, _p, _i]),
     "orc_trmm": (None, [_c, _c, _c, _i, _i, _d, _p, _i, _p, _i]),
@@ -138,6 +139,15 @@ def trtri(diag, L):
     X = np.array(L, dtype=np.float64, order="F", copy=True)
     info = lib().orc_trtri(ch(diag), X.shape[0], ptr(X), X.shape[0])
     return X, info
+
+
+def pstrf(A):
+    """pivoted Cholesky of the upper triangle of A; returns (R, piv (0-based), rank)"""
+    R = np.array(A, dtype=np.float64, order="F", copy=True)
+    n = R.shape[0]
+    piv = np.zeros(max(n, 1), dtype=np.int32)
+    rank = lib().orc_pstrf(n, ptr(R), n, ptr(piv))
+    return R, piv[:n], rank
 
 
 def lauum(L):

@@ -10,8 +10,8 @@ expected values, but seeded and committed:
   - expected outputs from MKL dpotrf / dtrsm / dsyrk / dgemm / dpotrs, with
     POSV side 'R' computed exactly as testing/batch_triangular/test_Xposv_batch.cpp:313-326
     (dpotrf, then dtrsm(R,L,T,N) and dtrsm(R,L,N,N)).
-The inverse family (dtrtri / dlauum / dpotri, and dtrmm) goes to a file of its
-own, lapack_golden_inv.npz, written by ``--inv`` from its own seeded generator.
+The inverse family (dtrtri / dlauum / dpotri, and dtrmm) and dpstrf go to a file of
+their own, lapack_golden_inv.npz, written by ``--inv`` from its own seeded generator.
 Run in the build container only (needs MKL): python tests/golden/make_golden.py [--inv]
 """
 import ctypes
@@ -246,6 +246,20 @@ def inv_main():
                     out[key + "_A"] = L
                     out[key + "_B"] = B
                     out[key + "_X"] = dtrmm(side, trans, diag, 0.75, L, B)
+    # PSTRF (upper): full-rank SPD and low-rank PSD matrices with distinct diagonals
+    for n, r in ((5, 5), (16, 16), (32, 32), (33, 33), (48, 48), (64, 64), (20, 7), (40, 13), (64, 30)):
+        Bm = rng.standard_normal((n, r))
+        A = np.asfortranarray(Bm @ Bm.T + (np.diag(rng.uniform(0.5, 1.5, n)) if r == n else 0.0))
+        R = np.array(A, order="F", copy=True)
+        piv = np.zeros(n, dtype=np.int32)
+        rank = ci(0); info = ci(0)
+        work = np.zeros(2 * n)
+        mkl.dpstrf_(byr(cc(b"U")), byr(ci(n)), P(R), byr(ci(n)), P(piv), byr(rank), byr(cd(-1.0)), P(work),
+                    byr(info))
+        out["pstrf_A_%d_%d" % (n, r)] = A
+        out["pstrf_R_%d_%d" % (n, r)] = np.triu(R)
+        out["pstrf_piv_%d_%d" % (n, r)] = piv - 1
+        out["pstrf_rank_%d_%d" % (n, r)] = np.array(rank.value)
     np.savez_compressed(os.path.join(HERE, "lapack_golden_inv.npz"), **out)
     print("wrote", len(out), "arrays with", MKL_VERSION)
 

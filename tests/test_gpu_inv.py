@@ -242,3 +242,92 @@ def test_inv_edge_cases(h):
     assert h.lib.kblasDpotri_batch_strided(h.h, b"L", 4, kt.P(d), 3, 16, 2, kt.P(info)) == pykblas.KBLAS_Error_WrongInput
     assert h.lib.kblasDpotri_batch_strided(h.h, b"L", 1025, kt.P(d), 1025, 1025 * 1025, 1,
                                            kt.P(info)) == pykblas.KBLAS_NotImplemented
+
+
+# ------------------------------------------------------------------ PSTRF ----
+def pstrf_inputs(n, r, batch, seed):
+    g = np.random.default_rng(seed)
+    A = po.batch_array(batch, n, n)
+    for i in range(batch):
+        Bm = g.standard_normal((n, r))
+        M = Bm @ Bm.T + (np.diag(g.uniform(0.5, 1.5, n)) if r == n else 0.0)
+        M[np.tril_indices(n, -1)] = 1e30          # only the upper triangle is read
+        A[i] = M.T
+    return A
+
+
+@pytest.mark.parametrize("n,r", [(1, 1), (5, 5), (16, 16), (31, 9), (32, 32), (33, 33), (48, 20), (64, 64), (64, 30)])
+def test_dpstrf_strided_matches_oracle(h, n, r):
+    batch = 6
+    torch = kt.torch_dev()
+    A = pstrf_inputs(n, r, batch, 100 + n + r)
+    dA = kt.to_dev(A)
+    piv = torch.full((batch, n), -1, dtype=torch.int32, device="cuda")
+    ranks = torch.full((batch,), -1, dtype=torch.int32, device="cuda")
+    st = h.lib.kblasDpstrf_batch_strided(h.h, n, kt.P(dA), n, n * n, kt.P(piv), n, kt.P(ranks), batch)
+    assert st == pykblas.KBLAS_Success
+    R, pv, rk = kt.to_host(dA), kt.to_host(piv), kt.to_host(ranks)
+    for i in range(batch):
+        Ro, po_piv, ro = po.pstrf(po.col(A, i))
+        assert rk[i] == ro == r
+        assert np.array_equal(pv[i], po_piv)                      # integer output bit-exact
+        Rg = po.col(R, i)
+        assert np.array_equal(np.tril(Rg, -1), np.zeros((n, n)))
+        assert rel(np.triu(Rg)[:r], np.triu(Ro)[:r]) <= 16 * n * EPS
+
+
+def test_dpstrf_ties_and_pointer_array(h):
+    """Identity matrices (all pivots tie): pivots follow the reference's reduction, as the
+    oracle; pointer-array and strided calls give identical results."""
+    torch = kt.torch_dev()
+    n, batch = 40, 3
+    A = po.batch_array(batch, n, n)
+    for i in range(batch):
+        A[i] = np.eye(n) * (i + 1)
+    d1, d2 = kt.to_dev(A), kt.to_dev(A)
+    p1 = torch.zeros((batch, n), dtype=torch.int32, device="cuda")
+    p2 = torch.zeros((batch, n), dtype=torch.int32, device="cuda")
+    r1 = torch.zeros(batch, dtype=torch.int32, device="cuda")
+    r2 = torch.zeros(batch, dtype=torch.int32, device="cuda")
+    assert h.lib.kblasDpstrf_batch_strided(h.h, n, kt.P(d1), n, n * n, kt.P(p1), n, kt.P(r1), batch) == pykblas.KBLAS_Success
+    pA = kt.ptr_array(d2, batch, n * n)
+    pP = kt.ptr_array(p2, batch, n)
+    assert h.lib.kblasDpstrf_batch(h.h, n, kt.P(pA), n, kt.P(pP), kt.P(r2), batch) == pykblas.KBLAS_Success
+    assert np.array_equal(kt.to_host(d1), kt.to_host(d2))
+    assert np.array_equal(kt.to_host(p1), kt.to_host(p2)) and np.array_equal(kt.to_host(r1), kt.to_host(r2))
+    _, po_piv, _ = po.pstrf(np.eye(n))
+    assert all(np.array_equal(row, po_piv) for row in kt.to_host(p1))
+
+
+@pytest.mark.parametrize("n", [16, 64])
+def test_spstrf_strided(h, n):
+    batch = 4
+    torch = kt.torch_dev()
+    A = pstrf_inputs(n, n, batch, 7 + n)
+    A32 = A.astype(np.float32)
+    dA = kt.to_dev(A32)
+    piv = torch.zeros((batch, n), dtype=torch.int32, device="cuda")
+    ranks = torch.zeros(batch, dtype=torch.int32, device="cuda")
+    assert h.lib.kblasSpstrf_batch_strided(h.h, n, kt.P(dA), n, n * n, kt.P(piv), n, kt.P(ranks), batch) == pykblas.KBLAS_Success
+    R, pv = kt.to_host(dA), kt.to_host(piv)
+    for i in range(batch):
+        M = np.triu(po.col(A32, i).astype(np.float64))
+        M = M + np.triu(M, 1).T
+        Pm = np.eye(n)[:, pv[i]]
+        Rg = np.triu(po.col(R, i)).astype(np.float64)
+        assert rel(Rg.T @ Rg, Pm.T @ M @ Pm) <= 16 * n * kt.EPS32
+        assert sorted(pv[i].tolist()) == list(range(n))
+
+
+def test_pstrf_edge_cases(h):
+    torch = kt.torch_dev()
+    d = torch.zeros(65 * 65, dtype=torch.float64, device="cuda")
+    piv = torch.zeros(65, dtype=torch.int32, device="cuda")
+    ranks = torch.full((2,), -1, dtype=torch.int32, device="cuda")
+    assert h.lib.kblasDpstrf_batch_strided(h.h, 65, kt.P(d), 65, 65 * 65, kt.P(piv), 65, kt.P(ranks), 1) == pykblas.KBLAS_Error_WrongInput
+    assert h.lib.kblasDpstrf_batch_strided(h.h, 0, kt.P(d), 1, 0, kt.P(piv), 0, kt.P(ranks), 2) == pykblas.KBLAS_Success
+    assert list(kt.to_host(ranks)) == [0, 0]
+    # zero matrix: rank 0, nothing factored
+    z = torch.zeros(16, dtype=torch.float64, device="cuda")
+    assert h.lib.kblasDpstrf_batch_strided(h.h, 4, kt.P(z), 4, 16, kt.P(piv), 4, kt.P(ranks), 1) == pykblas.KBLAS_Success
+    assert kt.to_host(ranks)[0] == 0 and np.array_equal(kt.to_host(piv)[:4], np.arange(4))

@@ -137,3 +137,49 @@ def test_planted_mistake_is_caught():
     Xbad = np.tril(X).copy()
     Xbad[5:, 3] *= 1.0 + 1e-9
     assert relmax(np.tril(L) @ Xbad, np.eye(16)) > 8 * 16 * EPS
+
+
+PSTRF_CASES = [(5, 5), (16, 16), (32, 32), (33, 33), (48, 48), (64, 64), (20, 7), (40, 13), (64, 30)]
+
+
+@pytest.mark.parametrize("n,r", PSTRF_CASES)
+def test_pstrf_matches_lapack(n, r):
+    """MKL dpstrf('U', tol < 0): same pivots and rank, R within c n eps (distinct diagonals, no ties)."""
+    A = G["pstrf_A_%d_%d" % (n, r)]
+    R, piv, rank = po.pstrf(A)
+    assert rank == int(G["pstrf_rank_%d_%d" % (n, r)]) == r
+    assert np.array_equal(piv[:rank], G["pstrf_piv_%d_%d" % (n, r)][:rank])
+    assert relmax(np.triu(R)[:rank], G["pstrf_R_%d_%d" % (n, r)][:rank]) <= 8 * n * EPS
+    assert np.array_equal(np.tril(R, -1), np.zeros((n, n)))
+    # P^T A P = R^T R on the leading rank rows, the diagonal of R non-increasing
+    Pm = np.eye(n)[:, piv]
+    Rr = np.triu(R)[:rank]
+    assert relmax((Pm.T @ A @ Pm)[:rank, :rank], (Rr.T @ Rr)[:rank, :rank]) <= 16 * n * EPS
+    d = np.diag(R)[:rank]
+    assert np.all(d[1:] <= d[:-1] * (1 + 1e-12))
+    assert sorted(piv.tolist()) == list(range(n))
+
+
+def test_pstrf_tie_rule_on_identity():
+    """Equal diagonals: the pivot is the reference's 32-slot butterfly winner, not the first
+    maximum (LAPACK's choice); for I_4 step 1 picks index 2 (slot 0 keeps the even-side
+    winner 2 against the odd-side 1 on equal values)."""
+    R, piv, rank = po.pstrf(np.eye(4))
+    assert rank == 4 and list(piv) == [0, 2, 1, 3]
+    assert np.array_equal(R, np.eye(4))
+
+
+def test_pstrf_diagonal_closed_form():
+    d = np.array([3.0, 9.0, 1.0, 4.0, 0.0])
+    R, piv, rank = po.pstrf(np.diag(d))
+    assert rank == 4 and list(piv[:4]) == [1, 3, 0, 2]
+    assert np.array_equal(np.diag(R)[:4], np.sqrt(d[[1, 3, 0, 2]]))
+
+
+def test_pstrf_reads_upper_triangle_only():
+    A = G["pstrf_A_16_16"].copy()
+    B = A.copy()
+    B[np.tril_indices(16, -1)] = 1e30
+    R1, p1, r1 = po.pstrf(A)
+    R2, p2, r2 = po.pstrf(B)
+    assert r1 == r2 and np.array_equal(p1, p2) and np.array_equal(R1, R2)
