@@ -52,8 +52,9 @@ template <typename T>
 __global__ void __launch_bounds__(64)
 k_pstrf(int n, BatchArg<T> A, int* const* pivp, int* pivs, long spiv, int* ranks, int batch)
 {
-    constexpr int LD = 65;
-    __shared__ T S[64 * LD];
+    extern __shared__ __align__(16) unsigned char kb_ps_lds[];
+    T* S = reinterpret_cast<T*>(kb_ps_lds);   // n x (n + 1): sized to the order for occupancy
+    const int LD = n + 1;
     const long b = blockIdx.x;
     if (b >= batch) return;
     const int t = threadIdx.x;
@@ -99,6 +100,7 @@ k_pstrf(int n, BatchArg<T> A, int* const* pivp, int* pivs, long spiv, int* ranks
         __syncthreads();
         if (t > k && t < n) {
             const T u = S[k + t * LD];
+#pragma unroll 8
             for (int c = k + 1; c < n; ++c) S[t + c * LD] = fma(-u, S[k + c * LD], S[t + c * LD]);
         }
         __syncthreads();
@@ -113,11 +115,12 @@ extern "C" int kb_launch_pstrf(char prec, int n, kb_mat_t A, int* const* pivp, i
                                int batch, hipStream_t s)
 {
     if (n > 64) return (int)hipErrorNotSupported;
+    const size_t lds = (size_t)n * (n + 1) * (prec == 'd' ? sizeof(double) : sizeof(float));
     if (prec == 'd')
-        hipLaunchKernelGGL(k_pstrf<double>, dim3(batch), dim3(64), 0, s, n, to_arg<double>(A), pivp, pivs, spiv,
+        hipLaunchKernelGGL(k_pstrf<double>, dim3(batch), dim3(64), lds, s, n, to_arg<double>(A), pivp, pivs, spiv,
                            ranks, batch);
     else
-        hipLaunchKernelGGL(k_pstrf<float>, dim3(batch), dim3(64), 0, s, n, to_arg<float>(A), pivp, pivs, spiv,
+        hipLaunchKernelGGL(k_pstrf<float>, dim3(batch), dim3(64), lds, s, n, to_arg<float>(A), pivp, pivs, spiv,
                            ranks, batch);
     return (int)hipGetLastError();
 }
