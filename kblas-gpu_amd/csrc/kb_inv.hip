@@ -109,15 +109,15 @@ k_inv(int op, int unit, int n, BatchArg<T> A, int* info, const int* skip, int st
     if (staged) {
         T* S = reinterpret_cast<T*>(kb_inv_lds);
         const int ls = n + 1;   // odd leading dimension: column-wise reads spread over the banks
-        for (long e = r; e < (long)n * n; e += nt) {
-            const int i = (int)(e % n), j = (int)(e / n);
+        for (unsigned e = r; e < (unsigned)(n * n); e += nt) {   // staged sizes fit 32 bits
+            const int i = (int)(e % (unsigned)n), j = (int)(e / (unsigned)n);
             if (i >= j) S[i + (long)j * ls] = G[i + (long)j * lda];
         }
         __syncthreads();
         inf = inv_run<T>(op, unit, S, ls, n, r, &sflag);
         if (inf == 0)
-            for (long e = r; e < (long)n * n; e += nt) {
-                const int i = (int)(e % n), j = (int)(e / n);
+            for (unsigned e = r; e < (unsigned)(n * n); e += nt) {   // staged sizes fit 32 bits
+                const int i = (int)(e % (unsigned)n), j = (int)(e / (unsigned)n);
                 if (i >= j && !(unit && i == j)) G[i + (long)j * lda] = S[i + (long)j * ls];
             }
     } else {
@@ -263,8 +263,8 @@ k_trmm(int side_left, int trans, int unit, int m, int n, T alpha, BatchArg<T> A,
     const T* L = GA;
     int ldl = lda;
     if (stageA) {
-        for (long e = t; e < (long)ord * ord; e += nt) {
-            const int i = (int)(e % ord), j = (int)(e / ord);
+        for (unsigned e = t; e < (unsigned)(ord * ord); e += nt) {   // staged sizes fit 32 bits
+            const int i = (int)(e % (unsigned)ord), j = (int)(e / (unsigned)ord);
             if (i >= j) SA[i + (long)j * (ord + 1)] = GA[i + (long)j * lda];
         }
         L = SA;
@@ -276,13 +276,13 @@ k_trmm(int side_left, int trans, int unit, int m, int n, T alpha, BatchArg<T> A,
         const int w = total - p0 < panel ? total - p0 : panel;
         const int prow = side_left ? m : w, pcol = side_left ? w : n;
         __syncthreads();   // previous panel's reads done (and the triangle staged)
-        for (long e = t; e < (long)prow * pcol; e += nt) {
-            const int i = (int)(e % prow), j = (int)(e / prow);
+        for (unsigned e = t; e < (unsigned)(prow * pcol); e += nt) {   // staged sizes fit 32 bits
+            const int i = (int)(e % (unsigned)prow), j = (int)(e / (unsigned)prow);
             SP[e] = side_left ? GB[i + (long)(p0 + j) * ldb] : GB[p0 + i + (long)j * ldb];
         }
         __syncthreads();
-        for (long e = t; e < (long)prow * pcol; e += nt) {
-            const int i = (int)(e % prow), j = (int)(e / prow);
+        for (unsigned e = t; e < (unsigned)(prow * pcol); e += nt) {   // staged sizes fit 32 bits
+            const int i = (int)(e % (unsigned)prow), j = (int)(e / (unsigned)prow);
             T v = T(0);
             if (alpha != T(0)) v = alpha * trmm_elem<T>(side_left, trans, unit, L, ldl, ord, SP, prow, i, j);
             if (side_left) GB[i + (long)(p0 + j) * ldb] = v;
