@@ -245,8 +245,8 @@ def main():
                          "(the reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (the batched SPD path shards by matrix with "
-              "no exchange; see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py: measures one GPU only (the batched SPD path shards by matrix, each GPU a contiguous "
+              "range of the batch, with no exchange step); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch

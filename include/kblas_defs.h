@@ -33,7 +33,7 @@
 #define KBLAS_MAGMA_Error -11
 #define KBLAS_SVD_NoConvergence -12
 
-/* The HIP runtime is this build's device API; the reference's CUDA-named code
+/* The HIP runtime is the device API here; the reference's CUDA-named code
  * keeps its value and meaning (a device-runtime call failed). */
 #define KBLAS_HIP_Error KBLAS_CUDA_Error
 

@@ -1,7 +1,7 @@
 """GPU tests of the mixed-precision POSV (kblasDsposv_batch*): FP32 factor +
 FP64 refinement, checked against the FP64 oracle.
 
-Bar (north star): normwise backward error ||b - A x||inf / (||A||inf ||x||inf
+Bar: normwise backward error ||b - A x||inf / (||A||inf ||x||inf
 + ||b||inf) <= 1e-13 for every solve; forward error within kappa * 1e-13 of
 the FP64 oracle's solution; iteration counts within 2 of the oracle's dsposv
 restatement; fallback (iter < 0) exactly when the FP32 factor cannot serve.
