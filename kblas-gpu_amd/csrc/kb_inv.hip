@@ -7,7 +7,7 @@
 //   TRMM   B := alpha op(L) B  or  alpha B op(L)
 //                                    src/batch_triangular/Xtrmm_batch_kernels.cuh:37-175
 //
-// Orders up to 32: k_inv_reg, several matrices per wave, registers (below).
+// Orders up to 64: k_inv_reg, 64 / NL matrices per wave, registers (below).
 // Larger orders: k_inv, one workgroup per matrix, one thread per row (TRTRI)
 // or column (LAUUM) of the matrix, so n <= KB_INV_MAX_N.  The matrix is staged in LDS when its
 // n x n block fits in KB_INV_LDS bytes and is worked on in place in HBM
@@ -126,8 +126,8 @@ k_inv(int op, int unit, int n, BatchArg<T> A, int* info, const int* skip, int st
     if (info && r == 0) info[b] = inf;
 }
 
-// ------------------------------------------------- register form, n <= 32 ----
-// One wave holds 64 / NL matrices (NL = 16 or 32 lanes each).  The original
+// ------------------------------------------------- register form, n <= 64 ----
+// One wave holds 64 / NL matrices (NL = 16, 32 or 64 lanes each).  The original
 // triangle is staged once in LDS (zero outside n x n) and only read after
 // that, by broadcast: TRTRI keeps row r of X = L^-1 in lane r's registers and
 // needs column j of A at step j; LAUUM keeps column c of X in lane c and needs
@@ -303,6 +303,11 @@ static int launch_inv(int op, int unit, int n, kb_mat_t A, int* info, const int*
     if (n <= 32) {
         hipLaunchKernelGGL((k_inv_reg<T, 32>), dim3((batch + 1) / 2), dim3(64), 0, s, op, unit, n, to_arg<T>(A), info,
                            skip, batch);
+        return (int)hipGetLastError();
+    }
+    if (n <= 64) {
+        hipLaunchKernelGGL((k_inv_reg<T, 64>), dim3(batch), dim3(64), 0, s, op, unit, n, to_arg<T>(A), info, skip,
+                           batch);
         return (int)hipGetLastError();
     }
     const size_t need = (size_t)n * (n + 1) * sizeof(T);

@@ -19,7 +19,8 @@ import pyoracle as po
 pytestmark = pytest.mark.gpu
 EPS = kt.EPS64
 RNG = np.random.default_rng(77)
-# 1..64: one wave; 65..: several waves; staged in LDS up to n = 87 (FP64) / 123 (FP32)
+# register form up to 64 (16, 32 or 64 lanes per matrix); above: one workgroup per matrix,
+# staged in LDS up to n = 87 (FP64) / 123 (FP32), in place in HBM beyond
 SIZES = [1, 2, 5, 16, 31, 32, 33, 64, 65, 88, 100, 130]
 
 
