@@ -225,19 +225,23 @@ KB_DEV T trmm_elem(int side_left, int trans, int unit, const T* L, int ldl, int 
     if (side_left) {
         const T* pc = P + (long)c * ldp;
         if (!trans) {   // sum_{k<=r} L(r,k) B(k,c)
+#pragma unroll 8
             for (int k = 0; k < r; ++k) s = fma(L[r + (long)k * ldl], pc[k], s);
             s = unit ? s + pc[r] : fma(L[r + (long)r * ldl], pc[r], s);
         } else {        // sum_{k>=r} L(k,r) B(k,c)
             const T* lr = L + (long)r * ldl;
             s = unit ? pc[r] : lr[r] * pc[r];
+#pragma unroll 8
             for (int k = r + 1; k < ord; ++k) s = fma(lr[k], pc[k], s);
         }
     } else {
         if (!trans) {   // sum_{k>=c} B(r,k) L(k,c)
             const T* lc = L + (long)c * ldl;
             s = unit ? P[r + (long)c * ldp] : lc[c] * P[r + (long)c * ldp];
+#pragma unroll 8
             for (int k = c + 1; k < ord; ++k) s = fma(P[r + (long)k * ldp], lc[k], s);
         } else {        // sum_{k<=c} B(r,k) L(c,k)
+#pragma unroll 8
             for (int k = 0; k < c; ++k) s = fma(P[r + (long)k * ldp], L[c + (long)k * ldl], s);
             s = unit ? s + P[r + (long)c * ldp] : fma(P[r + (long)c * ldp], L[c + (long)c * ldl], s);
         }
