@@ -927,7 +927,8 @@ int orc_pstrf(int n, double* A, int lda, int* piv)
         double val;
         int p;
         pstrf_maxdiag(S, ld, n, k, &val, &p);
-        if (val <= tol) { rank = k; break; }
+        /* p outside [k, n): the -1 placeholder beat a tolerance below -1 (not semi-definite) */
+        if (val <= tol || p < k || p >= n) { rank = k; break; }
         if (p != k) {
             int ti = piv[k]; piv[k] = piv[p]; piv[p] = ti;
             for (int c = 0; c < n; ++c) { double t = S[k + (long)c * ld]; S[k + (long)c * ld] = S[p + (long)c * ld]; S[p + (long)c * ld] = t; }
