@@ -449,6 +449,23 @@ static void spotrs_vec(int n, const float* L, float* x)
     }
 }
 
+/* LAPACK dsposv (reference LAPACK 3.x, SRC/dsposv.f; the CPU comparator is
+ * MKL's build of it, and tests/golden/lapack_golden_mixed.npz holds its ITER
+ * and X): CTE = ||A||_inf * eps * sqrt(n); B and the lower triangle of A are
+ * converted to FP32 with DLAG2S / DLAT2S's overflow check (|v| > FLT_MAX gives
+ * ITER = -2); SPOTRF (failure: ITER = -3); x = SPOTRS(b); then up to
+ * ITERMAX = 30 times: r = b - A x (DSYMM, FP64), stop when every right-hand side
+ * has ||r||_inf <= ||x||_inf * CTE (ITER = the number of corrections), else
+ * convert r to FP32 (overflow: ITER = -2), solve, x += d.  No convergence gives
+ * ITER = -31.  Every failure ends in the FP64 path: X := B, DPOTRF, DPOTRS.
+ * One deliberate difference: a NaN in r counts as not converged (IDAMAX in
+ * reference BLAS skips NaNs, so LAPACK can report convergence on a NaN
+ * residual); NaN-free inputs take identical decisions. */
+static int fits_fp32(double v)
+{
+    return !(v > 3.4028234663852886e38 || v < -3.4028234663852886e38);
+}
+
 int orc_sposv(char side, int m, int n, const double* A, int lda,
               const double* B, int ldb, double* X, int ldx, int* iter)
 {
@@ -463,20 +480,36 @@ int orc_sposv(char side, int m, int n, const double* A, int lda,
             Af[i + (long)j * order] = IDX(A, lda, i, j);
             Af[j + (long)i * order] = IDX(A, lda, i, j);
         }
-    double anrm = 0.0;  /* infinity norm */
+    double anrm = 0.0;  /* infinity norm (DLANSY 'I') */
     for (int i = 0; i < order; ++i) {
         double s = 0.0;
         for (int j = 0; j < order; ++j) s += fabs(Af[i + (long)j * order]);
-        if (s > anrm) anrm = s;
+        if (s > anrm || s != s) anrm = s;
     }
     double cte = anrm * 1.1102230246251565e-16 * sqrt((double)order);
     float* L = (float*)malloc(sizeof(float) * (size_t)order * order);
-    for (long t = 0; t < (long)order * order; ++t) L[t] = (float)Af[t];
     float* xs = (float*)malloc(sizeof(float) * (size_t)order);
     double* r = (double*)malloc(sizeof(double) * (size_t)order);
-    int sinfo = orc_spotrf(order, L, order);
     int result_iter = 0, ok = 0;
-    if (sinfo == 0) {
+    /* DLAG2S(B), then DLAT2S(lower A) */
+    int ovf = 0;
+    for (int c = 0; c < nrhs && !ovf; ++c) {
+        long ib;
+        const double* b = rhs_ptr(side, (double*)B, ldb, c, &ib);
+        for (int i = 0; i < order; ++i)
+            if (!fits_fp32(b[i * ib])) { ovf = 1; break; }
+    }
+    for (int j = 0; j < order && !ovf; ++j)
+        for (int i = j; i < order; ++i)
+            if (!fits_fp32(Af[i + (long)j * order])) { ovf = 1; break; }
+    if (ovf) {
+        result_iter = -2;
+    } else {
+        for (long t = 0; t < (long)order * order; ++t) L[t] = (float)Af[t];
+        int sinfo = orc_spotrf(order, L, order);
+        if (sinfo != 0) result_iter = -3;
+    }
+    if (result_iter == 0) {
         /* x0 = solve(b) */
         for (int c = 0; c < nrhs; ++c) {
             long ib, ix;
@@ -486,7 +519,7 @@ int orc_sposv(char side, int m, int n, const double* A, int lda,
             spotrs_vec(order, L, xs);
             for (int i = 0; i < order; ++i) x[i * ix] = (double)xs[i];
         }
-        for (int it = 0; it <= itermax; ++it) {
+        for (int it = 0; it <= itermax && !ok && result_iter == 0; ++it) {
             /* residual and convergence test over all right-hand sides */
             int conv = 1;
             for (int c = 0; c < nrhs; ++c) {
@@ -498,29 +531,28 @@ int orc_sposv(char side, int m, int n, const double* A, int lda,
                     double s = b[i * ib];
                     for (int k = 0; k < order; ++k) s = fma(-Af[i + (long)k * order], x[k * ix], s);
                     r[i] = s;
-                    if (fabs(s) > rn) rn = fabs(s);
+                    if (fabs(s) > rn || s != s) rn = fabs(s);
                     if (fabs(x[i * ix]) > xn) xn = fabs(x[i * ix]);
                 }
                 if (!(rn <= xn * cte)) conv = 0;
             }
             if (conv) { result_iter = it; ok = 1; break; }
-            if (it == itermax) break;
-            for (int c = 0; c < nrhs; ++c) {
+            if (it == itermax) { result_iter = -(itermax + 1); break; }
+            for (int c = 0; c < nrhs && result_iter == 0; ++c) {
                 long ib, ix;
                 const double* b = rhs_ptr(side, (double*)B, ldb, c, &ib);
                 double* x = rhs_ptr(side, X, ldx, c, &ix);
                 for (int i = 0; i < order; ++i) {
                     double s = b[i * ib];
                     for (int k = 0; k < order; ++k) s = fma(-Af[i + (long)k * order], x[k * ix], s);
+                    if (!fits_fp32(s)) { result_iter = -2; break; }   /* DLAG2S(R) */
                     xs[i] = (float)s;
                 }
+                if (result_iter) break;
                 spotrs_vec(order, L, xs);
                 for (int i = 0; i < order; ++i) x[i * ix] = x[i * ix] + (double)xs[i];
             }
         }
-        if (!ok) result_iter = -(itermax + 1);
-    } else {
-        result_iter = -3;
     }
     int info = 0;
     if (!ok) {

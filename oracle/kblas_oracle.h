@@ -85,7 +85,8 @@ int  orc_posv(char side, int m, int n, double* A, int lda, double* B, int ldb);
 
 /* Mixed precision (LAPACK dsposv restated).  A is read-only; X receives the
  * solution.  Returns info; *iter = #refinement steps, or -(reason) when it
- * fell back to FP64 (-3: FP32 factor failed, -31: no convergence).          */
+ * fell back to FP64 (-2: an entry of B, A or a residual beyond the FP32
+ * range, -3: FP32 factor failed, -31: no convergence).                      */
 int  orc_sposv(char side, int m, int n, const double* A, int lda,
                const double* B, int ldb, double* X, int ldx, int* iter);
 

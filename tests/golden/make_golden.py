@@ -12,7 +12,8 @@ expected values, but seeded and committed:
     (dpotrf, then dtrsm(R,L,T,N) and dtrsm(R,L,N,N)).
 The inverse family (dtrtri / dlauum / dpotri, and dtrmm) and dpstrf go to a file of
 their own, lapack_golden_inv.npz, written by ``--inv`` from its own seeded generator.
-Run in the build container only (needs MKL): python tests/golden/make_golden.py [--inv]
+Mixed precision (MKL dsposv / spotrf) goes to lapack_golden_mixed.npz, written by ``--mixed``.
+Run in the build container only (needs MKL): python tests/golden/make_golden.py [--inv | --mixed]
 """
 import ctypes
 import os
@@ -264,8 +265,91 @@ def inv_main():
     print("wrote", len(out), "arrays with", MKL_VERSION)
 
 
+def dsposv(A, B):
+    """MKL dsposv('L') on copies: returns (X, ITER, INFO); B is n x nrhs."""
+    Aw = np.array(A, order="F", copy=True)
+    Bw = np.asfortranarray(B, dtype=np.float64)
+    n, nrhs = Bw.shape
+    X = np.zeros((n, nrhs), order="F")
+    work = np.zeros(n * nrhs)
+    swork = np.zeros(n * (n + nrhs), dtype=np.float32)
+    it = ci(0); info = ci(0)
+    mkl.dsposv_(byr(cc(b"L")), byr(ci(n)), byr(ci(nrhs)), P(Aw), byr(ci(n)), P(Bw), byr(ci(n)), P(X), byr(ci(n)),
+                P(work), P(swork), byr(it), byr(info))
+    return X, it.value, info.value
+
+
+def spotrf(A32):
+    L = np.array(A32, dtype=np.float32, order="F", copy=True)
+    n = ci(L.shape[0]); info = ci(0)
+    mkl.spotrf_(byr(cc(b"L")), byr(n), P(L), byr(n), byr(info))
+    return L, info.value
+
+
+def mixed_main():
+    """Mixed precision (LAPACK dsposv): the inputs are the oracle's own seeded
+    Q diag(s) Q^T generator (seed, matrix id, order, log10 cond range), which the
+    fixture records together with a checksum of each generated A so the test can
+    regenerate it and prove it is the same matrix; the expected outputs are MKL's
+    ITER, INFO and X.  spotrf fixtures hold their (small) inputs explicitly."""
+    sys.path.insert(0, os.path.join(HERE, "..", "..", "oracle"))
+    import pyoracle as po
+    out = {"mkl_version": np.array(MKL_VERSION)}
+    cases = []
+    # n = 64 family of the headline workload (cond 1e2..1e6, 4 right-hand sides) and smaller orders
+    for n, cnt in ((16, 6), (32, 6), (64, 12)):
+        for mid in range(cnt):
+            cases.append((n, 4, 2.0, 6.0, 7100 + n, mid))
+    # the large non-uniform family (cond 1e2..1e4), fewer right-hand sides to keep the fixture small
+    for n, cnt in ((128, 2), (200, 2), (256, 2), (333, 1), (512, 2)):
+        for mid in range(cnt):
+            cases.append((n, 8, 2.0, 4.0, 7500 + n, mid))
+    # beyond FP32: cond 1e7..1e9 (refinement stalls or the FP32 factor fails) and 1e12
+    for n, lo, hi, cnt in ((32, 7.0, 9.0, 4), (64, 7.5, 8.5, 4), (48, 12.0, 12.0, 1)):
+        for mid in range(cnt):
+            cases.append((n, 3, lo, hi, 7900 + n, mid))
+    meta = []
+    for k, (n, nrhs, lo, hi, seed, mid) in enumerate(cases):
+        A, kappa = po.gen_spd_cond(seed, mid, n, lo, hi)
+        B = po.gen_rhs(seed + 1, mid, n, nrhs)
+        X, it, info = dsposv(A, B)
+        meta.append([n, nrhs, lo, hi, seed, mid, kappa, it, info, float(np.sum(A)), float(np.sum(A * A))])
+        out["sposv_X_%d" % k] = X
+    out["sposv_meta"] = np.array(meta)
+    # ITER = -2: an entry of B beyond the FP32 range (LAPACK's DLAG2S check)
+    A, _ = po.gen_spd_cond(7990, 0, 16, 2.0, 3.0)
+    B = po.gen_rhs(7991, 0, 16, 2)
+    B[3, 1] = 1e39
+    X, it, info = dsposv(A, B)
+    out["sposv_ovf_A"] = A
+    out["sposv_ovf_B"] = B
+    out["sposv_ovf_X"] = X
+    out["sposv_ovf_iter"] = np.array(it)
+    # ITER = -2 from A: scaled beyond the FP32 range (B within it)
+    As = A * 1e39
+    X, it, info = dsposv(As, B / 1e30)
+    out["sposv_ovfA_A"] = As
+    out["sposv_ovfA_B"] = B / 1e30
+    out["sposv_ovfA_X"] = X
+    out["sposv_ovfA_iter"] = np.array(it)
+    # spotrf on FP32 copies of the same family (orders 16..64)
+    for n in (16, 33, 64):
+        A, _ = po.gen_spd_cond(7800 + n, 0, n, 2.0, 4.0)
+        A32 = np.asfortranarray(A.astype(np.float32))
+        L, info = spotrf(A32)
+        assert info == 0
+        out["spotrf_A_%d" % n] = A32
+        out["spotrf_L_%d" % n] = np.tril(L)
+    np.savez_compressed(os.path.join(HERE, "lapack_golden_mixed.npz"), **out)
+    print("wrote", len(out), "arrays with", MKL_VERSION)
+    its = [int(m[7]) for m in meta]
+    print("ITER values:", its)
+
+
 if __name__ == "__main__":
     if "--inv" in sys.argv[1:]:
         inv_main()
+    elif "--mixed" in sys.argv[1:]:
+        mixed_main()
     else:
         main()
