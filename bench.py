@@ -1,23 +1,39 @@
 #!/usr/bin/env python3
-"""Benchmark of the batched SPD path on one MI355X.
+"""Benchmark of the batched SPD (Cholesky) path on one MI355X.
 
-Default workload (the headline metric): 50,000 SPD 64x64 systems,
-A = Q diag(s) Q^T with Q Haar-orthogonal (Stewart: product of Householder
-reflectors from Gaussian vectors) and s log-spaced
-so cond(A) = kappa, log10(kappa) ~ U[2, 6]; 4 right-hand sides ~ N(0,1) in the
-reference's Right-side layout (B is 4 x 64 per system).  One step = one
-mixed-precision batched POSV (FP32 factor + FP64 refinement, FP64 fallback)
-over the whole batch; the plain FP64 batched POSV on the same inputs is timed
-the same way.  value = mixed solves/s.
+One run times every workload of the path and prints ONE JSON line.  The
+headline (the line's top-level metric/value) is the mixed-precision batched
+POSV: 50,000 SPD 64x64 systems A = Q diag(s) Q^T (Q Haar-orthogonal, built as
+a product of Householder reflectors from Gaussian vectors, s log-spaced with
+pinned endpoints so cond(A) = kappa, log10(kappa) ~ U[2, 6]), 4 right-hand
+sides ~ N(0,1) in the reference's Right-side layout (B is 4 x 64 per system);
+one step = one kblasDsposv_batch_strided call over the whole batch (FP32
+factor + FP64 refinement, FP64 fallback), the plain FP64 kblasDposv on the
+same inputs timed beside it.  The other workloads go under "configs":
+  c1  DPOTRF n=32 x 10,000 (A = B B^T + 32 I)
+  c2  DPOSV  n=16 x 100,000, 1 RHS
+  c3  DPOSV  non-uniform n ~ U{8..256} x 20,000, 4 RHS, pointer arrays
+  c5  DSPOSV non-uniform n ~ U{128..512} x 2,000, 64 RHS, cond in [1e2, 1e4]
 
-Inputs are generated on the GPU with torch (plumbing only) and staged W+K
-times in HBM, so the timed loop holds exactly K library calls back to back,
-bracketed by synchronisation, timed with HIP events on the library's stream.
-The CPU baseline (the oracle's OpenMP FP64 POSV restatement, and LAPACK when a
-library is loadable) runs on a bounded sample of the same inputs, which also
-yields the per-solve backward errors of the GPU results.
+Every step is one library call captured as a HIP graph (torch.cuda.CUDAGraph
+on a side stream) and replayed; inputs are generated on the GPU with torch
+(plumbing only) and are resident in HBM before timing starts.  c1/c2/c4 stage
+W+K copies so the K timed replays run back to back between two
+synchronisations; c3/c5 restore their inputs with a device copy between
+replays, outside the per-step events.  Each step is bracketed by HIP events
+on the replay stream: ms_per_step is the mean, and the median and minimum are
+reported beside it.  roofline.achieved = algorithmic bytes (or flops) per
+step / step time.
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c4]
+cpu_baseline is the reference test's CPU comparator: looped LAPACK (MKL,
+sequential per thread) dpotrf + dtrsm(R,L,T) + dtrsm(R,L,N), as in
+testing/batch_triangular/test_Xposv_batch.cpp:313-326 (dpotrf alone for
+POTRF, test_Xpotrf_batch.cpp:293-305), over a bounded sample of the same
+inputs, OpenMP over the batch on every host core, warmed up, median of 5;
+a single-thread figure and the oracle's own C restatement are reported
+beside it.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config all|c1|c2|c3|c4|c5|poti]
 """
 import argparse
 import json
@@ -31,100 +47,137 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "kblas-gpu_amd"))
 
 HBM_PEAK_GBS = 8000.0
+DEV = "cuda"
+FP64_PEAK_GFS = 78600.0
+FP32_PEAK_GFS = 157300.0
 
 CONFIGS = {
-    "c1": dict(op="potrf", n=32, batch=10000, nrhs=0, gen="shift",
+    "c1": dict(op="potrf", n=32, batch=10000, nrhs=0, gen="shift", seed=1,
                workload="dpotrf_batch_strided n=32 x 10000, A=B*B^T+32I"),
-    "c2": dict(op="posv", n=16, batch=100000, nrhs=1, gen="shift",
+    "c2": dict(op="posv", n=16, batch=100000, nrhs=1, gen="shift", seed=2,
                workload="dposv_batch_strided n=16 x 100000, 1 RHS, A=B*B^T+16I"),
-    "c3": dict(op="vposv", n=256, nmin=8, batch=20000, nrhs=4, gen="shift",
+    "c3": dict(op="vposv", n=256, nmin=8, batch=20000, nrhs=4, gen="shift", seed=3,
                workload="dposv_vbatch n~U{8..256} x 20000, 4 RHS, pointer arrays, one persistent launch "
-                        "over a device size-sorted queue, each step a captured HIP graph"),
-    "c5": dict(op="vsposv", n=512, nmin=128, batch=2000, nrhs=64, gen="cond", lkmin=2.0, lkmax=4.0,
-               workload="dsposv_vbatch n~U{128..512} x 2000, 64 RHS, cond in [1e2,1e4], non-uniform mixed POSV"),
-    "c4": dict(op="sposv", n=64, batch=50000, nrhs=4, gen="cond",
+                        "over a device size-sorted queue"),
+    "c4": dict(op="sposv", n=64, batch=50000, nrhs=4, gen="cond", lkmin=2.0, lkmax=6.0, seed=4,
                workload="dsposv_batch_strided n=64 x 50000, 4 RHS, cond in [1e2,1e6], mixed FP32+IR vs FP64"),
-    # inverse family (SPD inverse), sized like c1; not a headline configuration
+    "c5": dict(op="vsposv", n=512, nmin=128, batch=2000, nrhs=64, gen="cond", lkmin=2.0, lkmax=4.0, seed=5,
+               workload="dsposv_vbatch n~U{128..512} x 2000, 64 RHS, cond in [1e2,1e4], non-uniform mixed POSV"),
+    # inverse family (SPD inverse), sized like c1; not part of the default run
     "poti": dict(op="poti", n=32, batch=10000, nrhs=0, gen="shift", seed=1240,
                  workload="dpoti_batch_strided n=32 x 10000 (A := A^-1, POTRF then POTRI), A=B*B^T+32I"),
 }
 
 
-def gen_inputs(torch, cfg, seed):
+def log(msg):
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------------------
+# input generation (torch on the GPU: plumbing only)
+# ---------------------------------------------------------------------------
+def gen_shift(torch, g, bc, n):
+    Bm = torch.rand((bc, n, n), generator=g, device=DEV, dtype=torch.float64) * 2 - 1
+    A = torch.bmm(Bm, Bm.transpose(1, 2).contiguous())
+    A = 0.5 * (A + A.transpose(1, 2))
+    A += n * torch.eye(n, device=DEV, dtype=torch.float64)
+    return A.contiguous()
+
+
+def gen_cond(torch, g, ns, nmax, lkmin, lkmax):
+    """A_b = Q_b diag(s_b) Q_b^T of order ns[b] in the leading block of an
+    nmax x nmax slot: Q_b = H_0 ... H_{n-2}, H_p the Householder reflector of a
+    Gaussian vector supported on p..n_b-1 (Stewart's construction of a Haar
+    orthogonal matrix, the distribution of the Q factor of a Gaussian matrix),
+    s_i = kappa^(-t_i) with t_0 = 0, t_{n-1} = 1 and the rest U(0,1), so
+    cond(A_b) = kappa_b, log10(kappa_b) ~ U[lkmin, lkmax].  Slots must be
+    sorted by order (ascending) so the matrices still being built at step p
+    are a suffix of the batch."""
+    bc = ns.shape[0]
+    dev = DEV
+    lk = lkmin + (lkmax - lkmin) * torch.rand((bc, 1), generator=g, device=dev, dtype=torch.float64)
+    idx = torch.arange(nmax, device=dev)
+    t = torch.rand((bc, nmax), generator=g, device=dev, dtype=torch.float64)
+    t[:, 0] = 0.0
+    t = torch.where(idx[None, :] == (ns[:, None] - 1), torch.ones_like(t), t)
+    s = torch.pow(10.0, -lk * t)
+    s = torch.where(idx[None, :] < ns[:, None], s, torch.zeros_like(s))
+    A = torch.diag_embed(s)
+    ns_h = ns.cpu().numpy()
+    for p in range(nmax - 2, -1, -1):
+        first = int(np.searchsorted(ns_h, p + 2))      # slots with n_b > p + 1
+        if first >= bc:
+            continue
+        As = A[first:]
+        nb = ns[first:]
+        v = torch.randn((bc - first, nmax, 1), generator=g, device=dev, dtype=torch.float64)
+        keep = (idx[None, :, None] >= p) & (idx[None, :, None] < nb[:, None, None])
+        v = torch.where(keep, v, torch.zeros_like(v))
+        vv = (v * v).sum(dim=1, keepdim=True)
+        tau = torch.where(vv > 0, 2.0 / torch.clamp(vv, min=1e-300), torch.zeros_like(vv))
+        w = tau * torch.bmm(As, v)
+        alpha = 0.5 * tau * (v * w).sum(dim=1, keepdim=True)
+        w = w - alpha * v
+        left = torch.cat((v, w), dim=2).contiguous()                  # (k, nmax, 2)
+        right = torch.cat((w, v), dim=2).transpose(1, 2).contiguous()  # (k, 2, nmax)
+        As.baddbmm_(left, right, alpha=-1.0)
+    A = 0.5 * (A + A.transpose(1, 2))
+    return A.contiguous(), torch.pow(10.0, lk[:, 0])
+
+
+def gen_uniform(torch, cfg):
+    g = torch.Generator(device=DEV)
+    g.manual_seed(1234 + cfg["seed"])
     n, bc, nrhs = cfg["n"], cfg["batch"], cfg["nrhs"]
-    g = torch.Generator(device="cuda")
-    g.manual_seed(seed)
-    dev = "cuda"
+    kappa = None
     if cfg["gen"] == "shift":
-        Bm = torch.rand((bc, n, n), generator=g, device=dev, dtype=torch.float64) * 2 - 1
-        A = torch.bmm(Bm, Bm.transpose(1, 2).contiguous())
-        A = 0.5 * (A + A.transpose(1, 2))
-        A += n * torch.eye(n, device=dev, dtype=torch.float64)
-        kappa = None
+        A = gen_shift(torch, g, bc, n)
     else:
-        # A = Q diag(s) Q^T, Q = H_0 ... H_{n-2} with H_p a Householder reflector
-        # from a Gaussian vector supported on entries p..n-1 (Stewart's
-        # construction of a Haar orthogonal matrix), applied two-sidedly to
-        # diag(s) in a few hundred batched kernels.
-        lk = 2.0 + 4.0 * torch.rand((bc, 1), generator=g, device=dev, dtype=torch.float64)
-        t = torch.rand((bc, n), generator=g, device=dev, dtype=torch.float64)
-        t[:, 0] = 0.0
-        t[:, -1] = 1.0
-        s = torch.pow(10.0, -lk * t)
-        A = torch.diag_embed(s)
-        for p in range(n - 2, -1, -1):
-            v = torch.randn((bc, n, 1), generator=g, device=dev, dtype=torch.float64)
-            v[:, :p, :] = 0.0
-            tau = 2.0 / (v * v).sum(dim=1, keepdim=True)          # (bc,1,1)
-            w = tau * torch.bmm(A, v)                              # (bc,n,1)
-            alpha = 0.5 * tau * (v * w).sum(dim=1, keepdim=True)
-            w = w - alpha * v
-            A = A - torch.bmm(v, w.transpose(1, 2).contiguous()) - torch.bmm(w, v.transpose(1, 2).contiguous())
-        A = 0.5 * (A + A.transpose(1, 2))
-        kappa = torch.pow(10.0, lk[:, 0])
-    A = A.contiguous()                       # symmetric: row- and column-major agree
+        ns = torch.full((bc,), n, dtype=torch.int64, device=DEV)
+        A, kappa = gen_cond(torch, g, ns, n, cfg["lkmin"], cfg["lkmax"])
     B = None
     if nrhs:
-        # Right side (reference layout): B is nrhs x n, ldb = nrhs, column-major
-        B = torch.randn((bc, n, nrhs), generator=g, device=dev, dtype=torch.float64).contiguous()
+        # right side (reference layout): B is nrhs x n, ldb = nrhs, column-major
+        B = torch.randn((bc, n, nrhs), generator=g, device=DEV, dtype=torch.float64).contiguous()
     return A, B, kappa
 
 
-def gen_vbatch(torch, cfg, seed):
-    """Non-uniform batch: sizes n_i ~ U{nmin..n}, A_i = B_i B_i^T + n_i I packed one
-    after another (column-major, lda = n_i), B_i (right side: nrhs x n_i, ldb = nrhs)."""
-    g = torch.Generator(device="cuda")
-    g.manual_seed(seed)
+def gen_vbatch(torch, cfg):
+    """Non-uniform batch: n_b ~ U{nmin..n}; A_b packed one after another
+    (column-major, lda = n_b), B_b (right side: nrhs x n_b, ldb = nrhs)."""
+    g = torch.Generator(device=DEV)
+    g.manual_seed(1234 + cfg["seed"])
     bc, nmax, nmin, nrhs = cfg["batch"], cfg["n"], cfg["nmin"], cfg["nrhs"]
-    ns = torch.randint(nmin, nmax + 1, (bc,), generator=g, device="cuda", dtype=torch.int32)
+    ns = torch.randint(nmin, nmax + 1, (bc,), generator=g, device=DEV, dtype=torch.int32)
     ns_h = ns.cpu().numpy()
     offA = np.zeros(bc + 1, dtype=np.int64)
     offA[1:] = np.cumsum(ns_h.astype(np.int64) ** 2)
     offB = np.zeros(bc + 1, dtype=np.int64)
     offB[1:] = np.cumsum(ns_h.astype(np.int64) * nrhs)
-    Abuf = torch.empty(int(offA[-1]), device="cuda", dtype=torch.float64)
-    Bbuf = torch.randn(int(offB[-1]), generator=g, device="cuda", dtype=torch.float64)
-    for n in np.unique(ns_h):
-        idx = np.nonzero(ns_h == n)[0]
-        k = len(idx)
-        if cfg["gen"] == "cond":
-            # A = Q diag(s) Q^T, Q from the QR factorization of a Gaussian matrix,
-            # s log-spaced with endpoints 1 and 1/kappa, log10(kappa) ~ U[lkmin, lkmax]
-            G = torch.randn((k, n, n), generator=g, device="cuda", dtype=torch.float64)
-            Q, _ = torch.linalg.qr(G)
-            lk = cfg["lkmin"] + (cfg["lkmax"] - cfg["lkmin"]) * torch.rand((k, 1), generator=g, device="cuda",
-                                                                           dtype=torch.float64)
-            t = torch.linspace(0.0, 1.0, int(n), device="cuda", dtype=torch.float64)[None, :]
-            sv = torch.pow(10.0, -lk * t)
-            A = torch.bmm(Q * sv[:, None, :], Q.transpose(1, 2).contiguous())
-            A = 0.5 * (A + A.transpose(1, 2))
-        else:
-            Bm = torch.rand((k, n, n), generator=g, device="cuda", dtype=torch.float64) * 2 - 1
-            A = torch.bmm(Bm, Bm.transpose(1, 2).contiguous())
-            A = 0.5 * (A + A.transpose(1, 2)) + n * torch.eye(int(n), device="cuda", dtype=torch.float64)
-        pos = torch.from_numpy((offA[idx][:, None] + np.arange(n * n)[None, :]).reshape(-1)).cuda()
-        Abuf.index_copy_(0, pos, A.reshape(-1))
-    return ns, ns_h, offA, offB, Abuf, Bbuf
+    Abuf = torch.empty(int(offA[-1]), device=DEV, dtype=torch.float64)
+    Bbuf = torch.randn(int(offB[-1]), generator=g, device=DEV, dtype=torch.float64)
+    kappa = None
+    if cfg["gen"] == "cond":
+        order = np.argsort(ns_h, kind="stable")
+        ns_sorted = torch.from_numpy(ns_h[order].astype(np.int64)).to(DEV)
+        Aslots, kap_sorted = gen_cond(torch, g, ns_sorted, nmax, cfg["lkmin"], cfg["lkmax"])
+        kappa = np.empty(bc)
+        kappa[order] = kap_sorted.cpu().numpy()
+        for n in np.unique(ns_h):
+            sel = np.nonzero(ns_h[order] == n)[0]          # slots of this order
+            dst = order[sel]                                # their batch positions
+            blk = Aslots[torch.from_numpy(sel).to(DEV), :n, :n]
+            # symmetric: the slot's row-major n x n block is also its column-major one
+            pos = torch.from_numpy((offA[dst][:, None] + np.arange(n * n)[None, :]).reshape(-1)).to(DEV)
+            Abuf.index_copy_(0, pos, blk.reshape(-1))
+        del Aslots
+    else:
+        for n in np.unique(ns_h):
+            idx = np.nonzero(ns_h == n)[0]
+            A = gen_shift(torch, g, len(idx), int(n))
+            pos = torch.from_numpy((offA[idx][:, None] + np.arange(n * n)[None, :]).reshape(-1)).to(DEV)
+            Abuf.index_copy_(0, pos, A.reshape(-1))
+    return dict(ns=ns, ns_h=ns_h, offA=offA, offB=offB, A=Abuf, B=Bbuf, kappa=kappa)
 
 
 def measured_traffic(config):
@@ -134,102 +187,495 @@ def measured_traffic(config):
     size).  None when no summary exists for this workload."""
     path = os.path.join(ROOT, "profiles", "traffic_%s.md" % config)
     if not os.path.exists(path):
-        return None
+        return None, None
     for ln in open(path):
         if ln.startswith("traffic_bytes_per_launch:"):
-            return {"bytes_per_launch": float(ln.split(":", 1)[1].split()[0]),
-                    "source": "profiles/traffic_%s.md" % config}
-    return None
+            return float(ln.split(":", 1)[1].split()[0]), "profiles/traffic_%s.md" % config
+    return None, None
 
 
-def time_steps(torch, call, K, W):
-    for i in range(W):
-        call(i)
-    torch.cuda.synchronize()
-    s = torch.cuda.current_stream()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    e0.record(s)
-    for i in range(K):
-        call(W + i)
-    e1.record(s)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    return e0.elapsed_time(e1) / 1000.0, wall
+# ---------------------------------------------------------------------------
+# timing: one HIP graph per step, HIP events around every replay
+# ---------------------------------------------------------------------------
+class Stepper:
+    """Captures issue(slot) once per input slot as a HIP graph on a side stream
+    (the library handle follows the capturing stream) and replays it."""
+
+    def __init__(self, torch, h, issue, slots):
+        self.torch = torch
+        self.graphs = []
+        self.stream = torch.cuda.Stream()
+        torch.cuda.synchronize()
+        for s in range(slots):
+            gr = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(gr, stream=self.stream):
+                h.set_stream(torch.cuda.current_stream().cuda_stream)
+                issue(s)
+            self.graphs.append(gr)
+        h.set_stream(torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+
+    def run(self, K, W, slot_of, restore=None):
+        torch = self.torch
+        st = self.stream
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(K)]
+        e_all0 = torch.cuda.Event(enable_timing=True)
+        e_all1 = torch.cuda.Event(enable_timing=True)
+        with torch.cuda.stream(st):
+            for i in range(W):
+                if restore:
+                    restore(i)
+                self.graphs[slot_of(i)].replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        with torch.cuda.stream(st):
+            e_all0.record(st)
+            for i in range(K):
+                if restore:
+                    restore(W + i)
+                ev[i][0].record(st)
+                self.graphs[slot_of(W + i)].replay()
+                ev[i][1].record(st)
+            e_all1.record(st)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        per = np.array([a.elapsed_time(b) for a, b in ev]) / 1000.0
+        return dict(total_s=float(per.sum()), span_s=e_all0.elapsed_time(e_all1) / 1000.0, wall_s=wall,
+                    median_ms=float(np.median(per) * 1e3), min_ms=float(per.min() * 1e3),
+                    mean_ms=float(per.mean() * 1e3))
 
 
-def cpu_baseline(cfg, A_s, B_s, X_gpu, cores):
-    """Oracle (FP64 POSV restatement, OpenMP over the sample) on host cores."""
+# ---------------------------------------------------------------------------
+# CPU baseline (the reference test's comparator) and the oracle port
+# ---------------------------------------------------------------------------
+def host_info():
+    model = "unknown"
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model
+
+
+def host_cores():
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count() or 1
+    return env if env > 0 else aff
+
+
+def median_rate(fn, units, reps=5):
+    fn()                                     # warm-up (MKL initialisation, page faults)
+    ts = []
+    for _ in range(reps):
+        ts.append(fn())
+    return units / float(np.median(ts)), [units / t for t in ts]
+
+
+def cpu_uniform(cfg, A_s, B_s, cores):
+    """Looped LAPACK over the sample, all cores and one thread; the oracle port beside it."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
     L = po.lib()
-    L.orc_set_threads(cores)
     n, nrhs = cfg["n"], cfg["nrhs"]
     bs = A_s.shape[0]
-    res = {}
-    # accuracy of the GPU results on the sample, against the FP64 oracle metrics
-    if X_gpu is not None:
-        be = np.array([po.solve_berr("R", A_s[i].T, X_gpu[i].T, B_s[i].T) for i in range(bs)])
-        res["berr_max"] = float(be.max())
-        res["berr_median"] = float(np.median(be))
     A = np.ascontiguousarray(A_s)
-    reps, t_best = 0, None
-    t_end = time.perf_counter() + 12.0
-    while reps < 5 and (reps == 0 or time.perf_counter() < t_end):
-        Aw = A.copy()
-        if cfg["op"] == "potrf":
+    B = np.ascontiguousarray(B_s) if B_s is not None else None
+
+    def lap(th, count):
+        def f():
+            L.orc_set_threads(th)
+            Aw = A[:count].copy()
+            Bw = B[:count].copy() if B is not None else None
             t0 = time.perf_counter()
-            L.orc_potrf_batch(n, po.ptr(Aw), n, n * n, bs, None)
-        else:
-            Bw = np.ascontiguousarray(B_s).copy()
+            if cfg["op"] == "potrf":
+                L.orc_lapack_potrf_batch(n, po.ptr(Aw), n, n * n, count, None)
+            else:
+                L.orc_lapack_posv_batch(b"R", nrhs, n, po.ptr(Aw), n, n * n, po.ptr(Bw), nrhs, nrhs * n, count,
+                                        None)
+            return time.perf_counter() - t0
+        return f
+
+    def port(th, count):
+        def f():
+            L.orc_set_threads(th)
+            Aw = A[:count].copy()
+            Bw = B[:count].copy() if B is not None else None
             t0 = time.perf_counter()
-            L.orc_posv_batch(b"R", nrhs, n, po.ptr(Aw), n, n * n, po.ptr(Bw), nrhs, nrhs * n, bs, None)
-        dt = time.perf_counter() - t0
-        t_best = dt if t_best is None else min(t_best, dt)
-        reps += 1
-    res["value"] = bs / t_best
-    lap = None
+            if cfg["op"] == "potrf":
+                L.orc_potrf_batch(n, po.ptr(Aw), n, n * n, count, None)
+            else:
+                L.orc_posv_batch(b"R", nrhs, n, po.ptr(Aw), n, n * n, po.ptr(Bw), nrhs, nrhs * n, count, None)
+            return time.perf_counter() - t0
+        return f
+    res = {}
     if L.orc_lapack_available():
-        Aw = A.copy()
-        t0 = time.perf_counter()
-        if cfg["op"] == "potrf":
-            L.orc_lapack_potrf_batch(n, po.ptr(Aw), n, n * n, bs, None)
-        else:
-            Bw = np.ascontiguousarray(B_s).copy()
-            L.orc_lapack_posv_batch(b"R", nrhs, n, po.ptr(Aw), n, n * n, po.ptr(Bw), nrhs, nrhs * n, bs, None)
-        lap = {"value": bs / (time.perf_counter() - t0), "library": L.orc_lapack_name().decode()}
-    res["lapack"] = lap
+        v, runs = median_rate(lap(cores, bs), bs)
+        one = max(1, bs // max(cores, 1))
+        v1, _ = median_rate(lap(1, one), one)
+        res = {"value": v, "cores": cores, "kind": "port",
+               "what": "looped LAPACK dpotrf%s (%s, sequential per thread), OpenMP over the batch" %
+                       ("" if cfg["op"] == "potrf" else " + dtrsm(R,L,T) + dtrsm(R,L,N)",
+                        L.orc_lapack_name().decode()),
+               "runs": runs, "single_thread": v1, "single_thread_sample": one}
+    pv, _ = median_rate(port(cores, min(bs, 4000)), min(bs, 4000), reps=3)
+    res["oracle_port"] = {"value": pv, "cores": cores, "sample": min(bs, 4000)}
+    if "value" not in res:   # no LAPACK on this host: the port is the baseline
+        res.update({"value": pv, "cores": cores, "kind": "port", "what": "oracle C restatement (no LAPACK found)"})
+    res["sample"] = "%d of the %d matrices (the first ones of the same seeded batch)" % (bs, cfg["batch"])
+    res["cpu_model"] = host_info()
     return res
 
 
-def cpu_baseline_vbatch(cfg, vb, K, unit):
-    """Oracle FP64 POSV (OpenMP over matrices) on the first matrices of the same batch."""
+def cpu_vbatch(cfg, vb, cores, count):
+    """Looped LAPACK over the first `count` matrices of a non-uniform batch."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
-    import ctypes
     L = po.lib()
-    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
-    L.orc_set_threads(cores)
-    ns, ns_h, offA, offB, Abuf, Bbuf = vb
     nrhs = cfg["nrhs"]
-    bs = min(2000 if cfg["op"] == "vposv" else 40, cfg["batch"])
-    A = Abuf[: int(offA[bs])].cpu().numpy()
-    B = Bbuf[: int(offB[bs])].cpu().numpy()
-    best = None
-    for rep in range(3):
-        Aw, Bw = A.copy(), B.copy()
+    ns = np.ascontiguousarray(vb["ns_h"][:count].astype(np.int32))
+    offA = np.ascontiguousarray(vb["offA"][:count].astype(np.int64))
+    offB = np.ascontiguousarray(vb["offB"][:count].astype(np.int64))
+    A = vb["A"][: int(vb["offA"][count])].cpu().numpy()
+    B = vb["B"][: int(vb["offB"][count])].cpu().numpy()
+    # largest first, so the dynamic schedule balances
+    order = np.argsort(-ns, kind="stable")
+    ns_o, offA_o, offB_o = ns[order].copy(), offA[order].copy(), offB[order].copy()
+
+    def mk(fn, th):
+        def f():
+            L.orc_set_threads(th)
+            Aw, Bw = A.copy(), B.copy()
+            t0 = time.perf_counter()
+            fn(nrhs, po.ptr(ns_o), po.ptr(Aw), po.ptr(offA_o), po.ptr(Bw), po.ptr(offB_o), count, None)
+            return time.perf_counter() - t0
+        return f
+    res = {}
+    if L.orc_lapack_available():
+        v, runs = median_rate(mk(L.orc_lapack_posv_vbatch, cores), count, reps=5)
+        v1, _ = median_rate(mk(L.orc_lapack_posv_vbatch, 1), count, reps=3)
+        res = {"value": v, "cores": cores, "kind": "port",
+               "what": "looped LAPACK dpotrf + dtrsm(R,L,T) + dtrsm(R,L,N) (%s, sequential per thread), "
+                       "OpenMP dynamic over the matrices" % L.orc_lapack_name().decode(),
+               "runs": runs, "single_thread": v1}
+    pv, _ = median_rate(mk(L.orc_posv_vbatch, cores), count, reps=3)
+    res["oracle_port"] = {"value": pv, "cores": cores}
+    if "value" not in res:
+        res.update({"value": pv, "cores": cores, "kind": "port", "what": "oracle C restatement (no LAPACK found)"})
+    res["sample"] = "first %d of the %d matrices of the same seeded non-uniform batch" % (count, cfg["batch"])
+    res["cpu_model"] = host_info()
+    return res
+
+
+def berr_stats(side, A_s, X_s, B_s):
+    """normwise backward errors ||b - A x|| / (||A|| ||x|| + ||b||) of a sample (oracle metric)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    be = np.array([po.solve_berr(side, A_s[i].T, X_s[i].T, B_s[i].T) for i in range(A_s.shape[0])])
+    return {"max": float(be.max()), "median": float(np.median(be)), "sample": int(A_s.shape[0])}
+
+
+# ---------------------------------------------------------------------------
+# workloads
+# ---------------------------------------------------------------------------
+def hist_of(a):
+    h = {}
+    for v in a.tolist():
+        h[str(v)] = h.get(str(v), 0) + 1
+    return dict(sorted(h.items(), key=lambda kv: int(kv[0])))
+
+
+def run_uniform(torch, pykblas, h, name, cfg, K, W, with_cpu, cpu_sample):
+    n, bc, nrhs = cfg["n"], cfg["batch"], cfg["nrhs"]
+    NA = n * n
+    lib = h.lib
+    log("%s: %s, generating inputs" % (name, cfg["workload"]))
+    A0, B0, kappa = gen_uniform(torch, cfg)
+    info = torch.zeros(bc, dtype=torch.int32, device=DEV)
+    iters = torch.zeros(bc, dtype=torch.int32, device=DEV)
+    S = K + W
+    out = {"workload": cfg["workload"], "batch": bc, "n": n, "nrhs": nrhs}
+    if cfg["op"] == "potrf":
+        As = [A0.clone() for _ in range(S)]
+
+        def issue(s):
+            assert lib.kblasDpotrf_batch_strided(h.h, b"L", n, As[s].data_ptr(), n, NA, bc,
+                                                 info.data_ptr()) == pykblas.KBLAS_Success
+        r = Stepper(torch, h, issue, S).run(K, W, lambda i: i)
+        assert int((info != 0).sum().item()) == 0
+        bytes_alg, bytes_full = 8 * n * (n + 1), 16 * n * n
+        unit = "factorizations/s"
+        del As
+    elif cfg["op"] == "posv":
+        As = [A0.clone() for _ in range(S)]
+        Bs = [B0.clone() for _ in range(S)]
+
+        def issue(s):
+            assert lib.kblasDposv_batch_strided(h.h, b"R", b"L", nrhs, n, As[s].data_ptr(), n, NA,
+                                                Bs[s].data_ptr(), nrhs, nrhs * n, bc,
+                                                info.data_ptr()) == pykblas.KBLAS_Success
+        r = Stepper(torch, h, issue, S).run(K, W, lambda i: i)
+        assert int((info != 0).sum().item()) == 0
+        bytes_alg, bytes_full = 8 * n * (n + 1) + 16 * n * nrhs, 16 * n * n + 16 * n * nrhs
+        unit = "solves/s"
+        X_sample = Bs[0]
+        del As
+    else:
+        # mixed precision: A and B read-only, X written (A only on fallback)
+        As = [A0.clone() for _ in range(S)]
+        Xs = torch.empty_like(B0)
+
+        def issue(s):
+            assert lib.kblasDsposv_batch_strided(h.h, b"R", b"L", nrhs, n, As[s].data_ptr(), n, NA,
+                                                 B0.data_ptr(), nrhs, nrhs * n, Xs.data_ptr(), nrhs, nrhs * n,
+                                                 bc, info.data_ptr(), iters.data_ptr()) == pykblas.KBLAS_Success
+        log("%s: timing mixed FP32+IR POSV" % name)
+        r = Stepper(torch, h, issue, S).run(K, W, lambda i: i)
+        it_h = iters.cpu().numpy()
+        X_mixed = Xs
+        del As
+        # the plain FP64 POSV on the same inputs
+        As = [A0.clone() for _ in range(S)]
+        Bs = [B0.clone() for _ in range(S)]
+
+        def issue64(s):
+            assert lib.kblasDposv_batch_strided(h.h, b"R", b"L", nrhs, n, As[s].data_ptr(), n, NA,
+                                                Bs[s].data_ptr(), nrhs, nrhs * n, bc,
+                                                info.data_ptr()) == pykblas.KBLAS_Success
+        log("%s: timing FP64 POSV" % name)
+        r64 = Stepper(torch, h, issue64, S).run(K, W, lambda i: i)
+        X64 = Bs[0]
+        del As, Bs
+        bytes_alg = 4 * n * (n + 1) + 16 * n * nrhs       # A lower read once (FP64), B in, X out
+        bytes64 = 8 * n * (n + 1) + 16 * n * nrhs
+        unit = "solves/s"
+        out["fp64_posv"] = {"value": bc * K / r64["total_s"], "ms_per_step": 1e3 * r64["total_s"] / K,
+                            "median_ms": r64["median_ms"], "min_ms": r64["min_ms"],
+                            "achieved_GBs": bytes64 * bc * K / r64["total_s"] / 1e9,
+                            "bytes_per_unit": bytes64}
+        out["mixed_over_fp64"] = r64["total_s"] / r["total_s"]
+        out["iter_hist"] = hist_of(it_h)
+        out["fallbacks"] = int((it_h < 0).sum())
+        bytes_full = None
+    t = r["total_s"]
+    out.update({"value": bc * K / t, "unit": unit, "ms_per_step": 1e3 * t / K, "median_ms": r["median_ms"],
+                "min_ms": r["min_ms"], "graph_replay": True})
+    ach = bytes_alg * bc * K / t / 1e9
+    roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+            "traffic": None, "bytes_per_unit": bytes_alg, "units_per_launch": bc}
+    if bytes_full:
+        roof["achieved_full_tile"] = bytes_full * bc * K / t / 1e9
+        roof["frac_full_tile"] = roof["achieved_full_tile"] / HBM_PEAK_GBS
+        roof["bytes_per_unit_full_tile"] = bytes_full
+    tr, src = measured_traffic(name)
+    if tr is not None:
+        roof["traffic"], roof["traffic_source"] = tr, src
+    out["roofline"] = roof
+    if cfg["op"] == "sposv":
+        # accuracy of both GPU paths on a sample, oracle metrics (FP64)
+        ss = min(2000, bc)
+        A_s, B_s = A0[:ss].cpu().numpy(), B0[:ss].cpu().numpy()
+        out["berr_mixed"] = berr_stats("R", A_s, X_mixed[:ss].cpu().numpy(), B_s)
+        out["berr_fp64"] = berr_stats("R", A_s, X64[:ss].cpu().numpy(), B_s)
+        out["kappa_range"] = [float(kappa.min()), float(kappa.max())]
+    elif cfg["op"] == "posv":
+        ss = min(2000, bc)
+        out["berr_fp64"] = berr_stats("R", A0[:ss].cpu().numpy(), X_sample[:ss].cpu().numpy(),
+                                      B0[:ss].cpu().numpy())
+    if with_cpu:
+        cores = host_cores()
+        bs = min(cpu_sample or {"potrf": 20000, "posv": 50000, "sposv": 10000}[cfg["op"]], bc)
+        log("%s: CPU baseline on %d matrices, %d threads" % (name, bs, cores))
+        A_s = A0[:bs].cpu().numpy()
+        B_s = B0[:bs].cpu().numpy() if B0 is not None else None
+        cb = cpu_uniform(cfg, A_s, B_s, cores)
+        cb["unit"] = unit
+        out["cpu_baseline"] = cb
+    return out
+
+
+def run_vbatch(torch, pykblas, h, name, cfg, K, W, with_cpu):
+    bc, nmax, nrhs = cfg["batch"], cfg["n"], cfg["nrhs"]
+    lib = h.lib
+    log("%s: %s, generating inputs" % (name, cfg["workload"]))
+    vb = gen_vbatch(torch, cfg)
+    ns, ns_h, offA, offB = vb["ns"], vb["ns_h"], vb["offA"], vb["offB"]
+    A0, B0 = vb["A"], vb["B"]
+    torch.cuda.synchronize()
+    info = torch.zeros(bc, dtype=torch.int32, device=DEV)
+    iters = torch.zeros(bc, dtype=torch.int32, device=DEV)
+    nrhs_arr = torch.full((bc,), nrhs, dtype=torch.int32, device=DEV)
+    lib.kblas_posv_vbatch_wsquery(h.h, nmax, bc)
+    assert lib.kblasAllocateWorkspace(h.h) == pykblas.KBLAS_Success
+    offA_d = torch.from_numpy(offA[:-1] * 8).to(DEV)
+    offB_d = torch.from_numpy(offB[:-1] * 8).to(DEV)
+    Aw = A0.clone()
+    Bw = B0.clone()
+    pA = offA_d + Aw.data_ptr()
+    pB = offB_d + Bw.data_ptr()
+    mixed = cfg["op"] == "vsposv"
+    if mixed:
+        Xw = torch.empty_like(B0)
+        pX = offB_d + Xw.data_ptr()
+
+        def issue(s):
+            assert lib.kblasDsposv_vbatch(h.h, b"R", b"L", nrhs_arr.data_ptr(), ns.data_ptr(), nrhs, nmax,
+                                          pA.data_ptr(), ns.data_ptr(), pB.data_ptr(), nrhs_arr.data_ptr(),
+                                          pX.data_ptr(), nrhs_arr.data_ptr(), bc, info.data_ptr(),
+                                          iters.data_ptr()) == pykblas.KBLAS_Success
+
+        def restore(i):
+            Aw.copy_(A0)          # a fallback overwrites A with its FP64 factor
+    else:
+        def issue(s):
+            assert lib.kblasDposv_vbatch(h.h, b"R", b"L", nrhs_arr.data_ptr(), ns.data_ptr(), nrhs, nmax,
+                                         pA.data_ptr(), ns.data_ptr(), pB.data_ptr(), nrhs_arr.data_ptr(),
+                                         bc, info.data_ptr()) == pykblas.KBLAS_Success
+
+        def restore(i):
+            Aw.copy_(A0)
+            Bw.copy_(B0)
+    log("%s: capturing and timing" % name)
+    r = Stepper(torch, h, issue, 1).run(K, W, lambda i: 0, restore=restore)
+    t = r["total_s"]
+    nn = ns_h.astype(np.float64)
+    flops = float(np.sum(nn ** 3 / 3 + nn ** 2 / 2 + nn / 6 + 2 * nrhs * nn ** 2))
+    if mixed:
+        bytes_alg = float(np.sum(4 * nn * (nn + 1) + 16 * nn * nrhs))
+    else:
+        bytes_alg = float(np.sum(8 * nn * (nn + 1) + 16 * nn * nrhs))
+    out = {"workload": cfg["workload"], "batch": bc, "n_range": [int(ns_h.min()), int(ns_h.max())], "nrhs": nrhs,
+           "value": bc * K / t, "unit": "solves/s", "ms_per_step": 1e3 * t / K, "median_ms": r["median_ms"],
+           "min_ms": r["min_ms"], "graph_replay": True,
+           "gflops": flops * K / t / 1e9}
+    out["fp64_peak_frac"] = out["gflops"] / FP64_PEAK_GFS
+    ach = bytes_alg * K / t / 1e9
+    roof = {"bound": "fp64" if not mixed else "mixed", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": ach / HBM_PEAK_GBS, "traffic": None, "bytes_per_step": bytes_alg,
+            "gflops": out["gflops"], "fp64_frac": out["fp64_peak_frac"]}
+    tr, src = measured_traffic(name)
+    if tr is not None:
+        roof["traffic"], roof["traffic_source"] = tr, src
+    out["roofline"] = roof
+    out["info_nonzero"] = int((info != 0).sum().item())
+    if mixed:
+        it_h = iters.cpu().numpy()
+        out["iter_hist"] = hist_of(it_h)
+        out["fallbacks"] = int((it_h < 0).sum())
+        # accuracy per matrix on a sample against the FP64 oracle solution
+        X = Xw
+        ss = min(cfg.get("acc_sample", 64), bc)
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as po
+        Ah = A0[: int(offA[ss])].cpu().numpy()
+        Bh = B0[: int(offB[ss])].cpu().numpy()
+        Xh = X[: int(offB[ss])].cpu().numpy()
+        be, fe = [], []
+        for i in range(ss):
+            n = int(ns_h[i])
+            Ai = Ah[offA[i]:offA[i + 1]].reshape(n, n).T
+            Bi = Bh[offB[i]:offB[i + 1]].reshape(n, nrhs).T
+            Xi = Xh[offB[i]:offB[i + 1]].reshape(n, nrhs).T
+            be.append(po.solve_berr("R", Ai, Xi, Bi))
+            _, Xo, _ = po.posv("R", Ai, Bi)
+            fe.append(float(np.max(np.abs(Xi - Xo)) / np.max(np.abs(Xo))))
+        out["berr_mixed"] = {"max": float(max(be)), "median": float(np.median(be)), "sample": ss}
+        out["fwd_err_vs_fp64_oracle"] = {"max": float(max(fe)), "median": float(np.median(fe)), "sample": ss}
+    if with_cpu:
+        cores = host_cores()
+        count = min(cfg.get("cpu_count", 2000 if not mixed else 500), bc)
+        log("%s: CPU baseline on %d matrices, %d threads" % (name, count, cores))
+        cb = cpu_vbatch(cfg, vb, cores, count)
+        cb["unit"] = "solves/s"
+        out["cpu_baseline"] = cb
+    del Aw, Bw, vb
+    return out
+
+
+def run_poti(torch, pykblas, h, cfg, K, W, with_cpu):
+    """inverse family, sized like c1 (not part of the default run)"""
+    n, bc = cfg["n"], cfg["batch"]
+    NA = n * n
+    lib = h.lib
+    A0, _, _ = gen_uniform(torch, cfg)
+    info = torch.zeros(bc, dtype=torch.int32, device=DEV)
+    S = K + W
+    As = [A0.clone() for _ in range(S)]
+
+    def issue(s):
+        assert lib.kblasDpoti_batch_strided(h.h, b"L", n, As[s].data_ptr(), n, NA, bc,
+                                            info.data_ptr()) == pykblas.KBLAS_Success
+    r = Stepper(torch, h, issue, S).run(K, W, lambda i: i)
+    assert int((info != 0).sum().item()) == 0
+    X_sample = As[0]
+    Ls = [A0.clone() for _ in range(S)]
+    for i in range(S):
+        assert lib.kblasDpotrf_batch_strided(h.h, b"L", n, Ls[i].data_ptr(), n, NA, bc,
+                                             info.data_ptr()) == pykblas.KBLAS_Success
+
+    def issue_potri(s):
+        assert lib.kblasDpotri_batch_strided(h.h, b"L", n, Ls[s].data_ptr(), n, NA, bc,
+                                             info.data_ptr()) == pykblas.KBLAS_Success
+    rp = Stepper(torch, h, issue_potri, S).run(K, W, lambda i: i)
+    Bt = [torch.randn((bc, n, n), device=DEV, dtype=torch.float64) for _ in range(S)]
+
+    def issue_trmm(s):
+        assert lib.kblasDtrmm_batch_strided(h.h, b"L", b"L", b"T", b"N", n, n, 1.0, Ls[s].data_ptr(), n, NA,
+                                            Bt[s].data_ptr(), n, NA, bc) == pykblas.KBLAS_Success
+    rt = Stepper(torch, h, issue_trmm, S).run(K, W, lambda i: i)
+    del Bt
+    Ps = [A0.clone() for _ in range(S)]
+    pv = torch.zeros((bc, n), dtype=torch.int32, device=DEV)
+    rk = torch.zeros(bc, dtype=torch.int32, device=DEV)
+
+    def issue_pstrf(s):
+        assert lib.kblasDpstrf_batch_strided(h.h, n, Ps[s].data_ptr(), n, NA, pv.data_ptr(), n, rk.data_ptr(),
+                                             bc) == pykblas.KBLAS_Success
+    rq = Stepper(torch, h, issue_pstrf, S).run(K, W, lambda i: i)
+    assert int((rk != n).sum().item()) == 0
+    del Ps, Ls
+    t = r["total_s"]
+    bytes_alg = 8 * n * (n + 1)
+    ach = bytes_alg * bc * K / t / 1e9
+    out = {"workload": cfg["workload"], "batch": bc, "n": n, "value": bc * K / t, "unit": "inversions/s",
+           "ms_per_step": 1e3 * t / K, "median_ms": r["median_ms"], "min_ms": r["min_ms"],
+           "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": ach / HBM_PEAK_GBS, "traffic": None, "bytes_per_unit": bytes_alg},
+           "potri": {"value": bc * K / rp["total_s"], "ms_per_step": 1e3 * rp["total_s"] / K,
+                     "achieved_GBs": bytes_alg * bc * K / rp["total_s"] / 1e9},
+           "trmm_LLTN": {"value": bc * K / rt["total_s"], "unit": "products/s", "ms_per_step": 1e3 * rt["total_s"] / K,
+                         "achieved_GBs": (4 * n * (n + 1) + 16 * n * n) * bc * K / rt["total_s"] / 1e9},
+           # PSTRF reads and writes the full n x n matrix, plus the pivot vector
+           "pstrf": {"value": bc * K / rq["total_s"], "unit": "factorizations/s",
+                     "ms_per_step": 1e3 * rq["total_s"] / K,
+                     "achieved_GBs": (16 * n * n + 4 * n) * bc * K / rq["total_s"] / 1e9}}
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as po
+        bs = min(4000, bc)
+        A_s = A0[:bs].cpu().numpy()
+        Xg = X_sample[:bs].cpu().numpy()
         t0 = time.perf_counter()
-        # one OpenMP batch per distinct order keeps the loop inside C
-        for n in np.unique(ns_h[:bs]):
-            idx = np.nonzero(ns_h[:bs] == n)[0]
-            for i in idx:
-                L.orc_posv(b"R", nrhs, int(n), Aw[offA[i]:].ctypes.data_as(ctypes.c_void_p), int(n),
-                           Bw[offB[i]:].ctypes.data_as(ctypes.c_void_p), nrhs)
-        dt = time.perf_counter() - t0
-        best = dt if best is None else min(best, dt)
-    return {"value": bs / best, "unit": unit, "cores": 1, "kind": "port",
-            "sample": "first %d systems of the non-uniform batch, oracle FP64 POSV, one thread" % bs}
+        worst = 0.0
+        for i in range(bs):
+            L, inf = po.potrf(A_s[i])
+            Xo, inf2 = po.potri(L)
+            if i < 200:
+                Xi = np.tril(Xg[i].T)
+                Xf = Xi + np.tril(Xi, -1).T
+                worst = max(worst, float(np.abs(Xf @ A_s[i] - np.eye(n)).max()
+                                         / (np.abs(Xf).max() * np.abs(A_s[i]).max() * n)))
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": bs / tc, "unit": "inversions/s", "cores": 1, "kind": "port",
+                               "sample": "%d of the %d matrices, FP64 oracle POTRF + POTRI restatement, one thread "
+                                         "(includes the residual check of 200)" % (bs, bc)}
+        out["inverse_residual_max_sample"] = worst
+    return out
 
 
 def main():
@@ -237,7 +683,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c4", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="all", choices=["all"] + sorted(CONFIGS))
     ap.add_argument("--cpu-sample", type=int, default=0, help="matrices in the CPU-baseline sample (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -252,291 +698,57 @@ def main():
     import torch
     import pykblas
 
-    cfg = CONFIGS[args.config]
-    n, bc, nrhs = cfg["n"], cfg["batch"], cfg["nrhs"]
     K, W = args.steps, args.warmup
-    print("bench: config %s (%s), generating inputs" % (args.config, cfg["workload"]), file=sys.stderr, flush=True)
-    vb = None
-    if cfg["op"] in ("vposv", "vsposv"):
-        vb = gen_vbatch(torch, cfg, cfg.get("seed", 1234 + int(args.config[1])))
-        A0, B0, kappa = None, None, None
-    else:
-        A0, B0, kappa = gen_inputs(torch, cfg, cfg.get("seed") or 1234 + int(args.config[1]))
-    torch.cuda.synchronize()
     h = pykblas.Handle(torch.cuda.current_stream().cuda_stream)
-    lib = h.lib
-    info = torch.zeros(bc, dtype=torch.int32, device="cuda")
-    iters = torch.zeros(bc, dtype=torch.int32, device="cuda")
-    NA = n * n
-    out = {}
-
-    if cfg["op"] == "vsposv":
-        # non-uniform mixed POSV: A, B read-only, X written; every step restores A
-        # (the full-precision path overwrites it with L), outside the timed region
-        ns, ns_h, offA, offB, Abuf0, Bbuf0 = vb
-        nrhs_arr = torch.full((bc,), nrhs, dtype=torch.int32, device="cuda")
-        lib.kblas_posv_vbatch_wsquery(h.h, n, bc)
-        assert lib.kblasAllocateWorkspace(h.h) == pykblas.KBLAS_Success
-        offA_d = torch.from_numpy(offA[:-1] * 8).cuda()
-        offB_d = torch.from_numpy(offB[:-1] * 8).cuda()
-        As = [Abuf0.clone() for _ in range(K + W)]
-        Xbuf = torch.empty_like(Bbuf0)
-        pA = [offA_d + a.data_ptr() for a in As]
-        pB = offB_d + Bbuf0.data_ptr()
-        pX = offB_d + Xbuf.data_ptr()
+    names = ["c4", "c1", "c2", "c3", "c5"] if args.config == "all" else [args.config]
+    results = {}
+    t_start = time.perf_counter()
+    for name in names:
+        cfg = CONFIGS[name]
+        if cfg["op"] in ("vposv", "vsposv"):
+            results[name] = run_vbatch(torch, pykblas, h, name, cfg, K, W, not args.no_cpu)
+        elif cfg["op"] == "poti":
+            results[name] = run_poti(torch, pykblas, h, cfg, K, W, not args.no_cpu)
+        else:
+            results[name] = run_uniform(torch, pykblas, h, name, cfg, K, W, not args.no_cpu, args.cpu_sample)
         torch.cuda.synchronize()
-
-        def call(i):
-            st = lib.kblasDsposv_vbatch(h.h, b"R", b"L", nrhs_arr.data_ptr(), ns.data_ptr(), nrhs, n,
-                                        pA[i].data_ptr(), ns.data_ptr(), pB.data_ptr(), nrhs_arr.data_ptr(),
-                                        pX.data_ptr(), nrhs_arr.data_ptr(), bc, info.data_ptr(), iters.data_ptr())
-            assert st == pykblas.KBLAS_Success, st
-        t, wall = time_steps(torch, call, K, W)
-        nn = ns_h.astype(np.float64)
-        per_unit_bytes = float(np.mean(4 * nn * (nn + 1) + 16 * nn * nrhs))
-        flops = float(np.sum(nn ** 3 / 3 + nn ** 2 / 2 + nn / 6 + 2 * nrhs * nn ** 2))
-        value = bc * K / t
-        metric, unit = "solves/s (non-uniform mixed DSPOSV n~U{128..512}, 64 RHS)", "solves/s"
-        dtype = "f32+f64"
-        X_sample = None
-        out["gflops"] = flops * K / t / 1e9
-        out["fp64_peak_frac"] = out["gflops"] / 78600.0
-        it_h = iters.cpu().numpy()
-        hist = {}
-        for v in it_h.tolist():
-            hist[str(v)] = hist.get(str(v), 0) + 1
-        out["iter_hist"] = hist
-        out["info_nonzero"] = int((info != 0).sum().item())
-        del As
-    elif cfg["op"] == "vposv":
-        ns, ns_h, offA, offB, Abuf0, Bbuf0 = vb
-        nrhs_arr = torch.full((bc,), nrhs, dtype=torch.int32, device="cuda")
-        lib.kblas_posv_vbatch_wsquery(h.h, n, bc)
-        assert lib.kblasAllocateWorkspace(h.h) == pykblas.KBLAS_Success
-        offA_d = torch.from_numpy(offA[:-1] * 8).cuda()
-        offB_d = torch.from_numpy(offB[:-1] * 8).cuda()
-        As = [Abuf0.clone() for _ in range(K + W)]
-        Bs = [Bbuf0.clone() for _ in range(K + W)]
-        pA = [offA_d + a.data_ptr() for a in As]
-        pB = [offB_d + b.data_ptr() for b in Bs]
-        torch.cuda.synchronize()
-
-        def issue(i):
-            st = lib.kblasDposv_vbatch(h.h, b"R", b"L", nrhs_arr.data_ptr(), ns.data_ptr(), nrhs, n,
-                                       pA[i].data_ptr(), ns.data_ptr(), pB[i].data_ptr(), nrhs_arr.data_ptr(),
-                                       bc, info.data_ptr())
-            assert st == pykblas.KBLAS_Success, st
-        # one HIP graph per step (sort kernels + persistent kernel), captured on torch's stream
-        graphs = []
-        cap = torch.cuda.Stream()
-        for i in range(K + W):
-            gr = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(gr, stream=cap):
-                h.set_stream(torch.cuda.current_stream().cuda_stream)
-                issue(i)
-            graphs.append(gr)
-        h.set_stream(torch.cuda.current_stream().cuda_stream)
-        torch.cuda.synchronize()
-
-        def call(i):
-            graphs[i].replay()
-        t, wall = time_steps(torch, call, K, W)
-        nn = ns_h.astype(np.float64)
-        per_unit_bytes = float(np.mean(8 * nn * (nn + 1) + 16 * nn * nrhs))
-        flops = float(np.sum(nn ** 3 / 3 + nn ** 2 / 2 + nn / 6 + 2 * nrhs * nn ** 2))
-        value = bc * K / t
-        metric, unit = "solves/s (non-uniform batched DPOSV n~U{8..256}, 4 RHS)", "solves/s"
-        dtype = "f64"
-        X_sample = None
-        out["gflops"] = flops * K / t / 1e9
-        out["fp64_peak_frac"] = out["gflops"] / 78600.0
-        out["info_nonzero"] = int((info != 0).sum().item())
-        del As
-    elif cfg["op"] == "potrf":
-        As = [A0.clone() for _ in range(K + W)]
-
-        def call(i):
-            st = lib.kblasDpotrf_batch_strided(h.h, b"L", n, As[i].data_ptr(), n, NA, bc, info.data_ptr())
-            assert st == pykblas.KBLAS_Success, st
-        t, wall = time_steps(torch, call, K, W)
-        per_unit_bytes = 8 * n * (n + 1)
-        value = bc * K / t
-        metric, unit = "factorizations/s (batched DPOTRF n=%d)" % n, "factorizations/s"
-        dtype = "f64"
-        X_sample = None
-        del As
-    elif cfg["op"] == "poti":
-        As = [A0.clone() for _ in range(K + W)]
-
-        def call(i):
-            st = lib.kblasDpoti_batch_strided(h.h, b"L", n, As[i].data_ptr(), n, NA, bc, info.data_ptr())
-            assert st == pykblas.KBLAS_Success, st
-        t, wall = time_steps(torch, call, K, W)
-        assert int((info != 0).sum().item()) == 0
-        X_sample = As[0]
-        # POTRI alone, on the factors
-        Ls = [A0.clone() for _ in range(K + W)]
-        for i in range(K + W):
-            assert lib.kblasDpotrf_batch_strided(h.h, b"L", n, Ls[i].data_ptr(), n, NA, bc,
-                                                 info.data_ptr()) == pykblas.KBLAS_Success
-
-        def call_potri(i):
-            st = lib.kblasDpotri_batch_strided(h.h, b"L", n, Ls[i].data_ptr(), n, NA, bc, info.data_ptr())
-            assert st == pykblas.KBLAS_Success, st
-        tp, _ = time_steps(torch, call_potri, K, W)
-        out["potri"] = {"value": bc * K / tp, "ms_per_step": 1000 * tp / K,
-                        "achieved_GBs": 8 * n * (n + 1) * bc * K / tp / 1e9}
-        # TRMM B := L^T B (left, lower, transposed) with the factors and n right-hand sides
-        Bt = [torch.randn((bc, n, n), device="cuda", dtype=torch.float64) for _ in range(K + W)]
-
-        def call_trmm(i):
-            st = lib.kblasDtrmm_batch_strided(h.h, b"L", b"L", b"T", b"N", n, n, 1.0, Ls[i].data_ptr(), n, NA,
-                                              Bt[i].data_ptr(), n, NA, bc)
-            assert st == pykblas.KBLAS_Success, st
-        tt, _ = time_steps(torch, call_trmm, K, W)
-        out["trmm_LLTN"] = {"value": bc * K / tt, "unit": "products/s", "ms_per_step": 1000 * tt / K,
-                            "achieved_GBs": (4 * n * (n + 1) + 16 * n * n) * bc * K / tt / 1e9}
-        del Bt
-        # pivoted Cholesky (PSTRF) of the same SPD inputs
-        Ps = [A0.clone() for _ in range(K + W)]
-        pv = torch.zeros((bc, n), dtype=torch.int32, device="cuda")
-        rk = torch.zeros(bc, dtype=torch.int32, device="cuda")
-
-        def call_pstrf(i):
-            st = lib.kblasDpstrf_batch_strided(h.h, n, Ps[i].data_ptr(), n, NA, pv.data_ptr(), n, rk.data_ptr(), bc)
-            assert st == pykblas.KBLAS_Success, st
-        tq, _ = time_steps(torch, call_pstrf, K, W)
-        assert int((rk != n).sum().item()) == 0
-        out["pstrf"] = {"value": bc * K / tq, "unit": "factorizations/s", "ms_per_step": 1000 * tq / K,
-                        "achieved_GBs": (8 * n * (n + 1) + 4 * n) * bc * K / tq / 1e9}
-        del Ps
-        per_unit_bytes = 8 * n * (n + 1)
-        value = bc * K / t
-        metric, unit = "inversions/s (batched DPOTI n=%d)" % n, "inversions/s"
-        dtype = "f64"
-        del Ls
-    elif cfg["op"] == "posv":
-        As = [A0.clone() for _ in range(K + W)]
-        Bs = [B0.clone() for _ in range(K + W)]
-
-        def call(i):
-            st = lib.kblasDposv_batch_strided(h.h, b"R", b"L", nrhs, n, As[i].data_ptr(), n, NA,
-                                              Bs[i].data_ptr(), nrhs, nrhs * n, bc, info.data_ptr())
-            assert st == pykblas.KBLAS_Success, st
-        t, wall = time_steps(torch, call, K, W)
-        per_unit_bytes = 8 * n * (n + 1) + 16 * n * nrhs
-        value = bc * K / t
-        metric, unit = "solves/s (batched DPOSV n=%d, %d RHS)" % (n, nrhs), "solves/s"
-        dtype = "f64"
-        X_sample = Bs[0]
-        del As
-    else:
-        # mixed precision: A and B read-only, X written (A only on fallback)
-        As = [A0.clone() for _ in range(K + W)]
-        Xs = torch.empty_like(B0)
-
-        def call_mixed(i):
-            st = lib.kblasDsposv_batch_strided(h.h, b"R", b"L", nrhs, n, As[i].data_ptr(), n, NA,
-                                               B0.data_ptr(), nrhs, nrhs * n, Xs.data_ptr(), nrhs, nrhs * n,
-                                               bc, info.data_ptr(), iters.data_ptr())
-            assert st == pykblas.KBLAS_Success, st
-        print("bench: timing mixed FP32+IR POSV", file=sys.stderr, flush=True)
-        t, wall = time_steps(torch, call_mixed, K, W)
-        it_h = iters.cpu().numpy()
-        X_sample = Xs
-        del As
-        # plain FP64 POSV on the same inputs
-        As = [A0.clone() for _ in range(K + W)]
-        Bs = [B0.clone() for _ in range(K + W)]
-
-        def call_fp64(i):
-            st = lib.kblasDposv_batch_strided(h.h, b"R", b"L", nrhs, n, As[i].data_ptr(), n, NA,
-                                              Bs[i].data_ptr(), nrhs, nrhs * n, bc, info.data_ptr())
-            assert st == pykblas.KBLAS_Success, st
-        print("bench: timing FP64 POSV", file=sys.stderr, flush=True)
-        t64, _ = time_steps(torch, call_fp64, K, W)
-        X64 = Bs[0]
-        del As
-        per_unit_bytes = 4 * n * (n + 1) + 16 * n * nrhs     # A lower read once (FP64), B in, X out
-        value = bc * K / t
-        metric = "solves/s (batched POSV n=64 mixed FP32+IR, bwd err<=1e-13)"
-        unit = "solves/s"
-        dtype = "f32+f64"
-        out["fp64_posv"] = {"value": bc * K / t64, "ms_per_step": 1000 * t64 / K,
-                            "achieved_GBs": (8 * n * (n + 1) + 16 * n * nrhs) * bc * K / t64 / 1e9}
-        out["mixed_over_fp64"] = (bc * K / t) / (bc * K / t64)
-        hist = {}
-        for v in it_h.tolist():
-            hist[str(v)] = hist.get(str(v), 0) + 1
-        out["iter_hist"] = hist
-        out["fallbacks"] = int((it_h < 0).sum())
-
-    ms = 1000.0 * t / K
-    achieved = per_unit_bytes * bc / (t / K) / 1e9
-    line = {
-        "metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": K, "warmup": W,
-        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": dtype, "data": "synthetic (seeded, generated on the GPU)",
-        "config": {"workload": cfg["workload"], "batch": bc, "n": n, "nrhs": nrhs},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_unit": per_unit_bytes},
-    }
-    line.update(out)
-    tr = measured_traffic(args.config)
-    if tr is not None:
-        line["roofline"]["traffic"] = tr["bytes_per_launch"]
-        line["roofline"]["traffic_source"] = tr["source"]
-    torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        log("%s: %.4g %s, %.4f ms/step" % (name, results[name]["value"], results[name]["unit"],
+                                            results[name]["ms_per_step"]))
     if args.copy_calibration > 0:
         nb = args.copy_calibration << 20
-        src = torch.ones(nb // 8, dtype=torch.float64, device="cuda")
+        src = torch.ones(nb // 8, dtype=torch.float64, device=DEV)
         dst = torch.empty_like(src)
         for _ in range(3):
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    if not args.no_cpu and cfg["op"] == "poti":
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import pyoracle as po
-        bs = min(args.cpu_sample or 4000, bc)
-        A_s = A0[:bs].cpu().numpy()
-        Xg = X_sample[:bs].cpu().numpy()
-        print("bench: CPU baseline on %d matrices, 1 thread" % bs, file=sys.stderr, flush=True)
-        t0 = time.perf_counter()
-        worst = 0.0
-        for i in range(bs):
-            L, inf = po.potrf(A_s[i])
-            Xo, inf2 = po.potri(L)
-            if i < 200:   # accuracy of the GPU inverse on a sub-sample: |X A - I| / (|X| |A|)
-                Xi = np.tril(Xg[i].T)
-                Xf = Xi + np.tril(Xi, -1).T
-                worst = max(worst, float(np.abs(Xf @ A_s[i] - np.eye(n)).max()
-                                         / (np.abs(Xf).max() * np.abs(A_s[i]).max() * n)))
-        tc = time.perf_counter() - t0
-        line["cpu_baseline"] = {"value": bs / tc, "unit": unit, "cores": 1, "kind": "port",
-                                "sample": "%d of the %d matrices, FP64 oracle POTRF + POTRI restatement, "
-                                          "one thread (includes the residual check of 200)" % (bs, bc)}
-        line["inverse_residual_max_sample"] = worst
-    elif not args.no_cpu and cfg["op"] in ("vposv", "vsposv"):
-        line["cpu_baseline"] = cpu_baseline_vbatch(cfg, vb, K, unit)
-    elif not args.no_cpu:
-        cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
-        bs = args.cpu_sample or {"c1": 20000, "c2": 50000, "c4": 2000}[args.config]
-        bs = min(bs, bc)
-        A_s = A0[:bs].cpu().numpy()
-        B_s = B0[:bs].cpu().numpy() if B0 is not None else None
-        Xg = X_sample[:bs].cpu().numpy() if X_sample is not None else None
-        print("bench: CPU baseline on %d matrices, %d threads" % (bs, cores), file=sys.stderr, flush=True)
-        cb = cpu_baseline(cfg, A_s, B_s, Xg, cores)
-        line["cpu_baseline"] = {"value": cb["value"], "unit": unit, "cores": cores, "kind": "port",
-                                "sample": "%d of the %d systems, FP64 oracle POSV/POTRF restatement, OpenMP"
-                                          % (bs, bc),
-                                "lapack": cb.get("lapack")}
-        if "berr_max" in cb:
-            line["berr_max_sample"] = cb["berr_max"]
-            line["berr_median_sample"] = cb["berr_median"]
     h.close()
+
+    head_name = names[0]
+    head = results[head_name]
+    cfg = CONFIGS[head_name]
+    metric = {
+        "sposv": "solves/s (batched POSV n=64 mixed FP32+IR, bwd err<=1e-13)",
+        "potrf": "factorizations/s (batched DPOTRF n=%d)" % cfg["n"],
+        "posv": "solves/s (batched DPOSV n=%d, %d RHS)" % (cfg["n"], cfg["nrhs"]),
+        "vposv": "solves/s (non-uniform batched DPOSV n~U{8..256}, 4 RHS)",
+        "vsposv": "solves/s (non-uniform mixed DSPOSV n~U{128..512}, 64 RHS)",
+        "poti": "inversions/s (batched DPOTI n=%d)" % cfg["n"],
+    }[cfg["op"]]
+    line = {
+        "metric": metric, "value": head["value"], "unit": head["unit"], "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64" if cfg["op"] in ("sposv", "vsposv") else "f64",
+        "data": "synthetic (seeded, generated on the GPU)",
+        "config": {"workload": cfg["workload"], "batch": cfg["batch"], "n": cfg["n"], "nrhs": cfg["nrhs"]},
+    }
+    for k, v in head.items():
+        if k not in ("workload", "batch", "n", "nrhs", "value", "unit", "ms_per_step"):
+            line[k] = v
+    if len(names) > 1:
+        line["configs"] = {k: results[k] for k in names[1:]}
+    line["bench_wall_s"] = time.perf_counter() - t_start
     print(json.dumps(line))
 
 

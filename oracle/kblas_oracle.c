@@ -817,6 +817,42 @@ int orc_lapack_posv_batch(char side, int m, int n, double* A, int lda, long stri
     return 1;
 }
 
+/* The same loop over a non-uniform batch packed in one buffer: matrix b is
+ * n[b] x n[b] at A + offA[b] (lda = n[b]), its right-hand sides (side 'R':
+ * nrhs x n[b], ldb = nrhs) at B + offB[b]; dynamic schedule, largest first
+ * when the caller orders them so. */
+int orc_lapack_posv_vbatch(int nrhs, const int* n, double* A, const long* offA, double* B,
+                           const long* offB, int count, int* info)
+{
+    lapack_load();
+    if (!g_lapack) return 0;
+    const double one = 1.0;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(orc_threads())
+    for (int b = 0; b < count; ++b) {
+        int r = 0, nb = n[b], m = nrhs;
+        double* Ab = A + offA[b];
+        double* Bb = B + offB[b];
+        p_potrf("L", &nb, Ab, &nb, &r);
+        if (r == 0) {
+            p_trsm("R", "L", "T", "N", &m, &nb, &one, Ab, &nb, Bb, &m);
+            p_trsm("R", "L", "N", "N", &m, &nb, &one, Ab, &nb, Bb, &m);
+        }
+        if (info) info[b] = r;
+    }
+    return 1;
+}
+
+/* the oracle's own POSV over the same packed non-uniform batch */
+void orc_posv_vbatch(int nrhs, const int* n, double* A, const long* offA, double* B, const long* offB,
+                     int count, int* info)
+{
+#pragma omp parallel for schedule(dynamic, 1) num_threads(orc_threads())
+    for (int b = 0; b < count; ++b) {
+        int r = orc_posv('R', nrhs, n[b], A + offA[b], n[b], B + offB[b], nrhs);
+        if (info) info[b] = r;
+    }
+}
+
 /* ======================= inverse family and TRMM ======================== */
 /* TRTRI, lower: the row recurrence of the reference register kernel
  * (src/batch_triangular/Xtrtri_batch_kernels.cuh:37-79):

@@ -125,6 +125,10 @@ void orc_sposv_batch(char side, int m, int n, const double* A, int lda, long str
 int  orc_lapack_available(void);
 const char* orc_lapack_name(void);
 int  orc_lapack_potrf_batch(int n, double* A, int lda, long strideA, int batch, int* info);
+int  orc_lapack_posv_vbatch(int nrhs, const int* n, double* A, const long* offA, double* B,
+                            const long* offB, int count, int* info);
+void orc_posv_vbatch(int nrhs, const int* n, double* A, const long* offA, double* B, const long* offB,
+                     int count, int* info);
 int  orc_lapack_posv_batch(char side, int m, int n, double* A, int lda, long strideA,
                            double* B, int ldb, long strideB, int batch, int* info);
 

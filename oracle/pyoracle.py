@@ -60,6 +60,8 @@ _SIG = {
     "orc_lapack_name": (ctypes.c_char_p, []),
     "orc_lapack_potrf_batch": (_i, [_i, _p, _i, _l, _i, _p]),
     "orc_lapack_posv_batch": (_i, [_c, _i, _i, _p, _i, _l, _p, _i, _l, _i, _p]),
+    "orc_lapack_posv_vbatch": (_i, [_i, _p, _p, _p, _p, _p, _i, _p]),
+    "orc_posv_vbatch": (None, [_i, _p, _p, _p, _p, _p, _i, _p]),
     "orc_gen_spd_shift_batch": (None, [_u64, _i, _d, _p, _i, _l, _l, _i]),
     "orc_gen_spd_cond_batch": (None, [_u64, _i, _d, _d, _p, _i, _l, _l, _i, _p]),
     "orc_gen_rhs_batch": (None, [_u64, _i, _i, _p, _i, _l, _l, _i]),
