@@ -43,6 +43,18 @@ int kblasDposv_vbatch(kblasHandle_t handle, char side, char uplo, int* m, int* n
 int kblasSposv_vbatch(kblasHandle_t handle, char side, char uplo, int* m, int* n,
                       int max_m, int max_n, float** A, int* lda, float** B, int* ldb,
                       int batchCount, int* info_array);
+/* No counterpart in the reference.  For callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): kblasDposv_vbatch with its size-sorted queue kept in a fixed
+ * device-resident scratch area of the library (one per GPU, held until the
+ * process ends) instead of the handle's workspace, so it needs no workspace
+ * query, allocates nothing and performs no host synchronisation.  At most
+ * 1,048,576 matrices per call (KBLAS_InsufficientWorkspace above).  Calls
+ * that overlap on one GPU (two streams or two host threads) share the area and
+ * race; serialise them.  Same arguments and results as kblasDposv_vbatch.     */
+int kblasDposv_vbatch_noalloc(kblasHandle_t handle, char side, char uplo, int* m, int* n,
+                              int max_m, int max_n, double** A, int* lda, double** B, int* ldb,
+                              int batchCount, int* info_array);
 
 /* ---- mixed precision POSV (LAPACK dsposv semantics, batched) -------------
  * FP32 Cholesky and triangular sweeps, FP64 residual refinement until

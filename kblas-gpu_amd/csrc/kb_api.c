@@ -431,6 +431,33 @@ int kblasDsposv_vbatch(kblasHandle_t h, char side, char uplo, int* m, int* n, in
                        int* info, int* iter)
 { (void)max_m; return vchol(h, 'm', KB_MODE_SPOSV, side, uplo, m, n, max_n, A, lda, (void*)B, ldb, X, ldx, bc, info, iter); }
 
+/* kblasDposv_vbatch with the queue in the library's fixed per-device scratch
+ * area instead of the handle's workspace: no allocation, no workspace query.
+ * At most KB_NOALLOC_MAX_BATCH matrices (KBLAS_InsufficientWorkspace above). */
+int kblasDposv_vbatch_noalloc(kblasHandle_t h, char side, char uplo, int* m, int* n, int max_m, int max_n,
+                              double** A, int* lda, double** B, int* ldb, int bc, int* info)
+{
+    (void)max_m;
+    if (!h) return KBLAS_Error_NotInitialized;
+    int u = upl(uplo);
+    int sd = sideof(side);
+    if (u < 0 || sd < 0 || bc < 0 || !n || !A || !lda || !m || !B || !ldb) return KBLAS_Error_WrongInput;
+    if (u == 1) return KBLAS_NotImplemented;
+    if (bc == 0) return KBLAS_Success;
+    if (bc > KB_NOALLOC_MAX_BATCH) {
+        fprintf(stderr, "kblasDposv_vbatch_noalloc: batchCount %d exceeds the fixed scratch capacity of %d "
+                        "matrices; use kblasDposv_vbatch with a workspace\n", bc, KB_NOALLOC_MAX_BATCH);
+        return KBLAS_InsufficientWorkspace;
+    }
+    int err = 0;
+    void* ws = kb_noalloc_scratch(&err);
+    if (!ws) return kb_hip_status(err ? err : 1);
+    kb_mat_t Am = mat_vptrs(A, lda);
+    kb_mat_t Bm = mat_vptrs(B, ldb);
+    return launched(kb_launch_vchol('d', KB_MODE_POSV, m, n, sd == 0, Am, Bm, Bm, info, NULL, bc, max_n, ws,
+                                    h->stream));
+}
+
 /* ============ inverse family: TRTRI / LAUUM / POTRI / POTI ============== */
 /* Front-ends of src/batch_triangular/X{trtri,lauum,potri,poti}_batch.cu: Upper
  * returns KBLAS_NotImplemented (Xpotri_batch_drivers.cuh:38-41,

@@ -24,6 +24,9 @@ int kblasCreate(kblasHandle_t* handle)
     if (!h) return KBLAS_Error_Allocation;
     h->device = dev;
     h->stream = 0;
+    /* this device's copy of the fixed scratch area of kblasDposv_vbatch_noalloc;
+     * a failed lookup is reported by that entry point, which retries it */
+    (void)kb_noalloc_init();
     *handle = h;
     return KBLAS_Success;
 }

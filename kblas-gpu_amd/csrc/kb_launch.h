@@ -65,7 +65,17 @@ int kb_launch_trsm(char prec, int side_left, int trans, int unit, int m, int n, 
  * the solution, iter as in dsposv).  ws: device workspace of
  * kb_vchol_ws_bytes(batch) bytes. */
 enum { KB_MODE_SPOSV = 3 };
+/* ints of the queue header (histogram, offsets, counters) before the order array */
+#define KB_VCHOL_WS_HDR_INTS (2 * 4096 + 64)
 size_t kb_vchol_ws_bytes(int batch);
+
+/* Fixed per-device scratch area holding the queue for up to
+ * KB_NOALLOC_MAX_BATCH matrices (kb_noalloc.hip).  kb_noalloc_init looks up
+ * the current device's copy (hipSuccess or the lookup's error);
+ * kb_noalloc_scratch returns its device address (NULL and *err on failure). */
+#define KB_NOALLOC_MAX_BATCH (1 << 20)
+int kb_noalloc_init(void);
+void* kb_noalloc_scratch(int* err);
 int kb_launch_vchol(char prec, int mode, const int* vm, const int* vn, int side_left,
                     kb_mat_t A, kb_mat_t B, kb_mat_t X, int* info, int* iter, int batch,
                     int max_n, void* ws, hipStream_t s);

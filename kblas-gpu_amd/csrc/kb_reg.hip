@@ -61,7 +61,7 @@ extern "C" int kb_vchol_m(int, const int*, const int*, int, kb_mat_t, kb_mat_t, 
 // hist + offsets + counters + the order array (see kb_vchol_impl.h)
 extern "C" size_t kb_vchol_ws_bytes(int batch)
 {
-    return sizeof(int) * ((size_t)(2 * 4096 + 64) + (size_t)(batch > 0 ? batch : 0));
+    return sizeof(int) * ((size_t)KB_VCHOL_WS_HDR_INTS + (size_t)(batch > 0 ? batch : 0));
 }
 
 extern "C" int kb_launch_vchol(char prec, int mode, const int* vm, const int* vn, int side_left,

@@ -21,6 +21,7 @@
 
 constexpr int KB_VQ_BINS = 4096;   // orders >= 4095 share the last bin
 constexpr int KB_VQ_HDR = 2 * KB_VQ_BINS + 64;   // ints: hist, offsets, counters
+static_assert(KB_VQ_HDR == KB_VCHOL_WS_HDR_INTS, "queue header size");
 // counter[0]: next ticket of the workgroup phase; counter[1]: number of orders
 // above the register range (tickets [0, counter[1])); counter[2]: next ticket
 // of the wavefront phase (starts at counter[1])

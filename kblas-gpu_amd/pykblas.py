@@ -104,6 +104,7 @@ _DONLY = {
                                           P, c_int, c_long, c_int, P, P]),
     "kblasDsposv_batch": (c_int, [P, c_char, c_char, c_int, c_int, P, c_int, P, c_int, P, c_int, c_int, P, P]),
     "kblasDsposv_vbatch": (c_int, [P, c_char, c_char, P, P, c_int, c_int, P, P, P, P, P, P, c_int, P, P]),
+    "kblasDposv_vbatch_noalloc": (c_int, [P, c_char, c_char, P, P, c_int, c_int, P, P, P, P, c_int, P]),
 }
 _TSIG["pstrf_batch"] = (c_int, [P, c_int, P, c_int, P, P, c_int])
 _TSIG["pstrf_batch_strided"] = (c_int, [P, c_int, P, c_int, c_int, P, c_int, P, c_int])
