@@ -260,14 +260,23 @@ KB_DEV void reg_sposv_one(int n, int nrhs, int side_left, double* Ap, int lda, d
     const double anrm = group_max<G>(rowsum);
     const double cte = anrm * 1.1102230246251565e-16 * __builtin_sqrt((double)n);
 
+    // FP32 range of B, then of A (dsposv's DLAG2S / DLAT2S checks): ITER = -2
+    int small = 1;
+    for (int c = 0; c < nrhs; ++c)
+        if (r < n) small &= !(__builtin_fabs(gld(rhs_elem(Bp, ldb, side_left, r, c))) > 3.4028234663852886e38);
+#pragma unroll
+    for (int k = 0; k < NB; ++k) small &= !(k <= r && __builtin_fabs(ad[k]) > 3.4028234663852886e38);
+    const bool ovf = !group_all<G>(small);
+
     float a[NB];
 #pragma unroll
     for (int k = 0; k < NB; ++k) a[k] = (k <= r) ? (float)ad[k] : 0.0f;
     float d32 = 1.0f, inv32 = 1.0f;
-    const int finfo = reg_potrf<float, NB, G>(a, n, r, d32, inv32);
+    int finfo = 0;
+    if (!ovf) finfo = reg_potrf<float, NB, G>(a, n, r, d32, inv32);
 
-    int iters = 0;
-    bool ok = (finfo == 0);
+    int iters = ovf ? -2 : (finfo != 0 ? -3 : 0);
+    bool ok = !ovf && (finfo == 0);
     if (ok) {
 #pragma unroll
         for (int k = 0; k < NB; ++k) a[k] = (k < r) ? a[k] : 0.0f;
@@ -282,9 +291,13 @@ KB_DEV void reg_sposv_one(int n, int nrhs, int side_left, double* Ap, int lda, d
                 x[c] = 0.0;
                 res[c] = v;
             }
-            bool conv = false;
+            bool conv = false, rovf = false;
             int it = 0;
             for (; it <= KB_IR_ITERMAX; ++it) {
+                int fits = 1;
+#pragma unroll
+                for (int c = 0; c < RC; ++c) fits &= !(__builtin_fabs(res[c]) > 3.4028234663852886e38);
+                if (!group_all<G>(fits)) { rovf = true; break; }     // DLAG2S(R) overflow
                 float z[RC];
 #pragma unroll
                 for (int c = 0; c < RC; ++c) z[c] = (float)res[c];
@@ -314,7 +327,11 @@ KB_DEV void reg_sposv_one(int n, int nrhs, int side_left, double* Ap, int lda, d
                 }
                 if (all) { conv = true; break; }
             }
-            if (!conv) { ok = false; break; }
+            if (!conv) {
+                ok = false;
+                iters = rovf ? -2 : -(KB_IR_ITERMAX + 1);
+                break;
+            }
             if (it > iters) iters = it;
 #pragma unroll
             for (int c = 0; c < RC; ++c)
@@ -323,7 +340,6 @@ KB_DEV void reg_sposv_one(int n, int nrhs, int side_left, double* Ap, int lda, d
     }
     int inf = 0;
     if (!ok) {
-        iters = (finfo != 0) ? -3 : -(KB_IR_ITERMAX + 1);
         // FP64 fallback on the same registers; A receives L as in LAPACK dsposv
         double dd = 1.0, dinv = 1.0;
         inf = reg_potrf<double, NB, G>(ad, n, r, dd, dinv);

@@ -6,11 +6,13 @@
 // wavefront: FP64 A kept in registers as all sixteen 16x16 tiles (the lower
 // ten as loaded, the upper six built by transposition) for the residual, its
 // FP32 copy factored with FP32 MFMA (2x the FP64 matrix-core rate), the unit
-// FP32 factor kept in LDS for the refinement sweeps; on a non-positive FP32
-// pivot or no convergence after 30 steps the FP64 tiles are factored in place
-// (A then holds L, as dsposv) and solved in FP64.
+// FP32 factor kept in LDS for the refinement sweeps; on an FP32 overflow of A,
+// B or a residual (ITER = -2), a non-positive FP32 pivot (-3) or no
+// convergence after 30 steps (-31) the FP64 tiles are factored in place (A
+// then holds L, as dsposv) and solved in FP64.
 // ---------------------------------------------------------------------------
 constexpr int KB_TILE_ITERMAX = 30;
+constexpr double KB_FLT_MAX = 3.4028234663852886e38;
 
 struct MixLds {
     union {
@@ -283,10 +285,29 @@ k_tile_sposv(int n, int nrhs, int side_left, BatchArg<double> A, BatchArg<double
     kb_d4 ta[10];
     tile_load<double>(ta, Ap, lda, n, lane);
 
+    // FP32 range of B, then of A (dsposv's DLAG2S / DLAT2S checks): ITER = -2
+    bool big = false;
+    for (int c0 = 0; c0 < nrhs; c0 += 4) {
+#pragma unroll
+        for (int K = 0; K < 4; ++K) {
+            const int row = 16 * K + i;
+            const bool in = c0 + g < nrhs && row < n;
+            const double v = gld(rhs_elem(Bp, ldb, side_left, in ? row : 0, in ? c0 + g : 0));
+            big = big || (in && __builtin_fabs(v) > KB_FLT_MAX);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 10; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) big = big || __builtin_fabs(ta[t][r]) > KB_FLT_MAX;
+    const bool ovf = __builtin_amdgcn_ballot_w64(big) != 0;
+
     // FP32 factorization; the unit factor Lu = L D^-1 goes to LDS
     int iters = 0;
-    bool ok;
-    {
+    bool ok = false;
+    if (ovf) {
+        iters = -2;
+    } else {
         kb_f4 tf[10];
 #pragma unroll
         for (int t = 0; t < 10; ++t)
@@ -340,6 +361,10 @@ k_tile_sposv(int n, int nrhs, int side_left, BatchArg<double> A, BatchArg<double
         int it = 0;
         for (; it <= KB_TILE_ITERMAX; ++it) {
             // x += (L32 L32^T)^-1 r ; r = b - A x
+            bool rbig = false;
+#pragma unroll
+            for (int K = 0; K < 4; ++K) rbig = rbig || __builtin_fabs(rr[K]) > KB_FLT_MAX;
+            if (__builtin_amdgcn_ballot_w64(rbig) != 0) break;      // DLAG2S(R) overflow
             float z[4];
 #pragma unroll
             for (int K = 0; K < 4; ++K) z[K] = (float)rr[K];
@@ -360,7 +385,11 @@ k_tile_sposv(int n, int nrhs, int side_left, BatchArg<double> A, BatchArg<double
             const bool mine = !act || (rn <= xn * cte);
             if (__builtin_amdgcn_ballot_w64(!mine) == 0) { conv = true; break; }
         }
-        if (!conv) { ok = false; break; }
+        if (!conv) {
+            ok = false;
+            iters = (it > KB_TILE_ITERMAX) ? -(KB_TILE_ITERMAX + 1) : -2;
+            break;
+        }
         if (it > iters) iters = it;
 #pragma unroll
         for (int K = 0; K < 4; ++K) {
@@ -371,7 +400,6 @@ k_tile_sposv(int n, int nrhs, int side_left, BatchArg<double> A, BatchArg<double
 
     int inf = 0;
     if (!ok) {
-        if (iters != -3) iters = -(KB_TILE_ITERMAX + 1);
         wave_lds_sync();
         // the lower tiles of ta are A's (the mirrored upper parts of the
         // diagonal tiles never feed the lower triangle and are not stored)

@@ -38,7 +38,7 @@ def run_sposv(h, side, A, B, nrhs_rows, order):
     return kt.to_host(dX), kt.to_host(info), kt.to_host(it), kt.to_host(dA)
 
 
-@pytest.mark.parametrize("n", [16, 32, 64])
+@pytest.mark.parametrize("n", [16, 32, 40, 57, 64])
 @pytest.mark.parametrize("side", ["R", "L"])
 def test_sposv_conditioned_batch(h, n, side):
     batch = 40
@@ -92,3 +92,32 @@ def test_sposv_large_order_full_precision(h, n, side):
     for i in range(batch):
         assert po.solve_berr(side, po.col(A, i), po.col(X, i), po.col(B, i)) <= 1e-13
         assert po.potrf_berr(po.col(A, i), po.col(Aout, i)) <= 4 * n * kt.EPS64
+
+
+@pytest.mark.parametrize("n", [24, 50, 64])
+def test_sposv_fp32_overflow_iter_minus_2(h, n):
+    """ITER = -2 (LAPACK's DLAG2S / DLAT2S overflow checks): a column of B
+    beyond FP32 range in matrix 0, A scaled beyond FP32 range in matrix 2;
+    both are solved in FP64 (X equal to the FP64 oracle's within the solve
+    bar), the other matrices refine as usual and A stays untouched for them."""
+    batch = 4
+    A, kap = po.gen_spd_cond_batch(61, n, batch, 2.0, 3.0)
+    B = po.gen_rhs_batch(62, n, 3, batch)                 # side L: n x 3
+    B[0][1, :] *= 1e39                                    # right-hand side 1 of matrix 0
+    A[2] *= 1e39 / np.max(np.abs(A[2]))                   # |A| up to 1e39 > FLT_MAX
+    X, info, it, Aout = run_sposv(h, "L", A, B, 3, n)
+    assert np.all(info == 0)
+    assert it[0] == -2 and it[2] == -2, it
+    assert it[1] >= 0 and it[3] >= 0, it
+    for i in range(batch):
+        Ai, Bi, Xi = po.col(A, i), po.col(B, i), po.col(X, i)
+        _, _, it_r = po.sposv("L", Ai, Bi)
+        if i in (0, 2):
+            assert it_r == -2
+        assert po.solve_berr("L", Ai, Xi, Bi) <= 1e-13, (i, it[i])
+        if i in (0, 2):
+            _, Xo, _ = po.posv("L", Ai, Bi)
+            assert np.max(np.abs(Xi - Xo)) <= 64 * n * kt.EPS64 * np.max(np.abs(Xo)) * kap[i], i
+            assert po.potrf_berr(Ai, po.col(Aout, i)) <= 4 * n * kt.EPS64    # A holds L
+        else:
+            assert np.array_equal(Aout[i], A[i])
