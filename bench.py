@@ -185,6 +185,44 @@ def gen_vbatch(torch, cfg):
     return dict(ns=ns, ns_h=ns_h, offA=offA, offB=offB, A=Abuf, B=Bbuf, kappa=kappa)
 
 
+_CEILING = {}
+
+
+def copy_ceiling(torch, lib, total_bytes):
+    """Achievable HBM rate for a job that moves total_bytes (read + write):
+    the library's streaming copy kernel (16-byte vector accesses) copying
+    total_bytes / 2, gated and timed like the workloads, the best grid of a
+    few, median of five runs.  GB/s."""
+    half = (int(total_bytes) // 2 + 4095) // 4096 * 4096
+    if half in _CEILING:
+        return _CEILING[half]
+    import ctypes
+    f = lib.kb_stream_copy
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long, ctypes.c_int, ctypes.c_void_p]
+    src = torch.ones(half // 8, dtype=torch.float64, device=DEV)
+    dst = torch.empty_like(src)
+    st = torch.cuda.current_stream()
+    best = 0.0
+    for blocks in (1024, 2048, 4096, 8192):
+        ts = []
+        for r in range(6):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            torch.cuda._sleep(GATE_CYCLES)
+            e0.record(st)
+            assert f(dst.data_ptr(), src.data_ptr(), half, blocks, st.cuda_stream) == 0
+            e1.record(st)
+            torch.cuda.synchronize()
+            if r:
+                ts.append(e0.elapsed_time(e1) / 1e3)
+        best = max(best, 2 * half / float(np.median(ts)) / 1e9)
+    assert torch.equal(src[:1024], dst[:1024])
+    del src, dst
+    _CEILING[half] = best
+    return best
+
+
 def measured_traffic(config):
     """HBM bytes per launch of the workload's library kernel, from the committed
     rocprofv3 counter summary profiles/traffic_<config>.md (FETCH_SIZE +
@@ -503,6 +541,10 @@ def run_uniform(torch, pykblas, h, name, cfg, K, W, with_cpu, cpu_sample):
         roof["achieved_full_tile"] = bytes_full * bc * K / t / 1e9
         roof["frac_full_tile"] = roof["achieved_full_tile"] / HBM_PEAK_GBS
         roof["bytes_per_unit_full_tile"] = bytes_full
+    # what a streaming copy moving the same bytes per step reaches on this GPU
+    cc = copy_ceiling(torch, lib, (bytes_full or bytes_alg) * bc)
+    roof["copy_ceiling_GBs"] = cc
+    roof["frac_of_copy_ceiling"] = (roof.get("achieved_full_tile") or ach) / cc
     tr, src = measured_traffic(name)
     if tr is not None:
         roof["traffic"], roof["traffic_source"] = tr, src
@@ -541,7 +583,11 @@ def run_vbatch(torch, pykblas, h, name, cfg, K, W, with_cpu):
     info = torch.zeros(bc, dtype=torch.int32, device=DEV)
     iters = torch.zeros(bc, dtype=torch.int32, device=DEV)
     nrhs_arr = torch.full((bc,), nrhs, dtype=torch.int32, device=DEV)
-    lib.kblas_posv_vbatch_wsquery(h.h, nmax, bc)
+    mixed = cfg["op"] == "vsposv"
+    if mixed:   # the queue and the FP32 slots of the mixed phase for orders above 64
+        lib.kblas_sposv_vbatch_wsquery(h.h, nrhs, nmax, bc)
+    else:
+        lib.kblas_posv_vbatch_wsquery(h.h, nmax, bc)
     assert lib.kblasAllocateWorkspace(h.h) == pykblas.KBLAS_Success
     offA_d = torch.from_numpy(offA[:-1] * 8).to(DEV)
     offB_d = torch.from_numpy(offB[:-1] * 8).to(DEV)
@@ -549,7 +595,6 @@ def run_vbatch(torch, pykblas, h, name, cfg, K, W, with_cpu):
     Bw = B0.clone()
     pA = offA_d + Aw.data_ptr()
     pB = offB_d + Bw.data_ptr()
-    mixed = cfg["op"] == "vsposv"
     if mixed:
         Xw = torch.empty_like(B0)
         pX = offB_d + Xw.data_ptr()
@@ -574,6 +619,22 @@ def run_vbatch(torch, pykblas, h, name, cfg, K, W, with_cpu):
     log("%s: capturing and timing" % name)
     r = Stepper(torch, h, issue, 1).run(K, W, lambda i: 0, restore=restore)
     t = r["total_s"]
+    r64 = None
+    if mixed:
+        it_h = iters.cpu().numpy()
+        X_mixed = Xw.clone()
+        # the FP64 non-uniform POSV on the same inputs, for comparison
+        def issue64(s):
+            assert lib.kblasDposv_vbatch(h.h, b"R", b"L", nrhs_arr.data_ptr(), ns.data_ptr(), nrhs, nmax,
+                                         pA.data_ptr(), ns.data_ptr(), pB.data_ptr(), nrhs_arr.data_ptr(),
+                                         bc, info.data_ptr()) == pykblas.KBLAS_Success
+
+        def restore64(i):
+            Aw.copy_(A0)
+            Bw.copy_(B0)
+        log("%s: timing the FP64 POSV beside it" % name)
+        r64 = Stepper(torch, h, issue64, 1).run(K, W, lambda i: 0, restore=restore64)
+        X64 = Bw.clone()
     nn = ns_h.astype(np.float64)
     flops = float(np.sum(nn ** 3 / 3 + nn ** 2 / 2 + nn / 6 + 2 * nrhs * nn ** 2))
     if mixed:
@@ -595,11 +656,14 @@ def run_vbatch(torch, pykblas, h, name, cfg, K, W, with_cpu):
     out["roofline"] = roof
     out["info_nonzero"] = int((info != 0).sum().item())
     if mixed:
-        it_h = iters.cpu().numpy()
         out["iter_hist"] = hist_of(it_h)
         out["fallbacks"] = int((it_h < 0).sum())
+        t64 = r64["total_s"]
+        out["fp64_posv"] = {"value": bc * K / t64, "ms_per_step": 1e3 * t64 / K, "median_ms": r64["median_ms"],
+                            "min_ms": r64["min_ms"], "gflops": flops * K / t64 / 1e9}
+        out["mixed_over_fp64"] = t64 / t
         # accuracy per matrix on a sample against the FP64 oracle solution
-        X = Xw
+        X = X_mixed
         ss = min(cfg.get("acc_sample", 64), bc)
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle as po
@@ -616,7 +680,16 @@ def run_vbatch(torch, pykblas, h, name, cfg, K, W, with_cpu):
             _, Xo, _ = po.posv("R", Ai, Bi)
             fe.append(float(np.max(np.abs(Xi - Xo)) / np.max(np.abs(Xo))))
         out["berr_mixed"] = {"max": float(max(be)), "median": float(np.median(be)), "sample": ss}
-        out["fwd_err_vs_fp64_oracle"] = {"max": float(max(fe)), "median": float(np.median(fe)), "sample": ss}
+        out["fwd_err_vs_fp64_oracle"] = {"max": float(max(fe)), "median": float(np.median(fe)), "sample": ss,
+                                         "per_matrix": [float("%.3g" % v) for v in fe]}
+        X64h = X64[: int(offB[ss])].cpu().numpy()
+        be64 = []
+        for i in range(ss):
+            n = int(ns_h[i])
+            Ai = Ah[offA[i]:offA[i + 1]].reshape(n, n).T
+            Bi = Bh[offB[i]:offB[i + 1]].reshape(n, nrhs).T
+            be64.append(po.solve_berr("R", Ai, X64h[offB[i]:offB[i + 1]].reshape(n, nrhs).T, Bi))
+        out["berr_fp64"] = {"max": float(max(be64)), "median": float(np.median(be64)), "sample": ss}
     if with_cpu:
         cores = host_cores()
         count = min(cfg.get("cpu_count", 2000 if not mixed else 500), bc)

@@ -62,8 +62,10 @@ int kblasDposv_vbatch_noalloc(kblasHandle_t handle, char side, char uplo, int* m
  * Cholesky fallback when the FP32 pivot is not positive or refinement stalls.
  * A is read-only unless the fallback ran (then A holds L, as in dsposv).
  * B is read-only; X receives the solution (X may equal B with ldx == ldb).
- * iter[i] >= 0: refinement steps used; -3: FP32 factor failed; -31: no
- * convergence (both then solved in FP64).  info[i] as in dposv.
+ * iter[i] >= 0: refinement steps used; -2: an entry of A, B or a residual
+ * beyond FP32 range; -3: FP32 factor failed; -31: no convergence (all three
+ * then solved in FP64); -1: order above 64, solved in FP64 directly (dsposv's
+ * "full precision for implementation reasons").  info[i] as in dposv.
  * side / m / n exactly as kblasDposv_batch_strided.                           */
 int kblasDsposv_batch_strided(kblasHandle_t handle, char side, char uplo,
                               const int m, const int n,
@@ -76,9 +78,42 @@ int kblasDsposv_batch(kblasHandle_t handle, char side, char uplo,
                       double** A_array, int lda, const double** B_array, int ldb,
                       double** X_array, int ldx,
                       int batchCount, int* info_array, int* iter_array);
+/* Non-uniform: orders up to 64 refine in registers; orders above 64 refine
+ * in the workgroup phase (FP32 blocked factor and solves, FP64 residuals on
+ * the matrix cores) when the workspace was sized with
+ * kblas_sposv_vbatch_wsquery(handle, max_nrhs, max_order, batchCount) and
+ * max_m / max_n bound the sizes (orders up to 1024, 64 right-hand sides);
+ * otherwise, or beyond those bounds, they take dsposv's full-precision path
+ * (iter = -1).  kblas_posv_vbatch_wsquery's queue alone is the minimum.      */
+void kblas_sposv_vbatch_wsquery(kblasHandle_t handle, int max_nrhs, int max_order, int batchCount);
 int kblasDsposv_vbatch(kblasHandle_t handle, char side, char uplo, int* m, int* n,
                        int max_m, int max_n, double** A, int* lda, const double** B, int* ldb,
                        double** X, int* ldx, int batchCount, int* info_array, int* iter_array);
+
+/* ---- kernel choice per size bucket (no reference counterpart: it replaces
+ * the reference's hard-coded launch-shape tables, Xpotrf_batch_drivers.cuh:45-61,
+ * Xtrsm_batch_drivers.cuh:72-83, and its compile-time tuning array,
+ * src/kblas_struct.h:36-39) ---------------------------------------------
+ * A process-wide table of (precision 'd'/'s', operation, order range) ->
+ * kernel variant, written by kblas-gpu_amd/tune.py from measurements on the
+ * device.  It is loaded on first use from $KBLAS_TUNING_FILE, else from
+ * tuning.json next to the library; without an entry the built-in rule runs.
+ * Operations: "potrf", "posv", "potrs", "sposv" (uniform batches) and
+ * "vsplit" (non-uniform batches: variant = the order, 0..64, at and below
+ * which a matrix is served by one wavefront instead of a workgroup).
+ * Variants: "rg16" (n <= 16) and "rg32" (n <= 32): four matrices per
+ * wavefront in registers; "rg4": POTRF, 32 < n <= 64, one matrix per 16-lane
+ * row; "tile": n <= 64, one matrix per wavefront (matrix-core tiles);
+ * "reg": sposv, n <= 32; "blk": a 256-thread workgroup per matrix, any n.
+ * kblas_tuning_set appends an entry (later entries win) and rejects a variant
+ * that cannot serve every order of its range; kblas_tuning_query returns the
+ * variant index (0 = built-in rule; for "vsplit" the order, or -1).  Not
+ * thread-safe: change the table before concurrent calls.                    */
+int kblas_tuning_load(const char* path);
+int kblas_tuning_set(char prec, const char* op, int n_lo, int n_hi, const char* variant);
+void kblas_tuning_clear(void);
+int kblas_tuning_query(char prec, const char* op, int n);
+const char* kblas_tuning_variant_name(int variant);
 
 #ifdef __cplusplus
 }  /* extern "C" */

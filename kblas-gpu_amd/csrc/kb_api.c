@@ -60,6 +60,19 @@ static void zero_info(kblasHandle_t h, int* info, int bc)
     if (info && bc > 0) kb_launch_fill_int(info, 0, bc, h->stream);
 }
 
+/* Kernel for a uniform batch of order n: the tuning table's variant for this
+ * (precision, operation, order) when it has one, else the built-in rule
+ * (register-resident kernels up to KB_REG_MAX_N, the blocked workgroup
+ * kernel above). */
+static int chol_dispatch(kblasHandle_t h, char prec, int mode, int n, int nrhs, int side_left, kb_mat_t A,
+                         kb_mat_t B, int* info, int bc)
+{
+    const int v = kb_tune_variant(prec, mode, n);
+    if (v == KB_VAR_BLK || n > KB_REG_MAX_N)
+        return kb_launch_blk_chol(prec, mode, n, nrhs, side_left, A, B, info, bc, h->stream);
+    return kb_launch_reg_chol_v(prec, mode, n, nrhs, side_left, A, B, info, bc, v, h->stream);
+}
+
 /* ============================== POTRF ================================== */
 static int potrf_core(kblasHandle_t h, char prec, char uplo, int n, kb_mat_t A,
                       long strideA, int strided, int bc, int* info)
@@ -71,9 +84,7 @@ static int potrf_core(kblasHandle_t h, char prec, char uplo, int n, kb_mat_t A,
     if (bc == 0) return KBLAS_Success;
     if (strided ? (!A.base || strideA < (long)A.ld * n) : !A.ptrs) return KBLAS_Error_WrongInput;
     if (n == 0) { zero_info(h, info, bc); return KBLAS_Success; }
-    if (n <= KB_REG_MAX_N)
-        return launched(kb_launch_reg_chol(prec, KB_MODE_POTRF, n, 0, 0, A, A, info, bc, h->stream));
-    return launched(kb_launch_blk_chol(prec, KB_MODE_POTRF, n, 0, 0, A, A, info, bc, h->stream));
+    return launched(chol_dispatch(h, prec, KB_MODE_POTRF, n, 0, 0, A, A, info, bc));
 }
 
 int kblasDpotrf_batch(kblasHandle_t h, char uplo, const int n, double** A, int lda, int bc, int* info)
@@ -111,9 +122,7 @@ static int solve_core(kblasHandle_t h, char prec, int mode, char side, char uplo
     }
     if (order == 0) { if (mode == KB_MODE_POSV) zero_info(h, info, bc); return KBLAS_Success; }
     if (nrhs == 0 && mode == KB_MODE_POTRS) return KBLAS_Success;
-    if (order <= KB_REG_MAX_N)
-        return launched(kb_launch_reg_chol(prec, mode, order, nrhs, sd == 0, A, B, info, bc, h->stream));
-    return launched(kb_launch_blk_chol(prec, mode, order, nrhs, sd == 0, A, B, info, bc, h->stream));
+    return launched(chol_dispatch(h, prec, mode, order, nrhs, sd == 0, A, B, info, bc));
 }
 
 int kblasDpotrs_batch(kblasHandle_t h, char side, char uplo, const int m, const int n,
@@ -167,7 +176,8 @@ static int sposv_core(kblasHandle_t h, char side, char uplo, int m, int n, kb_ma
         if (order == 0 || nrhs == 0) return KBLAS_Success;
     }
     if (order <= KB_REG_MAX_N)
-        return launched(kb_launch_reg_sposv(order, nrhs, sd == 0, A, B, X, info, iter, bc, h->stream));
+        return launched(kb_launch_reg_sposv_v(order, nrhs, sd == 0, A, B, X, info, iter, bc,
+                                              kb_tune_variant('d', KB_TUNE_SPOSV, order), h->stream));
     return launched(kb_launch_blk_sposv(order, nrhs, sd == 0, A, B, X, info, iter, bc, h->stream));
 }
 
@@ -392,6 +402,16 @@ void kblas_potrf_vbatch_wsquery(kblasHandle_t h, int max_n, int batchCount)
 void kblas_posv_vbatch_wsquery(kblasHandle_t h, int max_n, int batchCount)
 { kblas_potrf_vbatch_wsquery(h, max_n, batchCount); }
 
+/* Order at and below which a non-uniform matrix takes the wavefront phase:
+ * the tuning table's "vsplit" entry for max_n (the mixed mode keeps
+ * KB_REG_MAX_N: above it the mixed path is dsposv's full-precision one). */
+static int vsplit(char prec, int mode, int max_n)
+{
+    if (prec == 'm' || mode == KB_MODE_SPOSV) return KB_REG_MAX_N;
+    const int v = kb_tune_variant(prec, KB_TUNE_VSPLIT, max_n > 0 ? max_n : 0);
+    return (v >= 0 && v <= KB_REG_MAX_N) ? v : KB_REG_MAX_N;
+}
+
 static int vchol(kblasHandle_t h, char prec, int mode, char side, char uplo, int* m, int* n, int max_n,
                  void* A, int* lda, void* B, int* ldb, void* X, int* ldx, int bc, int* info, int* iter)
 {
@@ -409,8 +429,8 @@ static int vchol(kblasHandle_t h, char prec, int mode, char side, char uplo, int
     kb_mat_t Am = mat_vptrs(A, lda);
     kb_mat_t Bm = B ? mat_vptrs(B, ldb) : Am;
     kb_mat_t Xm = X ? mat_vptrs(X, ldx) : Bm;
-    return launched(kb_launch_vchol(prec, mode, m, n, sd == 0, Am, Bm, Xm, info, iter, bc, max_n,
-                                    h->d_data, h->stream));
+    return launched(kb_launch_vchol_split(prec, mode, m, n, sd == 0, Am, Bm, Xm, info, iter, bc, max_n,
+                                          h->d_data, vsplit(prec, mode, max_n), h->stream));
 }
 
 int kblasDpotrf_vbatch(kblasHandle_t h, char uplo, int* n, int max_n, double** A, int* lda, int bc, int* info)
@@ -426,10 +446,36 @@ int kblasDposv_vbatch(kblasHandle_t h, char side, char uplo, int* m, int* n, int
 int kblasSposv_vbatch(kblasHandle_t h, char side, char uplo, int* m, int* n, int max_m, int max_n,
                       float** A, int* lda, float** B, int* ldb, int bc, int* info)
 { (void)max_m; return vchol(h, 's', KB_MODE_POSV, side, uplo, m, n, max_n, A, lda, B, ldb, NULL, NULL, bc, info, NULL); }
+/* Mixed non-uniform POSV: the queue is required; when the workspace also
+ * holds the FP32 slots (kblas_sposv_vbatch_wsquery) the orders above 64 run
+ * the mixed method, else dsposv's full-precision path (ITER = -1). */
+void kblas_sposv_vbatch_wsquery(kblasHandle_t h, int max_nrhs, int max_order, int batchCount)
+{
+    if (!h) return;
+    kb_ws_state need;
+    memset(&need, 0, sizeof need);
+    need.d_data = kb_vsposv_ws_bytes(batchCount > 0 ? batchCount : 0, max_order, max_nrhs);
+    kb_ws_request(h, &need);
+}
+
 int kblasDsposv_vbatch(kblasHandle_t h, char side, char uplo, int* m, int* n, int max_m, int max_n,
                        double** A, int* lda, const double** B, int* ldb, double** X, int* ldx, int bc,
                        int* info, int* iter)
-{ (void)max_m; return vchol(h, 'm', KB_MODE_SPOSV, side, uplo, m, n, max_n, A, lda, (void*)B, ldb, X, ldx, bc, info, iter); }
+{
+    if (!h) return KBLAS_Error_NotInitialized;
+    int u = upl(uplo);
+    int sd = sideof(side);
+    if (u < 0 || sd < 0 || bc < 0 || !n || !m || !A || !lda || !B || !ldb || !X || !ldx) return KBLAS_Error_WrongInput;
+    if (u == 1) return KBLAS_NotImplemented;
+    if (bc == 0) return KBLAS_Success;
+    kb_ws_state need;
+    kb_ws_need_vbatch(&need, bc);
+    if (!kb_ws_sufficient(&need, &h->allocated)) return KBLAS_InsufficientWorkspace;
+    const int max_order = sd == 0 ? max_m : max_n, max_nrhs = sd == 0 ? max_n : max_m;
+    return launched(kb_launch_vsposv(m, n, sd == 0, mat_vptrs(A, lda), mat_vptrs((void*)B, ldb), mat_vptrs(X, ldx),
+                                     info, iter, bc, h->d_data, h->allocated.d_data, max_order, max_nrhs,
+                                     h->stream));
+}
 
 /* kblasDposv_vbatch with the queue in the library's fixed per-device scratch
  * area instead of the handle's workspace: no allocation, no workspace query.
@@ -454,8 +500,8 @@ int kblasDposv_vbatch_noalloc(kblasHandle_t h, char side, char uplo, int* m, int
     if (!ws) return kb_hip_status(err ? err : 1);
     kb_mat_t Am = mat_vptrs(A, lda);
     kb_mat_t Bm = mat_vptrs(B, ldb);
-    return launched(kb_launch_vchol('d', KB_MODE_POSV, m, n, sd == 0, Am, Bm, Bm, info, NULL, bc, max_n, ws,
-                                    h->stream));
+    return launched(kb_launch_vchol_split('d', KB_MODE_POSV, m, n, sd == 0, Am, Bm, Bm, info, NULL, bc, max_n, ws,
+                                          vsplit('d', KB_MODE_POSV, max_n), h->stream));
 }
 
 /* ============ inverse family: TRTRI / LAUUM / POTRI / POTI ============== */
@@ -478,9 +524,7 @@ static int inv_core(kblasHandle_t h, char prec, int what, char uplo, char diag, 
     if (n == 0) { zero_info(h, info, bc); return KBLAS_Success; }
     if (n > KB_INV_MAX_N) return KBLAS_NotImplemented;
     if (what == INV_POTI) {
-        int st = n <= KB_REG_MAX_N
-                     ? kb_launch_reg_chol(prec, KB_MODE_POTRF, n, 0, 0, A, A, info, bc, h->stream)
-                     : kb_launch_blk_chol(prec, KB_MODE_POTRF, n, 0, 0, A, A, info, bc, h->stream);
+        int st = chol_dispatch(h, prec, KB_MODE_POTRF, n, 0, 0, A, A, info, bc);
         if (st) return launched(st);
         return launched(kb_launch_inv(prec, KB_INV_OP_POTRI, 0, n, A, info, info, bc, h->stream));
     }

@@ -27,6 +27,7 @@ int kblasCreate(kblasHandle_t* handle)
     /* this device's copy of the fixed scratch area of kblasDposv_vbatch_noalloc;
      * a failed lookup is reported by that entry point, which retries it */
     (void)kb_noalloc_init();
+    kb_tune_init();
     *handle = h;
     return KBLAS_Success;
 }

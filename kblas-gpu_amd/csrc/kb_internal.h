@@ -37,6 +37,13 @@ void kb_ws_need_vbatch(kb_ws_state* ws, int batchCount);
 int  kb_ws_sufficient(const kb_ws_state* need, const kb_ws_state* have);
 void kb_ws_request(kblasHandle_t h, const kb_ws_state* need);
 
+/* per-size-bucket kernel choice (kb_tune.c); operation ids follow KB_MODE_* */
+enum { KB_TUNE_POTRF = 0, KB_TUNE_POSV = 1, KB_TUNE_POTRS = 2, KB_TUNE_SPOSV = 3, KB_TUNE_VSPLIT = 4 };
+void kb_tune_init(void);
+int  kb_tune_variant(char prec, int op, int n);
+int  kb_tune_legal(int op, int n, int variant);
+int  kb_tune_op_id(const char* op);
+
 /* status helpers */
 int kb_hip_status(int hip_err);
 

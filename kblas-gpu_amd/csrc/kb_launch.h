@@ -34,10 +34,22 @@ enum { KB_MODE_POTRF = 0, KB_MODE_POSV = 1, KB_MODE_POTRS = 2 };
 int kb_launch_reg_chol(char prec, int mode, int n, int nrhs, int side_left,
                        kb_mat_t A, kb_mat_t B, int* info, int batch, hipStream_t s);
 
+/* Kernel variants of the per-size-bucket choice (kb_tune.c). */
+enum { KB_VAR_AUTO = 0, KB_VAR_RG16, KB_VAR_RG32, KB_VAR_RG4, KB_VAR_TILE, KB_VAR_REG, KB_VAR_BLK };
+
+/* The same with an explicit kernel variant (KB_VAR_AUTO: the built-in rule;
+ * rg16 n <= 16, rg32 n <= 32, rg4 POTRF 32 < n <= 64, tile n <= 64);
+ * hipErrorInvalidValue for a variant that cannot serve n. */
+int kb_launch_reg_chol_v(char prec, int mode, int n, int nrhs, int side_left,
+                         kb_mat_t A, kb_mat_t B, int* info, int batch, int variant, hipStream_t s);
+
 /* Mixed precision POSV (FP32 factor + FP64 refinement + FP64 fallback), n <= 64.
  * A read-only unless the fallback ran (then it holds L, as LAPACK dsposv).   */
 int kb_launch_reg_sposv(int n, int nrhs, int side_left, kb_mat_t A, kb_mat_t B, kb_mat_t X,
                         int* info, int* iter, int batch, hipStream_t s);
+/* variant: 0 built-in, KB_VAR_REG (n <= 32), KB_VAR_TILE (n <= 64) */
+int kb_launch_reg_sposv_v(int n, int nrhs, int side_left, kb_mat_t A, kb_mat_t B, kb_mat_t X,
+                          int* info, int* iter, int batch, int variant, hipStream_t s);
 
 /* Blocked one-workgroup-per-matrix Cholesky family for any n (used for
  * n > KB_REG_MAX_N); same arguments as kb_launch_reg_chol. */
@@ -79,6 +91,23 @@ void* kb_noalloc_scratch(int* err);
 int kb_launch_vchol(char prec, int mode, const int* vm, const int* vn, int side_left,
                     kb_mat_t A, kb_mat_t B, kb_mat_t X, int* info, int* iter, int batch,
                     int max_n, void* ws, hipStream_t s);
+/* The same with the order at and below which a matrix goes to the wavefront
+ * phase (0..KB_REG_MAX_N; orders above it go to the workgroup phase).  The
+ * mixed mode always splits at KB_REG_MAX_N. */
+int kb_launch_vchol_split(char prec, int mode, const int* vm, const int* vn, int side_left,
+                          kb_mat_t A, kb_mat_t B, kb_mat_t X, int* info, int* iter, int batch,
+                          int max_n, void* ws, int split, hipStream_t s);
+
+/* Non-uniform mixed POSV.  ws holds ws_bytes bytes: the queue, then (when
+ * ws_bytes >= kb_vsposv_ws_bytes(batch, max_order, max_nrhs)) one FP32 slot
+ * per workgroup of the phase for orders above KB_REG_MAX_N, which then runs
+ * the mixed method; without the slots (or for orders / right-hand-side counts
+ * above max_order / max_nrhs, 1024 / 64 at most) those orders take dsposv's
+ * full-precision path (ITER = -1). */
+size_t kb_vsposv_ws_bytes(int batch, int max_order, int max_nrhs);
+int kb_launch_vsposv(const int* vm, const int* vn, int side_left, kb_mat_t A, kb_mat_t B, kb_mat_t X,
+                     int* info, int* iter, int batch, void* ws, size_t ws_bytes, int max_order, int max_nrhs,
+                     hipStream_t s);
 
 /* Batched inverse family on the lower triangle, one workgroup per matrix,
  * n <= KB_INV_MAX_N (hipErrorNotSupported above).  op: KB_INV_OP_TRTRI
@@ -92,8 +121,9 @@ enum { KB_INV_OP_TRTRI = 0, KB_INV_OP_LAUUM = 1, KB_INV_OP_POTRI = 2 };
 int kb_launch_inv(char prec, int op, int unit, int n, kb_mat_t A, int* info, const int* skip, int batch,
                   hipStream_t s);
 /* Batched lower TRMM: B := alpha op(L) B (side_left, L of order m) or
- * B := alpha B op(L) (L of order n); B is m x n.  Orders up to 8192 (FP32) /
- * 4096 (FP64) per staged line of B (hipErrorNotSupported above). */
+ * B := alpha B op(L) (L of order n); B is m x n.  One line of B (a column
+ * for side_left, else a row) is staged in 64 KB of LDS, so the order is at
+ * most 16384 (FP32) / 8192 (FP64) (hipErrorNotSupported above). */
 int kb_launch_trmm(char prec, int side_left, int trans, int unit, int m, int n, double alpha,
                    kb_mat_t A, kb_mat_t B, int batch, hipStream_t s);
 

@@ -18,6 +18,7 @@
 #pragma once
 #include "kb_blk_impl.h"
 #include "kb_reg_impl.h"
+#include "kb_sposv_big.h"
 
 constexpr int KB_VQ_BINS = 4096;   // orders >= 4095 share the last bin
 constexpr int KB_VQ_HDR = 2 * KB_VQ_BINS + 64;   // ints: hist, offsets, counters
@@ -72,8 +73,9 @@ __global__ void k_vq_hist(void* ws, const int* vm, const int* vn, int side_left,
         atomicAdd(&q.hist[vq_bin(vq_order(vm, vn, side_left, i))], 1);
 }
 
-// exclusive scan in descending order of n: offs[b] = sum_{b' > b} hist[b']
-__global__ void __launch_bounds__(1024) k_vq_scan(void* ws, int batch)
+// exclusive scan in descending order of n: offs[b] = sum_{b' > b} hist[b'];
+// counter[1] = counter[2] = the number of orders above split
+__global__ void __launch_bounds__(1024) k_vq_scan(void* ws, int batch, int split)
 {
     VQueue q = vq_view(ws, batch);
     __shared__ int part[1024];
@@ -94,7 +96,7 @@ __global__ void __launch_bounds__(1024) k_vq_scan(void* ws, int batch)
     for (int j = 0; j < PER; ++j) {
         const int b = KB_VQ_BINS - 1 - (t * PER + j);
         q.offs[b] = run;
-        if (b == KB_REG_MAX_N) {   // run = number of orders above the register range
+        if (b == split) {   // run = number of orders above the split
             q.counter[1] = run;
             q.counter[2] = run;
         }
@@ -232,12 +234,12 @@ static int vchol_grid(K kernel, int batch)
 
 template <typename T, int MODE>
 static int launch_vchol(const int* vm, const int* vn, int side_left, kb_mat_t A, kb_mat_t B, kb_mat_t X,
-                        int* info, int* iter, int batch, void* ws, hipStream_t s)
+                        int* info, int* iter, int batch, void* ws, int split, hipStream_t s)
 {
     hipLaunchKernelGGL(k_vq_clear, dim3(8), dim3(1024), 0, s, ws);
     const int gb = (batch + 255) / 256 < 1024 ? (batch + 255) / 256 : 1024;
     hipLaunchKernelGGL(k_vq_hist, dim3(gb), dim3(256), 0, s, ws, vm, vn, side_left, batch);
-    hipLaunchKernelGGL(k_vq_scan, dim3(1), dim3(1024), 0, s, ws, batch);
+    hipLaunchKernelGGL(k_vq_scan, dim3(1), dim3(1024), 0, s, ws, batch, split);
     hipLaunchKernelGGL(k_vq_scatter, dim3(gb), dim3(256), 0, s, ws, vm, vn, side_left, batch);
     const BatchArg<T> a = to_arg<T>(A), b = to_arg<T>(B), x = to_arg<T>(X);
     hipLaunchKernelGGL((k_vchol_blk<T, MODE>), dim3(vchol_grid(k_vchol_blk<T, MODE>, batch)), dim3(KB_BLK_WG), 0, s,
@@ -248,10 +250,11 @@ static int launch_vchol(const int* vm, const int* vn, int side_left, kb_mat_t A,
 }
 
 // ---------------------------------------------------------------------------
-// Mixed precision (SPOSV).  Orders above the register range take LAPACK
-// dsposv's full-precision path (ITER = -1): X := B (k_vsposv_big_prep), then
-// the FP64 POSV workgroup kernel above on (A, X).  The rest of the queue runs
-// the mixed register routine in k_vchol_mixed: a persistent kernel of
+// Mixed precision (SPOSV).  Orders above the register range run the mixed
+// workgroup routine of kb_sposv_big.h in k_vsposv_big (FP32 factor and solves
+// in a workspace slot per workgroup, FP64 residuals on the matrix cores;
+// without a slot, dsposv's full-precision path, ITER = -1).  The rest of the
+// queue runs the mixed register routine in k_vchol_mixed: a persistent kernel of
 // 256-thread workgroups that can also take the large orders itself (workgroup
 // phase first; once the queue reaches the register range every wavefront
 // takes its own tickets); with small_only set it skips that phase and its
@@ -259,25 +262,55 @@ static int launch_vchol(const int* vm, const int* vn, int side_left, kb_mat_t A,
 // mode into k_vchol_blk<SPOSV> + k_vchol_small<SPOSV> hung on the GPU, cause
 // not found, so neither is built for it.)
 // ---------------------------------------------------------------------------
-// X := B and ITER := -1 for the orders above the register range.
-template <typename T>
-__global__ void __launch_bounds__(KB_BLK_WG)
-k_vsposv_big_prep(const int* vm, const int* vn, int side_left, BatchArg<T> B, BatchArg<T> X, int* iter,
-                  int batch, void* ws)
+// The workgroup phase of the non-uniform mixed POSV: tickets [0, counter[1])
+// of the size-sorted queue (the orders above the register range), one matrix
+// per workgroup; workgroup b owns slot b of the FP32 workspace (slots == null:
+// no workspace for the mixed path, every order here takes the FP64 path).
+namespace {
+__global__ void __launch_bounds__(KB_BLK_WG) __attribute__((amdgpu_waves_per_eu(2)))
+k_vsposv_big(const int* vm, const int* vn, int side_left, BatchArg<double> A, BatchArg<double> B,
+             BatchArg<double> X, int* info, int* iter, int batch, void* ws, float* slots, long slot_floats,
+             int max_n, int max_nrhs)
 {
+    __shared__ SbSmem S;
+    __shared__ int s_ticket;
     VQueue q = vq_view(ws, batch);
     const int nbig = q.counter[1];
-    for (int t = blockIdx.x; t < nbig; t += gridDim.x) {
-        const long mat = q.order[t];
+    for (;;) {
+        if (threadIdx.x == 0) s_ticket = atomicAdd(q.counter, 1);
+        __syncthreads();
+        const int ticket = s_ticket;
+        __syncthreads();
+        if (ticket >= nbig) return;
+        const long mat = q.order[ticket];
         const int n = vq_order(vm, vn, side_left, mat);
         const int nrhs = vq_nrhs(vm, vn, side_left, mat);
-        const T* Bp = B.get(mat);
-        T* Xp = X.get(mat);
-        const int ldb = B.ldim(mat), ldx = X.ldim(mat);
-        copy_rhs<T>(Bp, ldb, Xp, ldx, side_left, n, nrhs, threadIdx.x, KB_BLK_WG);
-        if (threadIdx.x == 0 && iter) iter[mat] = -1;
+        double* Ap = A.get(mat);
+        const int lda = A.ldim(mat);
+        const double* Bp = B.get(mat);
+        const int ldb = B.ldim(mat);
+        double* Xp = X.get(mat);
+        const int ldx = X.ldim(mat);
+        if (nrhs <= 0) {
+            // nothing to solve (dsposv's quick return)
+            if (threadIdx.x == 0) { if (info) info[mat] = 0; if (iter) iter[mat] = 0; }
+        } else if (slots && n <= max_n && nrhs <= max_nrhs && n <= KB_SB_MAX_N && nrhs <= KB_SB_MAX_RHS) {
+            float* A32 = slots + (long)blockIdx.x * slot_floats;
+            float* Z = A32 + (long)max_n * max_n;
+            sb_sposv_one(mat, n, nrhs, side_left, Ap, lda, Bp, ldb, Xp, ldx, A32, Z, info, iter, S);
+        } else {
+            // dsposv's full-precision path (ITER = -1); A receives L
+            copy_rhs<double>(Bp, ldb, Xp, ldx, side_left, n, nrhs, threadIdx.x, KB_BLK_WG);
+            __syncthreads();
+            const int inf = blk_potrf<double>(Ap, lda, n, S.d, Xp, ldx, side_left, nrhs);
+            if (threadIdx.x == 0) { if (info) info[mat] = inf; if (iter) iter[mat] = -1; }
+            if (inf == 0) blk_solve_bwd<double>(Ap, lda, n, Xp, ldx, side_left, nrhs, S.d);
+        }
+        __syncthreads();
     }
 }
+}  // namespace
+
 // Shared memory of the persistent kernel: the blocked workgroup routine or,
 // in the register phase, one float LDS area per wavefront (only the FP32 and
 // mixed routines use it; the FP64 ones read their transposed factor back from
@@ -363,21 +396,25 @@ k_vchol_mixed(const int* vm, const int* vn, int side_left, BatchArg<T> A, BatchA
     }
 }
 
+// Persistent grid of k_vsposv_big (also the number of FP32 workspace slots).
+static int vsposv_grid(int batch) { return vchol_grid(k_vsposv_big, batch); }
+
 template <typename T, int MODE>
 static int launch_vchol_mixed(const int* vm, const int* vn, int side_left, kb_mat_t A, kb_mat_t B, kb_mat_t X,
-                        int* info, int* iter, int batch, void* ws, hipStream_t s)
+                        int* info, int* iter, int batch, void* ws, float* slots, long slot_floats, int max_n,
+                        int max_nrhs, hipStream_t s)
 {
     hipLaunchKernelGGL(k_vq_clear, dim3(8), dim3(1024), 0, s, ws);
     const int gb = (batch + 255) / 256 < 1024 ? (batch + 255) / 256 : 1024;
     hipLaunchKernelGGL(k_vq_hist, dim3(gb), dim3(256), 0, s, ws, vm, vn, side_left, batch);
-    hipLaunchKernelGGL(k_vq_scan, dim3(1), dim3(1024), 0, s, ws, batch);
+    hipLaunchKernelGGL(k_vq_scan, dim3(1), dim3(1024), 0, s, ws, batch, KB_REG_MAX_N);
     hipLaunchKernelGGL(k_vq_scatter, dim3(gb), dim3(256), 0, s, ws, vm, vn, side_left, batch);
     const BatchArg<T> a = to_arg<T>(A), b = to_arg<T>(B), x = to_arg<T>(X);
-    // orders above the register range: X := B, then the FP64 POSV workgroup kernel on (A, X)
-    hipLaunchKernelGGL((k_vsposv_big_prep<T>), dim3(vchol_grid(k_vsposv_big_prep<T>, batch)), dim3(KB_BLK_WG), 0, s,
-                       vm, vn, side_left, b, x, iter, batch, ws);
-    hipLaunchKernelGGL((k_vchol_blk<T, KB_MODE_POSV>), dim3(vchol_grid(k_vchol_blk<T, KB_MODE_POSV>, batch)),
-                       dim3(KB_BLK_WG), 0, s, vm, vn, side_left, a, x, x, info, iter, batch, ws);
+    // orders above the register range: the mixed workgroup kernel (FP32 slots
+    // in the workspace after the queue), FP64 for what does not fit
+    const int gbig = vsposv_grid(batch);
+    hipLaunchKernelGGL(k_vsposv_big, dim3(gbig), dim3(KB_BLK_WG), 0, s, vm, vn, side_left, a, b, x, info, iter,
+                       batch, ws, slots, slot_floats, max_n, max_nrhs);
     // the rest: the mixed register routine, one matrix per wavefront
     hipLaunchKernelGGL((k_vchol_mixed<T, MODE>), dim3(vchol_grid(k_vchol_mixed<T, MODE>, batch)), dim3(KB_BLK_WG), 0,
                        s, vm, vn, side_left, a, b, x, info, iter, batch, ws, 1);

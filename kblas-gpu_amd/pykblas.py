@@ -58,6 +58,7 @@ _SIG = {
     "kblas_gemm_batch_nonuniform_wsquery": (None, [P]),
     "kblas_potrf_vbatch_wsquery": (None, [P, c_int, c_int]),
     "kblas_posv_vbatch_wsquery": (None, [P, c_int, c_int]),
+    "kblas_sposv_vbatch_wsquery": (None, [P, c_int, c_int, c_int]),
     "kblas_trmm_batch_wsquery": (None, [P, c_char, c_int, c_int, c_int]),
     "kblas_trmm_batch_strided_wsquery": (None, [P, c_char, c_int, c_int, c_int]),
     "kblas_trtri_batch_wsquery": (None, [P, c_int, c_int]),
@@ -68,6 +69,12 @@ _SIG = {
     "kblas_potri_batch_strided_wsquery": (None, [P, c_int, c_int]),
     "kblas_poti_batch_wsquery": (None, [P, c_int, c_int]),
     "kblas_poti_batch_strided_wsquery": (None, [P, c_int, c_int]),
+    # kernel choice per size bucket
+    "kblas_tuning_load": (c_int, [ctypes.c_char_p]),
+    "kblas_tuning_set": (c_int, [c_char, ctypes.c_char_p, c_int, c_int, ctypes.c_char_p]),
+    "kblas_tuning_clear": (None, []),
+    "kblas_tuning_query": (c_int, [c_char, ctypes.c_char_p, c_int]),
+    "kblas_tuning_variant_name": (ctypes.c_char_p, [c_int]),
 }
 _TSIG = {
     "potrf_batch": (c_int, [P, c_char, c_int, P, c_int, c_int, P]),

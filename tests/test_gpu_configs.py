@@ -54,7 +54,9 @@ def test_dsposv_vbatch_large_orders_64_rhs(h):
     dA, dB = dev_list(As), dev_list(Bs)
     torch = kt.torch_dev()
     dX = [torch.zeros_like(t) for t in dB]
-    alloc_ws(h, int(ns.max()), batch)
+    # workspace with the FP32 slots: the orders above 64 refine too
+    h.lib.kblas_sposv_vbatch_wsquery(h.h, nrhs, int(ns.max()), batch)
+    assert h.lib.kblasAllocateWorkspace(h.h) == pykblas.KBLAS_Success
     dn = kt.to_dev(ns)
     dr = kt.to_dev(np.full(batch, nrhs, np.int32))
     info = kt.to_dev(np.full(batch, -7, np.int32))
@@ -73,8 +75,12 @@ def test_dsposv_vbatch_large_orders_64_rhs(h):
         _, Xo, _ = po.posv("R", As[i], Bs[i])
         fe = np.max(np.abs(X - Xo)) / np.max(np.abs(Xo))
         assert fe <= kap[i] * 1e-13, (i, n, fe)
-        assert itr[i] >= -1, (i, n, itr[i])          # no fallback at cond <= 1e4
+        # cond <= 1e4: refinement serves every order, within 2 of LAPACK dsposv's count
+        _, _, it_r = po.sposv("R", As[i], Bs[i])
+        assert itr[i] >= 0 and it_r >= 0, (i, n, itr[i], it_r)
+        assert abs(int(itr[i]) - it_r) <= 2, (i, n, itr[i], it_r)
         assert np.array_equal(kt.to_host(dB[i]).T, Bs[i])    # B read-only
+        assert np.array_equal(kt.to_host(dA[i]).T, As[i])    # A read-only when refinement served
 
 
 def test_dposv_vbatch_whole_range(h):
@@ -194,3 +200,59 @@ def test_graph_replay_mixed_posv(h):
         assert np.array_equal(kt.to_host(info), refI), rep
         assert np.array_equal(kt.to_host(it), refIt), rep
     assert np.array_equal(kt.to_host(dA), A)                                 # A read-only
+
+
+def test_dsposv_vbatch_large_orders_fallbacks(h):
+    """orders above 64 on the mixed path: an FP32 factorization that fails
+    (cond 1e12) gives ITER = -3, a right-hand side beyond FP32 range -2, both
+    solved in FP64 with A := L; the others refine; a workspace without the
+    FP32 slots sends every order above 64 to the FP64 path (ITER = -1)"""
+    ns = np.array([100, 130, 90, 70, 40], np.int32)
+    nrhs = 8
+    batch = len(ns)
+    As, Bs = [], []
+    for i, n in enumerate(ns):
+        lk = 12.0 if i == 0 else 3.0
+        A, _ = po.gen_spd_cond(707, i, int(n), lk, lk)
+        As.append(A)
+        B = po.gen_rhs(708, i, int(n), nrhs)                  # side L: n x nrhs
+        if i == 1:
+            B[:, 3] *= 1e39
+        Bs.append(B)
+    torch = kt.torch_dev()
+    dn = kt.to_dev(ns)
+    dr = kt.to_dev(np.full(batch, nrhs, np.int32))
+    for slots in (True, False):
+        dA, dB = dev_list(As), dev_list(Bs)
+        dX = [torch.zeros_like(t) for t in dB]
+        hh = h if slots else kt.handle()        # a fresh handle: only the queue
+        if slots:
+            hh.lib.kblas_sposv_vbatch_wsquery(hh.h, nrhs, int(ns.max()), batch)
+        else:
+            hh.lib.kblas_posv_vbatch_wsquery(hh.h, int(ns.max()), batch)
+        assert hh.lib.kblasAllocateWorkspace(hh.h) == pykblas.KBLAS_Success
+        info = kt.to_dev(np.full(batch, -7, np.int32))
+        it = kt.to_dev(np.full(batch, -99, np.int32))
+        pA, pB, pX = ptrs(dA), ptrs(dB), ptrs(dX)
+        st = hh.lib.kblasDsposv_vbatch(hh.h, b"L", b"L", kt.P(dn), kt.P(dr), int(ns.max()), nrhs, kt.P(pA),
+                                       kt.P(dn), kt.P(pB), kt.P(dn), kt.P(pX), kt.P(dn), batch, kt.P(info), kt.P(it))
+        assert st == pykblas.KBLAS_Success
+        inf, itr = kt.to_host(info), kt.to_host(it)
+        if not slots:
+            hh.close()
+        assert np.all(inf == 0), inf
+        want = [-3, -2, None, None, None] if slots else [-1, -1, -1, -1, None]
+        for i in range(batch):
+            n = int(ns[i])
+            X = kt.to_host(dX[i]).T
+            assert po.solve_berr("L", As[i], X, Bs[i]) <= 1e-13, (slots, i, itr[i])
+            if want[i] is None:
+                assert itr[i] >= 0, (slots, i, itr[i])
+                assert np.array_equal(kt.to_host(dA[i]).T, As[i])
+            else:
+                assert itr[i] == want[i], (slots, i, itr[i])
+                assert po.potrf_berr(As[i], kt.to_host(dA[i]).T) <= 4 * n * EPS   # A holds L
+        if slots:
+            for i in (0, 1):
+                _, _, it_r = po.sposv("L", As[i], Bs[i])
+                assert it_r == want[i], (i, it_r)

@@ -207,6 +207,27 @@ def test_dtrmm_strided(h, side, trans, diag, mn):
 
 
 @pytest.mark.parametrize("side", ["L", "R"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+def test_dtrmm_panel_loop(h, side, trans):
+    """an order too large for the triangle to be staged in LDS, so B goes
+    through LDS in panels of whole lines (1100 x 40 / 40 x 1100: six
+    seven-line panels), for both transposes"""
+    m, n = (1100, 40) if side == "L" else (40, 1100)
+    batch = 2
+    order = m if side == "L" else n
+    _, L = factors(order, batch, 77)
+    B = RNG.standard_normal((batch, n, m))
+    dL, dB = kt.to_dev(L), kt.to_dev(B)
+    st = h.lib.kblasDtrmm_batch_strided(h.h, side.encode(), b"L", trans.encode(), b"N", m, n, 1.5,
+                                        kt.P(dL), order, order * order, kt.P(dB), m, m * n, batch)
+    assert st == pykblas.KBLAS_Success
+    X = kt.to_host(dB)
+    for i in range(batch):
+        Xo = po.trmm(side, trans, "N", 1.5, po.col(L, i), po.col(B, i))
+        assert rel(po.col(X, i), Xo) <= 16 * order * EPS
+
+
+@pytest.mark.parametrize("side", ["L", "R"])
 def test_strmm_pointer_array_and_alpha_zero(h, side):
     m, n, batch = 20, 12, 4
     order = m if side == "L" else n
@@ -259,7 +280,9 @@ def pstrf_inputs(n, r, batch, seed):
 
 @pytest.mark.parametrize("n,r", [(1, 1), (5, 5), (16, 16), (31, 9), (32, 32), (33, 33), (48, 20), (64, 64), (64, 30)])
 def test_dpstrf_strided_matches_oracle(h, n, r):
-    batch = 6
+    # n <= 32 pairs two matrices per wave: an odd batch leaves a live matrix
+    # beside an idle half in the last wave
+    batch = 7 if n <= 32 else 6
     torch = kt.torch_dev()
     A = pstrf_inputs(n, r, batch, 100 + n + r)
     dA = kt.to_dev(A)
