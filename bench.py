@@ -21,12 +21,13 @@ on a side stream) and replayed; inputs are generated on the GPU with torch
 W+2K copies of their inputs; c3/c5 restore theirs with a device copy before
 each replay.  Each timed replay is queued behind a short spin kernel and
 bracketed by HIP events on the replay stream, so the events hold the
-library's device time and not the host's graph-launch latency: ms_per_step is
-the median and minimum per step.  c1/c2/c4 also replay K further steps back
-to back without the spin kernels (fresh input slots); their span / K is
-ms_per_step there (whole-job throughput of a stream of calls), the gated
-mean is reported beside it as gated_mean_ms; c3/c5 use the gated mean.
-roofline.achieved = algorithmic bytes (or flops) per step / step time.
+library's device time and not the host's graph-launch latency (median_ms,
+min_ms, gated_mean_ms).  c1/c2/c4 also capture K further steps back to back
+in ONE graph (fresh input slots) and replay it once behind a spin kernel; its
+span / K is ms_per_step there (whole-job throughput of a stream of calls,
+without the per-replay launch gap of a one-call graph, about 8 us here);
+c3/c5 use the gated mean.  roofline.achieved = algorithmic bytes (or flops)
+per step / ms_per_step.
 
 cpu_baseline is the reference test's CPU comparator: looped LAPACK (MKL,
 sequential per thread) dpotrf + dtrsm(R,L,T) + dtrsm(R,L,N), as in
@@ -244,9 +245,12 @@ class Stepper:
     """Captures issue(slot) once per input slot as a HIP graph on a side stream
     (the library handle follows the capturing stream) and replays it."""
 
-    def __init__(self, torch, h, issue, slots):
+    def __init__(self, torch, h, issue, slots, chain=None):
+        """slots: one graph per input slot 0..slots-1; chain: further slots
+        captured back to back into ONE graph (a stream of calls)."""
         self.torch = torch
         self.graphs = []
+        self.chain = None
         self.stream = torch.cuda.Stream()
         torch.cuda.synchronize()
         for s in range(slots):
@@ -255,6 +259,13 @@ class Stepper:
                 h.set_stream(torch.cuda.current_stream().cuda_stream)
                 issue(s)
             self.graphs.append(gr)
+        if chain:
+            gr = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(gr, stream=self.stream):
+                h.set_stream(torch.cuda.current_stream().cuda_stream)
+                for s in chain:
+                    issue(s)
+            self.chain = (gr, len(chain))
         h.set_stream(torch.cuda.current_stream().cuda_stream)
         torch.cuda.synchronize()
 
@@ -263,9 +274,11 @@ class Stepper:
         behind a short spin kernel (torch.cuda._sleep) so the host has queued
         the start event, the graph and the end event before the GPU reaches
         them: the events then bracket the library's own device time, not the
-        host's graph-launch latency.  A second pass replays the K steps back
-        to back without the spin kernels (inputs of slots W+K .. W+2K-1; only
-        when no restore step is needed); its span is reported beside."""
+        host's graph-launch latency.  Where the stepper holds a chain (K
+        further steps captured back to back in one graph, on fresh input
+        slots), it is replayed once behind a spin kernel and its span / K is
+        reported beside: the device time per call of a stream of calls,
+        without a graph launch per call."""
         torch = self.torch
         st = self.stream
         ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(K)]
@@ -290,14 +303,15 @@ class Stepper:
         wall = time.perf_counter() - t0
         per = np.array([a.elapsed_time(b) for a, b in ev]) / 1000.0
         span = None
-        if restore is None:
+        if self.chain is not None:
+            gr, kc = self.chain
             with torch.cuda.stream(st):
+                torch.cuda._sleep(GATE_CYCLES)
                 e_all0.record(st)
-                for i in range(K):
-                    self.graphs[slot_of(W + K + i)].replay()
+                gr.replay()
                 e_all1.record(st)
             torch.cuda.synchronize()
-            span = e_all0.elapsed_time(e_all1) / 1000.0
+            span = e_all0.elapsed_time(e_all1) / 1000.0 / kc * K
         return dict(total_s=float(per.sum()), wall_s=wall,
                     back_to_back_ms_per_step=None if span is None else 1e3 * span / K,
                     median_ms=float(np.median(per) * 1e3), min_ms=float(per.min() * 1e3),
@@ -467,6 +481,7 @@ def run_uniform(torch, pykblas, h, name, cfg, K, W, with_cpu, cpu_sample):
     info = torch.zeros(bc, dtype=torch.int32, device=DEV)
     iters = torch.zeros(bc, dtype=torch.int32, device=DEV)
     S = W + 2 * K          # warm-up, gated and back-to-back slots
+    CH = list(range(W + K, S))
     out = {"workload": cfg["workload"], "batch": bc, "n": n, "nrhs": nrhs}
     if cfg["op"] == "potrf":
         As = [A0.clone() for _ in range(S)]
@@ -474,7 +489,7 @@ def run_uniform(torch, pykblas, h, name, cfg, K, W, with_cpu, cpu_sample):
         def issue(s):
             assert lib.kblasDpotrf_batch_strided(h.h, b"L", n, As[s].data_ptr(), n, NA, bc,
                                                  info.data_ptr()) == pykblas.KBLAS_Success
-        r = Stepper(torch, h, issue, S).run(K, W, lambda i: i)
+        r = Stepper(torch, h, issue, W + K, chain=CH).run(K, W, lambda i: i)
         assert int((info != 0).sum().item()) == 0
         bytes_alg, bytes_full = 8 * n * (n + 1), 16 * n * n
         unit = "factorizations/s"
@@ -487,7 +502,7 @@ def run_uniform(torch, pykblas, h, name, cfg, K, W, with_cpu, cpu_sample):
             assert lib.kblasDposv_batch_strided(h.h, b"R", b"L", nrhs, n, As[s].data_ptr(), n, NA,
                                                 Bs[s].data_ptr(), nrhs, nrhs * n, bc,
                                                 info.data_ptr()) == pykblas.KBLAS_Success
-        r = Stepper(torch, h, issue, S).run(K, W, lambda i: i)
+        r = Stepper(torch, h, issue, W + K, chain=CH).run(K, W, lambda i: i)
         assert int((info != 0).sum().item()) == 0
         bytes_alg, bytes_full = 8 * n * (n + 1) + 16 * n * nrhs, 16 * n * n + 16 * n * nrhs
         unit = "solves/s"
@@ -503,7 +518,7 @@ def run_uniform(torch, pykblas, h, name, cfg, K, W, with_cpu, cpu_sample):
                                                  B0.data_ptr(), nrhs, nrhs * n, Xs.data_ptr(), nrhs, nrhs * n,
                                                  bc, info.data_ptr(), iters.data_ptr()) == pykblas.KBLAS_Success
         log("%s: timing mixed FP32+IR POSV" % name)
-        r = Stepper(torch, h, issue, S).run(K, W, lambda i: i)
+        r = Stepper(torch, h, issue, W + K, chain=CH).run(K, W, lambda i: i)
         it_h = iters.cpu().numpy()
         X_mixed = Xs
         del As
@@ -516,7 +531,7 @@ def run_uniform(torch, pykblas, h, name, cfg, K, W, with_cpu, cpu_sample):
                                                 Bs[s].data_ptr(), nrhs, nrhs * n, bc,
                                                 info.data_ptr()) == pykblas.KBLAS_Success
         log("%s: timing FP64 POSV" % name)
-        r64 = Stepper(torch, h, issue64, S).run(K, W, lambda i: i)
+        r64 = Stepper(torch, h, issue64, W + K, chain=CH).run(K, W, lambda i: i)
         X64 = Bs[0]
         del As, Bs
         bytes_alg = 4 * n * (n + 1) + 16 * n * nrhs       # A lower read once (FP64), B in, X out
@@ -709,12 +724,13 @@ def run_poti(torch, pykblas, h, cfg, K, W, with_cpu):
     A0, _, _ = gen_uniform(torch, cfg)
     info = torch.zeros(bc, dtype=torch.int32, device=DEV)
     S = W + 2 * K          # warm-up, gated and back-to-back slots
+    CH = list(range(W + K, S))
     As = [A0.clone() for _ in range(S)]
 
     def issue(s):
         assert lib.kblasDpoti_batch_strided(h.h, b"L", n, As[s].data_ptr(), n, NA, bc,
                                             info.data_ptr()) == pykblas.KBLAS_Success
-    r = Stepper(torch, h, issue, S).run(K, W, lambda i: i)
+    r = Stepper(torch, h, issue, W + K, chain=CH).run(K, W, lambda i: i)
     assert int((info != 0).sum().item()) == 0
     X_sample = As[0]
     Ls = [A0.clone() for _ in range(S)]

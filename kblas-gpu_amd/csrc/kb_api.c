@@ -16,6 +16,7 @@
 
 #include "kb_internal.h"
 #include "batch_pstrf.h"
+#include "batch_ara.h"
 
 #define IMAX(a, b) ((a) > (b) ? (a) : (b))
 
@@ -621,6 +622,65 @@ int kblasDpstrf_batch(kblasHandle_t h, int n, double** A, int lda, int** piv, in
 { return pstrf_core(h, 'd', n, mat_ptrs(A, lda), 0, 0, piv, NULL, 0, ranks, num_ops); }
 int kblasSpstrf_batch(kblasHandle_t h, int n, float** A, int lda, int** piv, int* ranks, int num_ops)
 { return pstrf_core(h, 's', n, mat_ptrs(A, lda), 0, 0, piv, NULL, 0, ranks, num_ops); }
+
+/* ============================ ARA SPD helpers ============================ */
+/* src/batch_svd/batch_ara.cu:497-561, ara_util.cuh:417-466,560-640,765-808:
+ * bs other than 16 / 32 and max_n > 32 (FP32 Gram product) return
+ * KBLAS_NotImplemented as there.  Added: negative counts and NULL arrays
+ * return KBLAS_Error_WrongInput; num_ops == 0 returns KBLAS_Success. */
+static int ara_potrf(char rprec, int* dims, double** A, int* lda, void** R, int* ldr, double* diag, int bs,
+                     int* ranks, int num_ops, hipStream_t stream)
+{
+    if (bs != 16 && bs != 32) return KBLAS_NotImplemented;
+    if (num_ops < 0) return KBLAS_Error_WrongInput;
+    if (num_ops == 0) return KBLAS_Success;
+    if (!dims || !A || !lda || !R || !ldr || !diag || !ranks) return KBLAS_Error_WrongInput;
+    return launched(kb_launch_ara_potrf(rprec, dims, A, lda, (void* const*)R, ldr, diag, bs, ranks, num_ops, stream));
+}
+
+int kblas_sara_fused_potrf_batch(int* op_samples, double** A_batch, int* lda_batch, float** R_batch, int* ldr_batch,
+                                 double* diag_R, int bs, int* block_ranks, int num_ops, hipStream_t stream)
+{ return ara_potrf('s', op_samples, A_batch, lda_batch, (void**)R_batch, ldr_batch, diag_R, bs, block_ranks, num_ops, stream); }
+int kblas_dara_fused_potrf_batch(int* op_samples, double** A_batch, int* lda_batch, double** R_batch,
+                                 int* ldr_batch, double* diag_R, int bs, int* block_ranks, int num_ops,
+                                 hipStream_t stream)
+{ return ara_potrf('d', op_samples, A_batch, lda_batch, (void**)R_batch, ldr_batch, diag_R, bs, block_ranks, num_ops, stream); }
+
+int kblas_sara_mp_syrk_batch(kblasHandle_t h, int* m, int* n, int max_m, int max_n, const float** A, int* lda,
+                             double** B, int* ldb, int batchCount)
+{
+    KB_CHECK_HANDLE(h);
+    (void)max_m;
+    if (max_n > 32) return KBLAS_NotImplemented;
+    if (batchCount < 0) return KBLAS_Error_WrongInput;
+    if (batchCount == 0) return KBLAS_Success;
+    if (!m || !n || !A || !lda || !B || !ldb) return KBLAS_Error_WrongInput;
+    return launched(kb_launch_ara_syrk(m, n, A, lda, B, ldb, batchCount, h->stream));
+}
+int kblas_dara_mp_syrk_batch(kblasHandle_t h, int* m, int* n, int max_m, int max_n, const double** A, int* lda,
+                             double** B, int* ldb, int batchCount)
+{
+    return kblasDgemm_vbatch(h, 'T', 'N', n, n, m, max_n, max_n, max_m, 1.0, A, lda, A, lda, 0.0, B, ldb,
+                             batchCount);
+}
+
+static int ara_trsm(kblasHandle_t h, char prec, void** B, int* ldb, void** A, int* lda, int* rows, int* cols,
+                    int num_ops, int max_rows, int bs)
+{
+    KB_CHECK_HANDLE(h);
+    if (bs != 16 && bs != 32) return KBLAS_NotImplemented;
+    if (num_ops < 0 || max_rows < 0) return KBLAS_Error_WrongInput;
+    if (num_ops == 0 || max_rows == 0) return KBLAS_Success;
+    if (!B || !ldb || !A || !lda || !rows || !cols) return KBLAS_Error_WrongInput;
+    return launched(kb_launch_ara_trsm(prec, (void* const*)B, ldb, (void* const*)A, lda, rows, cols, bs, max_rows,
+                                       num_ops, h->stream));
+}
+int kblas_sara_trsm_batch(kblasHandle_t h, float** B_batch, int* ldb_batch, float** A_batch, int* lda_batch,
+                          int* rows_batch, int* cols_batch, int num_ops, int max_rows, int bs)
+{ return ara_trsm(h, 's', (void**)B_batch, ldb_batch, (void**)A_batch, lda_batch, rows_batch, cols_batch, num_ops, max_rows, bs); }
+int kblas_dara_trsm_batch(kblasHandle_t h, double** B_batch, int* ldb_batch, double** A_batch, int* lda_batch,
+                          int* rows_batch, int* cols_batch, int num_ops, int max_rows, int bs)
+{ return ara_trsm(h, 'd', (void**)B_batch, ldb_batch, (void**)A_batch, lda_batch, rows_batch, cols_batch, num_ops, max_rows, bs); }
 
 /* ====================== workspace queries (all zero) ===================== */
 /* The uniform paths run without any workspace: no pointer arrays are built

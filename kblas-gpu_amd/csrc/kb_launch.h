@@ -132,6 +132,19 @@ int kb_launch_trmm(char prec, int side_left, int trans, int unit, int m, int n, 
 int kb_launch_pstrf(char prec, int n, kb_mat_t A, int* const* pivp, int* pivs, long spiv, int* ranks,
                     int batch, hipStream_t s);
 
+/* Randomized-range-finder SPD helpers (all operands per matrix through device
+ * arrays).  Rank-revealing Cholesky of dims[b] x dims[b] Gram matrices
+ * (dims clamped to bs <= 32), R written as FP32 (rprec 's') or FP64 ('d'). */
+int kb_launch_ara_potrf(char rprec, const int* dims, double* const* G, const int* ldg, void* const* R,
+                        const int* ldr, double* diag, int bs, int* ranks, int num_ops, hipStream_t s);
+/* B := A^T A (n x n, n clamped to 32) of FP32 m x n matrices, FP64 result. */
+int kb_launch_ara_syrk(const int* m, const int* n, const float* const* A, const int* lda,
+                       double* const* B, const int* ldb, int batch, hipStream_t s);
+/* B(:, 0:c) := B(:, 0:c) R^-1, R upper of order c = min(cols[b], bs), bs <= 32;
+ * rows[b] <= max_rows rows. */
+int kb_launch_ara_trsm(char prec, void* const* B, const int* ldb, void* const* R, const int* ldr,
+                       const int* rows, const int* cols, int bs, int max_rows, int num_ops, hipStream_t s);
+
 /* Fill count ints with v (a kernel, so it is safe inside captured graphs). */
 int kb_launch_fill_int(int* p, int v, long count, hipStream_t s);
 

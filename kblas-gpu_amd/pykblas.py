@@ -112,6 +112,13 @@ _DONLY = {
     "kblasDsposv_batch": (c_int, [P, c_char, c_char, c_int, c_int, P, c_int, P, c_int, P, c_int, c_int, P, P]),
     "kblasDsposv_vbatch": (c_int, [P, c_char, c_char, P, P, c_int, c_int, P, P, P, P, P, P, c_int, P, P]),
     "kblasDposv_vbatch_noalloc": (c_int, [P, c_char, c_char, P, P, c_int, c_int, P, P, P, P, c_int, P]),
+    # randomized-range-finder SPD helpers (include/batch_ara.h)
+    "kblas_sara_fused_potrf_batch": (c_int, [P, P, P, P, P, P, c_int, P, c_int, P]),
+    "kblas_dara_fused_potrf_batch": (c_int, [P, P, P, P, P, P, c_int, P, c_int, P]),
+    "kblas_sara_mp_syrk_batch": (c_int, [P, P, P, c_int, c_int, P, P, P, P, c_int]),
+    "kblas_dara_mp_syrk_batch": (c_int, [P, P, P, c_int, c_int, P, P, P, P, c_int]),
+    "kblas_sara_trsm_batch": (c_int, [P, P, P, P, P, P, P, c_int, c_int, c_int]),
+    "kblas_dara_trsm_batch": (c_int, [P, P, P, P, P, P, P, c_int, c_int, c_int]),
 }
 _TSIG["pstrf_batch"] = (c_int, [P, c_int, P, c_int, P, P, c_int])
 _TSIG["pstrf_batch_strided"] = (c_int, [P, c_int, P, c_int, c_int, P, c_int, P, c_int])

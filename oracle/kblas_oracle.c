@@ -1014,3 +1014,72 @@ int orc_pstrf(int n, double* A, int lda, int* piv)
     free(S);
     return rank;
 }
+
+/* ARA rank-revealing Cholesky, src/batch_svd/ara_util.cuh:163-275
+ * (ara_fused_potrf_kernel): dim = min(dim, bs); tol = max(0, max diag(G)) *
+ * 2^-52 (:220, KBlasEpsilon<double>, src/kblas_gpu_util.ch:63); for k = 0..:
+ * stop when the updated G(k,k) <= tol (:227), G(k,k) = sqrt, d[k] *= G(k,k)
+ * (:234-235), G(i,k) /= G(k,k) for i > k (:240), G(i,c) -= G(c,k) G(i,k)
+ * for i, c > k (:251, contracted to one FMA as the CUDA compiler does);
+ * then d[k] = 0 from the rank on (k < bs, :261), R(i,c) = G(c,i) for i <= c,
+ * 0 below (:272).  The lower triangle evolves on its own (the update of
+ * G(i,c), c <= i, reads column k only), so only it is kept.  dim <= 0:
+ * returns 0 and writes nothing else (:186-190).  Returns the rank. */
+int orc_ara_potrf(int dim, int bs, const double* G, int ldg, double* R, int ldr, double* d)
+{
+    if (dim > bs) dim = bs;
+    if (dim <= 0) return 0;
+    double* S = (double*)calloc((size_t)dim * dim, sizeof(double));
+    double mx = 0.0;
+    for (int j = 0; j < dim; ++j) {
+        for (int i = j; i < dim; ++i) S[i + (long)j * dim] = G[i + (long)j * ldg];
+        if (G[j + (long)j * ldg] > mx) mx = G[j + (long)j * ldg];
+    }
+    const double tol = mx * 2.2204460492503131e-16;
+    int k;
+    for (k = 0; k < dim; ++k) {
+        double* ck = S + (long)k * dim;
+        if (ck[k] <= tol) break;
+        ck[k] = sqrt(ck[k]);
+        d[k] *= ck[k];
+        for (int i = k + 1; i < dim; ++i) ck[i] = ck[i] / ck[k];
+        for (int c = k + 1; c < dim; ++c)
+            for (int i = c; i < dim; ++i) S[i + (long)c * dim] = fma(-ck[c], ck[i], S[i + (long)c * dim]);
+    }
+    for (int i = k; i < bs; ++i) d[i] = 0.0;
+    for (int c = 0; c < dim; ++c)
+        for (int i = 0; i < dim; ++i) R[i + (long)c * ldr] = i > c ? 0.0 : S[c + (long)i * dim];
+    free(S);
+    return k;
+}
+
+/* ARA Gram product B := A^T A (n x n, both triangles), A m x n:
+ * src/batch_svd/ara_util.cuh:651-744 (ara_mp_syrk_batch_kernel, FP32 A,
+ * :706,737): each B(i,j) sums A(r,i) A(r,j) over r = 0..m-1 in order in FP64
+ * (the products of FP32 entries are exact); :755-764 (FP64 A) is the batched
+ * GEMM 1 * A^T A + 0 * B, whose summation order is the library's. */
+void orc_ara_syrk(int m, int n, const double* A, int lda, double* B, int ldb)
+{
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            double s = 0.0;
+            for (int r = 0; r < m; ++r) s = fma(A[r + (long)i * lda], A[r + (long)j * lda], s);
+            B[i + (long)j * ldb] = s;
+        }
+}
+
+/* ARA right triangular solve B(:, 0:c) := B(:, 0:c) R^-1, c = min(cols, bs),
+ * R upper: src/batch_svd/ara_util.cuh:345-408 (ara_16x16_trsm_kernel, per
+ * row: b_j -= R(k,j) b_k for k < j, then b_j /= R(j,j)) and :560-640 (the
+ * 32-column case as two 16-column solves around B2 -= B1 R12, a batched
+ * GEMM); restated with one sequential sum over k < j. */
+void orc_ara_trsm(int rows, int cols, int bs, double* B, int ldb, const double* R, int ldr)
+{
+    if (cols > bs) cols = bs;
+    for (int r = 0; r < rows; ++r)
+        for (int j = 0; j < cols; ++j) {
+            double s = B[r + (long)j * ldb];
+            for (int k = 0; k < j; ++k) s = fma(-R[k + (long)j * ldr], B[r + (long)k * ldb], s);
+            B[r + (long)j * ldb] = s / R[j + (long)j * ldr];
+        }
+}

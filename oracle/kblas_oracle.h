@@ -18,6 +18,8 @@
  *   - TRTRI / LAUUM / POTRI / TRMM     src/batch_triangular/X{trtri,lauum,trmm}_batch_kernels.cuh,
  *                                       Xpotri_batch_drivers.cuh:78-110
  *   - PSTRF (pivoted Cholesky)         src/batch_svd/batch_pstrf.cu:44-158
+ *   - ARA rank-revealing POTRF, Gram   src/batch_svd/ara_util.cuh:163-275,345-408,
+ *     product, right TRSM                560-640,651-764
  *   - FLOP model                       testing/flops.h:74-150
  *   - reference test data              testing/testing_helper.cu:361-403
  * Extensions with no reference counterpart (documented as such):
@@ -67,6 +69,14 @@ int  orc_potrf(int n, double* A, int lda);
 int  orc_trtri(char diag, int n, double* A, int lda);
 /* Pivoted Cholesky P^T A P = R^T R (upper), n <= 64; piv 0-based; returns the rank. */
 int  orc_pstrf(int n, double* A, int lda, int* piv);
+/* ARA helpers (src/batch_svd/ara_util.cuh): rank-revealing Cholesky of the
+ * lower triangle of a dim x dim Gram matrix (dim clamped to bs; R = upper
+ * factor, d[0..bs) scaled by diag(R) below the rank, zeroed from it; returns
+ * the rank), the Gram product B = A^T A, and B(:, 0:min(cols,bs)) *= R^-1
+ * with R upper.                                                              */
+int  orc_ara_potrf(int dim, int bs, const double* G, int ldg, double* R, int ldr, double* d);
+void orc_ara_syrk(int m, int n, const double* A, int lda, double* B, int ldb);
+void orc_ara_trsm(int rows, int cols, int bs, double* B, int ldb, const double* R, int ldr);
 void orc_lauum(int n, double* A, int lda);
 int  orc_potri(int n, double* A, int lda);
 void orc_trmm(char side, char trans, char diag, int m, int n, double alpha, const double* A, int lda,

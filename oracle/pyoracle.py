@@ -38,6 +38,9 @@ _SIG = {
     "orc_potrs": (None, [_c, _i, _i, _p, _i, _p, _i]),
     "orc_trtri": (_i, [_c, _i, _p, _i]),
     "orc_pstrf": (_i, [_i, _p, _i, _p]),
+    "orc_ara_potrf": (_i, [_i, _i, _p, _i, _p, _i, _p]),
+    "orc_ara_syrk": (None, [_i, _i, _p, _i, _p, _i]),
+    "orc_ara_trsm": (None, [_i, _i, _i, _p, _i, _p, _i]),
     "orc_lauum": (None, [_i, _p, _i]),
     "orc_potri": (_i, [_i, _p, _i]),
     "orc_trmm": (None, [_c, _c, _c, _i, _i, _d, _p, _i, _p, _i]),
@@ -150,6 +153,37 @@ def pstrf(A):
     piv = np.zeros(max(n, 1), dtype=np.int32)
     rank = lib().orc_pstrf(n, ptr(R), n, ptr(piv))
     return R, piv[:n], rank
+
+
+def ara_potrf(G, bs, d, dim=None):
+    """rank-revealing Cholesky of the lower triangle of G's leading dim block;
+    returns (R (dim' x dim', dim' = min(dim, bs)), d (bs entries), rank)"""
+    G = np.asfortranarray(G, dtype=np.float64)
+    dim = G.shape[0] if dim is None else dim
+    dd = np.array(d, dtype=np.float64, copy=True)
+    assert dd.shape[0] >= bs
+    m = max(min(dim, bs), 1)
+    R = np.full((m, m), np.nan, order="F")
+    rank = lib().orc_ara_potrf(dim, bs, ptr(G), max(G.shape[0], 1), ptr(R), m, ptr(dd))
+    return R[:max(min(dim, bs), 0), :max(min(dim, bs), 0)], dd, rank
+
+
+def ara_syrk(A):
+    """A^T A summed over the rows in order (A given as float64 values)"""
+    A = np.asfortranarray(A, dtype=np.float64)
+    m, n = A.shape
+    B = np.zeros((n, n), order="F")
+    lib().orc_ara_syrk(m, n, ptr(A), max(m, 1), ptr(B), max(n, 1))
+    return B
+
+
+def ara_trsm(B, R, bs):
+    """B(:, 0:min(cols, bs)) R^-1 with R upper"""
+    X = np.array(B, dtype=np.float64, order="F", copy=True)
+    R = np.asfortranarray(R, dtype=np.float64)
+    rows, cols = X.shape
+    lib().orc_ara_trsm(rows, cols, bs, ptr(X), max(rows, 1), ptr(R), max(R.shape[0], 1))
+    return X
 
 
 def lauum(L):

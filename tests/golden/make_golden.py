@@ -13,7 +13,10 @@ expected values, but seeded and committed:
 The inverse family (dtrtri / dlauum / dpotri, and dtrmm) and dpstrf go to a file of
 their own, lapack_golden_inv.npz, written by ``--inv`` from its own seeded generator.
 Mixed precision (MKL dsposv / spotrf) goes to lapack_golden_mixed.npz, written by ``--mixed``.
-Run in the build container only (needs MKL): python tests/golden/make_golden.py [--inv | --mixed]
+The randomized-range-finder helpers (Gram matrices of FP32 samples: MKL dsyrk of
+the widened samples, dpotrf of the Gram matrix, dtrsm with its transpose) go to
+lapack_golden_ara.npz, written by ``--ara``.
+Run in the build container only (needs MKL): python tests/golden/make_golden.py [--inv | --mixed | --ara]
 """
 import ctypes
 import os
@@ -118,7 +121,7 @@ def main():
         out["potrf_L_%d" % n] = np.tril(L)
         # POSV side R (reference layout): B is m x n, m right-hand sides as rows
         for m in (1, 4):
-            B = np.asfortranarray(rng.standard_normal((m, n)))
+            B = np.asfortranarray(rng.standard_normal((40, n)))
             X = dtrsm(b"R", b"T", 1.0, L, B)
             X = dtrsm(b"R", b"N", 1.0, L, X)
             out["posvR_B_%d_%d" % (n, m)] = B
@@ -346,8 +349,36 @@ def mixed_main():
     print("ITER values:", its)
 
 
+def ara_main():
+    """Gram matrices of FP32 sample blocks Y (m x n, entries U(-1,1) rounded to
+    FP32), as the range finder forms them: G = Y^T Y from MKL dsyrk('L','T') of
+    the FP64-widened samples (lower, mirrored), its MKL dpotrf factor L
+    (R = L^T), and X = B R^-1 from MKL dtrsm('R','L','T') with L, for B of
+    40 x n N(0,1) entries."""
+    rng = np.random.default_rng(20240613)
+    out = {"mkl_version": np.array(MKL_VERSION)}
+    for m, n in ((40, 1), (64, 7), (100, 16), (96, 17), (250, 29), (300, 32)):
+        Y = np.asfortranarray(rng.uniform(-1.0, 1.0, (m, n)).astype(np.float32))
+        G = dsyrk(b"T", 1.0, Y.astype(np.float64), 0.0, np.zeros((n, n), order="F"))
+        G = np.tril(G) + np.tril(G, -1).T
+        L, info = dpotrf(G)
+        assert info == 0
+        B = np.asfortranarray(rng.standard_normal((40, n)))
+        X = dtrsm(b"R", b"T", 1.0, np.tril(L), B)
+        key = "ara_%d_%d" % (m, n)
+        out[key + "_Y"] = Y
+        out[key + "_G"] = G
+        out[key + "_L"] = np.tril(L)
+        out[key + "_B"] = B
+        out[key + "_X"] = X
+    np.savez_compressed(os.path.join(HERE, "lapack_golden_ara.npz"), **out)
+    print("wrote", len(out), "arrays with", MKL_VERSION)
+
+
 if __name__ == "__main__":
-    if "--inv" in sys.argv[1:]:
+    if "--ara" in sys.argv[1:]:
+        ara_main()
+    elif "--inv" in sys.argv[1:]:
         inv_main()
     elif "--mixed" in sys.argv[1:]:
         mixed_main()

@@ -71,3 +71,15 @@ def test_null_handle_rejected_without_device():
 def test_kernels_are_gfx950_code_objects():
     out = subprocess.run(["strings", pykblas.LIB_PATH], capture_output=True, text=True).stdout
     assert "gfx950" in out
+
+
+def test_ara_argument_checks_without_device():
+    lib = pykblas.load()
+    P = ctypes.c_void_p
+    z = P(0)
+    assert lib.kblas_dara_fused_potrf_batch(z, z, z, z, z, z, 24, z, 4, z) == pykblas.KBLAS_NotImplemented
+    assert lib.kblas_sara_fused_potrf_batch(z, z, z, z, z, z, 16, z, 0, z) == pykblas.KBLAS_Success
+    assert lib.kblas_dara_fused_potrf_batch(z, z, z, z, z, z, 32, z, -1, z) == pykblas.KBLAS_Error_WrongInput
+    assert lib.kblas_dara_fused_potrf_batch(z, z, z, z, z, z, 32, z, 3, z) == pykblas.KBLAS_Error_WrongInput
+    assert lib.kblas_sara_trsm_batch(None, z, z, z, z, z, z, 4, 10, 16) == pykblas.KBLAS_Error_NotInitialized
+    assert lib.kblas_sara_mp_syrk_batch(None, z, z, 10, 16, z, z, z, z, 4) == pykblas.KBLAS_Error_NotInitialized
