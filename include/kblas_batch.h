@@ -8,11 +8,18 @@
  * device pointers (as the reference's tests and kernels use them); strides are
  * in elements; calls are asynchronous on the handle's stream.
  *
- * Supported: uplo = Lower; TRSM diag = NonUnit; POTRS/POSV side = Right
- * (X*A = B, the reference's only side) and Left (A*X = B, LAPACK layout --
- * declared but NotImplemented in the reference).  Upper and Unit return
- * KBLAS_NotImplemented as in the reference.  Complex (C/Z) entry points
- * exist for link compatibility and return KBLAS_NotImplemented.
+ * Supported: uplo Lower and Upper (the reference returns NotImplemented for
+ * Upper; here A = U^T U with U read from and written to the upper triangle);
+ * TRSM diag NonUnit and Unit (Unit: NotImplemented in the reference);
+ * POTRS/POSV side Right (X*A = B, the reference's only side) and Left
+ * (A*X = B, LAPACK layout -- declared but NotImplemented in the reference).
+ * Complex (C/Z) POTRF / POTRS / POSV / TRSM / SYRK / GEMM compute what the
+ * reference's complex instantiations compute: complex symmetric arithmetic
+ * (A = L L^T without conjugation, principal square roots, SYRK = A A^T),
+ * trans 'C' a conjugate transpose for GEMM and TRSM (and rejected by SYRK);
+ * a zero or non-finite pivot sets info.  The complex inverse family (LAUUM =
+ * L^T L, POTRI = (L L^T)^-1) and TRMM ('C' conjugates the triangle) likewise;
+ * their upper-triangle variants return KBLAS_NotImplemented.
  * Extension: info_array is written for every matrix (0, or the LAPACK index of
  * the first non-positive leading minor), which the reference documents but
  * never sets (src/batch_triangular/Xpotrf_batch_kernels.cuh:123-125).

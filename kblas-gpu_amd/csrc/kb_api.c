@@ -29,6 +29,16 @@ static int transof(char c)
     return -1;
 }
 static int diagof(char c) { return c == 'N' || c == 'n' ? 0 : (c == 'U' || c == 'u') ? 1 : -1; }
+static int cplx(char prec) { return prec == 'c' || prec == 'z'; }
+/* op code of the generic kernels: N 0, T 1, C 2 (complex; C is T for real data) */
+static int opof(char prec, char c)
+{
+    const int t = transof(c);
+    return (t == 1 && cplx(prec) && (c == 'C' || c == 'c')) ? 2 : t;
+}
+/* scalars as (real, imaginary) pairs */
+#define RS(v) ((const double[2]){(double)(v), 0.0})
+#define CS(v) ((const double[2]){(double)(v).x, (double)(v).y})
 
 static kb_mat_t mat_strided(const void* p, int ld, long stride)
 {
@@ -66,8 +76,10 @@ static void zero_info(kblasHandle_t h, int* info, int bc)
  * (register-resident kernels up to KB_REG_MAX_N, the blocked workgroup
  * kernel above). */
 static int chol_dispatch(kblasHandle_t h, char prec, int mode, int n, int nrhs, int side_left, kb_mat_t A,
-                         kb_mat_t B, int* info, int bc)
+                         kb_mat_t B, int* info, int bc, int upper)
 {
+    /* the upper triangle and the complex types: the generic kernels */
+    if (upper || cplx(prec)) return kb_launch_gchol(prec, mode, upper, n, nrhs, side_left, A, B, info, bc, h->stream);
     const int v = kb_tune_variant(prec, mode, n);
     if (v == KB_VAR_BLK || n > KB_REG_MAX_N)
         return kb_launch_blk_chol(prec, mode, n, nrhs, side_left, A, B, info, bc, h->stream);
@@ -81,11 +93,10 @@ static int potrf_core(kblasHandle_t h, char prec, char uplo, int n, kb_mat_t A,
     KB_CHECK_HANDLE(h);
     int u = upl(uplo);
     if (u < 0 || n < 0 || bc < 0 || A.ld < IMAX(1, n)) return KBLAS_Error_WrongInput;
-    if (u == 1) return KBLAS_NotImplemented;
     if (bc == 0) return KBLAS_Success;
     if (strided ? (!A.base || strideA < (long)A.ld * n) : !A.ptrs) return KBLAS_Error_WrongInput;
     if (n == 0) { zero_info(h, info, bc); return KBLAS_Success; }
-    return launched(chol_dispatch(h, prec, KB_MODE_POTRF, n, 0, 0, A, A, info, bc));
+    return launched(chol_dispatch(h, prec, KB_MODE_POTRF, n, 0, 0, A, A, info, bc, u));
 }
 
 int kblasDpotrf_batch(kblasHandle_t h, char uplo, const int n, double** A, int lda, int bc, int* info)
@@ -113,7 +124,6 @@ static int solve_core(kblasHandle_t h, char prec, int mode, char side, char uplo
     int order = sd == 0 ? m : n;
     int nrhs = sd == 0 ? n : m;
     if (A.ld < IMAX(1, order) || B.ld < IMAX(1, m)) return KBLAS_Error_WrongInput;
-    if (u == 1) return KBLAS_NotImplemented;
     if (bc == 0) return KBLAS_Success;
     if (strided) {
         if (!A.base || !B.base || strideB < (long)B.ld * n || strideA < 0) return KBLAS_Error_WrongInput;
@@ -123,7 +133,7 @@ static int solve_core(kblasHandle_t h, char prec, int mode, char side, char uplo
     }
     if (order == 0) { if (mode == KB_MODE_POSV) zero_info(h, info, bc); return KBLAS_Success; }
     if (nrhs == 0 && mode == KB_MODE_POTRS) return KBLAS_Success;
-    return launched(chol_dispatch(h, prec, mode, order, nrhs, sd == 0, A, B, info, bc));
+    return launched(chol_dispatch(h, prec, mode, order, nrhs, sd == 0, A, B, info, bc, u));
 }
 
 int kblasDpotrs_batch(kblasHandle_t h, char side, char uplo, const int m, const int n,
@@ -199,81 +209,78 @@ int kblasDsposv_batch(kblasHandle_t h, char side, char uplo, const int m, const 
 
 /* ================================ TRSM =================================== */
 static int trsm_core(kblasHandle_t h, char prec, char side, char uplo, char trans, char diag,
-                     int m, int n, double alpha, kb_mat_t A, long sA, kb_mat_t B, long sB,
+                     int m, int n, const double* alpha, kb_mat_t A, long sA, kb_mat_t B, long sB,
                      int strided, int bc)
 {
     KB_CHECK_HANDLE(h);
-    int sd = sideof(side), u = upl(uplo), t = transof(trans), dg = diagof(diag);
+    int sd = sideof(side), u = upl(uplo), t = opof(prec, trans), dg = diagof(diag);
     if (sd < 0 || u < 0 || t < 0 || dg < 0 || m < 0 || n < 0 || bc < 0) return KBLAS_Error_WrongInput;
     int ka = sd == 0 ? m : n;
     if (A.ld < IMAX(1, ka) || B.ld < IMAX(1, m)) return KBLAS_Error_WrongInput;
-    /* src/batch_triangular/Xtrsm_batch_drivers.cuh:64-67 */
-    if (u == 1 || dg == 1) return KBLAS_NotImplemented;
     if (bc == 0 || m == 0 || n == 0) return KBLAS_Success;
     if (strided) {
         if (!A.base || !B.base || sA < 0 || sB < (long)B.ld * n) return KBLAS_Error_WrongInput;
     } else if (!A.ptrs || !B.ptrs) {
         return KBLAS_Error_WrongInput;
     }
-    return launched(kb_launch_trsm(prec, sd == 0, t, dg, m, n, alpha, A, B, NULL, NULL, bc, h->stream));
+    return launched(kb_launch_trsm_x(prec, sd == 0, u, t, dg, m, n, alpha, A, B, NULL, NULL, bc, h->stream));
 }
 
 int kblasDtrsm_batch(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m,
                      const int n, const double alpha, const double** A, int lda, double** B, int ldb, int bc)
-{ return trsm_core(h, 'd', side, uplo, trans, diag, m, n, alpha, mat_ptrs(A, lda), 0, mat_ptrs(B, ldb), 0, 0, bc); }
+{ return trsm_core(h, 'd', side, uplo, trans, diag, m, n, RS(alpha), mat_ptrs(A, lda), 0, mat_ptrs(B, ldb), 0, 0, bc); }
 int kblasStrsm_batch(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m,
                      const int n, const float alpha, const float** A, int lda, float** B, int ldb, int bc)
-{ return trsm_core(h, 's', side, uplo, trans, diag, m, n, alpha, mat_ptrs(A, lda), 0, mat_ptrs(B, ldb), 0, 0, bc); }
+{ return trsm_core(h, 's', side, uplo, trans, diag, m, n, RS(alpha), mat_ptrs(A, lda), 0, mat_ptrs(B, ldb), 0, 0, bc); }
 int kblasDtrsm_batch_strided(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m,
                              const int n, const double alpha, const double* A, int lda, long sA,
                              double* B, int ldb, long sB, int bc)
-{ return trsm_core(h, 'd', side, uplo, trans, diag, m, n, alpha, mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB, 1, bc); }
+{ return trsm_core(h, 'd', side, uplo, trans, diag, m, n, RS(alpha), mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB, 1, bc); }
 int kblasStrsm_batch_strided(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m,
                              const int n, const float alpha, const float* A, int lda, long sA,
                              float* B, int ldb, long sB, int bc)
-{ return trsm_core(h, 's', side, uplo, trans, diag, m, n, alpha, mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB, 1, bc); }
+{ return trsm_core(h, 's', side, uplo, trans, diag, m, n, RS(alpha), mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB, 1, bc); }
 
 /* ================================ SYRK =================================== */
-static int syrk_core(kblasHandle_t h, char prec, char uplo, char trans, int m, int n, double alpha,
-                     kb_mat_t A, long sA, double beta, kb_mat_t B, long sB, int strided, int bc)
+static int syrk_core(kblasHandle_t h, char prec, char uplo, char trans, int m, int n, const double* alpha,
+                     kb_mat_t A, long sA, const double* beta, kb_mat_t B, long sB, int strided, int bc)
 {
     KB_CHECK_HANDLE(h);
-    int u = upl(uplo), t = transof(trans);
-    if (u < 0 || t < 0 || m < 0 || n < 0 || bc < 0) return KBLAS_Error_WrongInput;
+    int u = upl(uplo), t = opof(prec, trans);
+    /* symmetric rank-k update: no conjugate-transposed form for complex data */
+    if (u < 0 || t < 0 || t == 2 || m < 0 || n < 0 || bc < 0) return KBLAS_Error_WrongInput;
     if (A.ld < IMAX(1, t ? n : m) || B.ld < IMAX(1, m)) return KBLAS_Error_WrongInput;
-    /* src/batch_triangular/Xsyrk_batch_drivers.cuh:133-136 */
-    if (u == 1) return KBLAS_NotImplemented;
     if (bc == 0 || m == 0) return KBLAS_Success;
     if (strided) {
         if (!A.base || !B.base || sA < 0 || sB < (long)B.ld * m) return KBLAS_Error_WrongInput;
     } else if (!A.ptrs || !B.ptrs) {
         return KBLAS_Error_WrongInput;
     }
-    return launched(kb_launch_syrk(prec, t, m, n, alpha, A, beta, B, NULL, NULL, bc, h->stream));
+    return launched(kb_launch_syrk_x(prec, u, t, m, n, alpha, A, beta, B, NULL, NULL, bc, h->stream));
 }
 
 int kblasDsyrk_batch(kblasHandle_t h, char uplo, char trans, const int m, const int n, const double alpha,
                      const double** A, int lda, const double beta, double** B, int ldb, int bc)
-{ return syrk_core(h, 'd', uplo, trans, m, n, alpha, mat_ptrs(A, lda), 0, beta, mat_ptrs(B, ldb), 0, 0, bc); }
+{ return syrk_core(h, 'd', uplo, trans, m, n, RS(alpha), mat_ptrs(A, lda), 0, RS(beta), mat_ptrs(B, ldb), 0, 0, bc); }
 int kblasSsyrk_batch(kblasHandle_t h, char uplo, char trans, const int m, const int n, const float alpha,
                      const float** A, int lda, const float beta, float** B, int ldb, int bc)
-{ return syrk_core(h, 's', uplo, trans, m, n, alpha, mat_ptrs(A, lda), 0, beta, mat_ptrs(B, ldb), 0, 0, bc); }
+{ return syrk_core(h, 's', uplo, trans, m, n, RS(alpha), mat_ptrs(A, lda), 0, RS(beta), mat_ptrs(B, ldb), 0, 0, bc); }
 int kblasDsyrk_batch_strided(kblasHandle_t h, char uplo, char trans, const int m, const int n,
                              const double alpha, const double* A, int lda, long sA,
                              const double beta, double* B, int ldb, long sB, int bc)
-{ return syrk_core(h, 'd', uplo, trans, m, n, alpha, mat_strided(A, lda, sA), sA, beta, mat_strided(B, ldb, sB), sB, 1, bc); }
+{ return syrk_core(h, 'd', uplo, trans, m, n, RS(alpha), mat_strided(A, lda, sA), sA, RS(beta), mat_strided(B, ldb, sB), sB, 1, bc); }
 int kblasSsyrk_batch_strided(kblasHandle_t h, char uplo, char trans, const int m, const int n,
                              const float alpha, const float* A, int lda, long sA,
                              const float beta, float* B, int ldb, long sB, int bc)
-{ return syrk_core(h, 's', uplo, trans, m, n, alpha, mat_strided(A, lda, sA), sA, beta, mat_strided(B, ldb, sB), sB, 1, bc); }
+{ return syrk_core(h, 's', uplo, trans, m, n, RS(alpha), mat_strided(A, lda, sA), sA, RS(beta), mat_strided(B, ldb, sB), sB, 1, bc); }
 
 /* ================================ GEMM =================================== */
-static int gemm_core(kblasHandle_t h, char prec, char ta, char tb, int m, int n, int k, double alpha,
-                     kb_mat_t A, long sA, kb_mat_t B, long sB, double beta, kb_mat_t C, long sC,
+static int gemm_core(kblasHandle_t h, char prec, char ta, char tb, int m, int n, int k, const double* alpha,
+                     kb_mat_t A, long sA, kb_mat_t B, long sB, const double* beta, kb_mat_t C, long sC,
                      int strided, int bc)
 {
     KB_CHECK_HANDLE(h);
-    int a = transof(ta), b = transof(tb);
+    int a = opof(prec, ta), b = opof(prec, tb);
     if (a < 0 || b < 0 || m < 0 || n < 0 || k < 0) return KBLAS_Error_WrongInput;
     /* src/batch_triangular/Xgemm_batch_core.cuh:181-182 rejects batchCount < 1 */
     if (bc < 0) return KBLAS_Error_WrongInput;
@@ -286,27 +293,27 @@ static int gemm_core(kblasHandle_t h, char prec, char ta, char tb, int m, int n,
     } else if (!A.ptrs || !B.ptrs || !C.ptrs) {
         return KBLAS_Error_WrongInput;
     }
-    return launched(kb_launch_gemm(prec, a, b, m, n, k, alpha, A, B, beta, C, NULL, NULL, NULL, bc, h->stream));
+    return launched(kb_launch_gemm_x(prec, a, b, m, n, k, alpha, A, B, beta, C, NULL, NULL, NULL, bc, h->stream));
 }
 
 int kblasDgemm_batch(kblasHandle_t h, char ta, char tb, const int m, const int n, const int k,
                      const double alpha, const double** A, int lda, const double** B, int ldb,
                      const double beta, double** C, int ldc, int bc)
-{ return gemm_core(h, 'd', ta, tb, m, n, k, alpha, mat_ptrs(A, lda), 0, mat_ptrs(B, ldb), 0, beta, mat_ptrs(C, ldc), 0, 0, bc); }
+{ return gemm_core(h, 'd', ta, tb, m, n, k, RS(alpha), mat_ptrs(A, lda), 0, mat_ptrs(B, ldb), 0, RS(beta), mat_ptrs(C, ldc), 0, 0, bc); }
 int kblasSgemm_batch(kblasHandle_t h, char ta, char tb, const int m, const int n, const int k,
                      const float alpha, const float** A, int lda, const float** B, int ldb,
                      const float beta, float** C, int ldc, int bc)
-{ return gemm_core(h, 's', ta, tb, m, n, k, alpha, mat_ptrs(A, lda), 0, mat_ptrs(B, ldb), 0, beta, mat_ptrs(C, ldc), 0, 0, bc); }
+{ return gemm_core(h, 's', ta, tb, m, n, k, RS(alpha), mat_ptrs(A, lda), 0, mat_ptrs(B, ldb), 0, RS(beta), mat_ptrs(C, ldc), 0, 0, bc); }
 int kblasDgemm_batch_strided(kblasHandle_t h, char ta, char tb, const int m, const int n, const int k,
                              const double alpha, const double* A, int lda, long sA,
                              const double* B, int ldb, long sB, const double beta,
                              double* C, int ldc, long sC, int bc)
-{ return gemm_core(h, 'd', ta, tb, m, n, k, alpha, mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB, beta, mat_strided(C, ldc, sC), sC, 1, bc); }
+{ return gemm_core(h, 'd', ta, tb, m, n, k, RS(alpha), mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB, RS(beta), mat_strided(C, ldc, sC), sC, 1, bc); }
 int kblasSgemm_batch_strided(kblasHandle_t h, char ta, char tb, const int m, const int n, const int k,
                              const float alpha, const float* A, int lda, long sA,
                              const float* B, int ldb, long sB, const float beta,
                              float* C, int ldc, long sC, int bc)
-{ return gemm_core(h, 's', ta, tb, m, n, k, alpha, mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB, beta, mat_strided(C, ldc, sC), sC, 1, bc); }
+{ return gemm_core(h, 's', ta, tb, m, n, k, RS(alpha), mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB, RS(beta), mat_strided(C, ldc, sC), sC, 1, bc); }
 
 /* ============================ non-uniform ================================ */
 /* Per-matrix sizes live on the device, so only host-visible arguments are
@@ -525,7 +532,7 @@ static int inv_core(kblasHandle_t h, char prec, int what, char uplo, char diag, 
     if (n == 0) { zero_info(h, info, bc); return KBLAS_Success; }
     if (n > KB_INV_MAX_N) return KBLAS_NotImplemented;
     if (what == INV_POTI) {
-        int st = chol_dispatch(h, prec, KB_MODE_POTRF, n, 0, 0, A, A, info, bc);
+        int st = chol_dispatch(h, prec, KB_MODE_POTRF, n, 0, 0, A, A, info, bc, 0);
         if (st) return launched(st);
         return launched(kb_launch_inv(prec, KB_INV_OP_POTRI, 0, n, A, info, info, bc, h->stream));
     }
@@ -562,11 +569,11 @@ int kblasStrtri_batch_strided(kblasHandle_t h, char uplo, char diag, const int n
 /* src/batch_triangular/Xtrmm_batch.cu:63-105 serves Left / Lower / NonUnit
  * only; here Right and Unit are served too, Upper returns NotImplemented. */
 static int trmm_core(kblasHandle_t h, char prec, char side, char uplo, char trans, char diag,
-                     int m, int n, double alpha, kb_mat_t A, long sA, kb_mat_t B, long sB,
+                     int m, int n, const double* alpha, kb_mat_t A, long sA, kb_mat_t B, long sB,
                      int strided, int bc)
 {
     KB_CHECK_HANDLE(h);
-    int sd = sideof(side), u = upl(uplo), t = transof(trans), dg = diagof(diag);
+    int sd = sideof(side), u = upl(uplo), t = opof(prec, trans), dg = diagof(diag);
     if (sd < 0 || u < 0 || t < 0 || dg < 0 || m < 0 || n < 0 || bc < 0) return KBLAS_Error_WrongInput;
     int ka = sd == 0 ? m : n;
     if (A.ld < IMAX(1, ka) || B.ld < IMAX(1, m)) return KBLAS_Error_WrongInput;
@@ -582,18 +589,18 @@ static int trmm_core(kblasHandle_t h, char prec, char side, char uplo, char tran
 
 int kblasDtrmm_batch(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n,
                      const double alpha, const double** A, int lda, double** B, int ldb, int bc)
-{ return trmm_core(h, 'd', side, uplo, trans, diag, m, n, alpha, mat_ptrs(A, lda), 0, mat_ptrs(B, ldb), 0, 0, bc); }
+{ return trmm_core(h, 'd', side, uplo, trans, diag, m, n, RS(alpha), mat_ptrs(A, lda), 0, mat_ptrs(B, ldb), 0, 0, bc); }
 int kblasStrmm_batch(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n,
                      const float alpha, const float** A, int lda, float** B, int ldb, int bc)
-{ return trmm_core(h, 's', side, uplo, trans, diag, m, n, alpha, mat_ptrs(A, lda), 0, mat_ptrs(B, ldb), 0, 0, bc); }
+{ return trmm_core(h, 's', side, uplo, trans, diag, m, n, RS(alpha), mat_ptrs(A, lda), 0, mat_ptrs(B, ldb), 0, 0, bc); }
 int kblasDtrmm_batch_strided(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m,
                              const int n, const double alpha, const double* A, int lda, long sA, double* B,
                              int ldb, long sB, int bc)
-{ return trmm_core(h, 'd', side, uplo, trans, diag, m, n, alpha, mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB, 1, bc); }
+{ return trmm_core(h, 'd', side, uplo, trans, diag, m, n, RS(alpha), mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB, 1, bc); }
 int kblasStrmm_batch_strided(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m,
                              const int n, const float alpha, const float* A, int lda, long sA, float* B,
                              int ldb, long sB, int bc)
-{ return trmm_core(h, 's', side, uplo, trans, diag, m, n, alpha, mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB, 1, bc); }
+{ return trmm_core(h, 's', side, uplo, trans, diag, m, n, RS(alpha), mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB, 1, bc); }
 
 /* ================================ PSTRF ==================================== */
 /* src/batch_svd/batch_pstrf.cu:160-223: n > 64 returns KBLAS_Error_WrongInput;
@@ -712,49 +719,98 @@ void kblas_poti_batch_strided_wsquery(kblasHandle_t h, const int n, int bc) { (v
 void kblas_trmm_batch_wsquery(kblasHandle_t h, char side, int m, int n, int bc) { (void)h; (void)side; (void)m; (void)n; (void)bc; }
 void kblas_trmm_batch_strided_wsquery(kblasHandle_t h, char side, int m, int n, int bc) { (void)h; (void)side; (void)m; (void)n; (void)bc; }
 
-/* ============ complex precisions: link-compatible, not implemented ======= */
+/* ========================= complex precisions ============================ */
+/* The C and Z entry points run the same kernel templates instantiated for
+ * the complex types (the reference instantiates its templates for them too,
+ * src/batch_triangular/Makefile:30-57): complex symmetric arithmetic as the
+ * reference's operators define it, 'C' a conjugate transpose (GEMM, TRSM,
+ * TRMM).  POTRF / POTRS / POSV run the generic Cholesky kernels. */
 #define KB_NI return KBLAS_NotImplemented
-int kblasCgemm_batch(kblasHandle_t h, char ta, char tb, const int m, const int n, const int k, const hipFloatComplex alpha, const hipFloatComplex** A, int lda, const hipFloatComplex** B, int ldb, const hipFloatComplex beta, hipFloatComplex** C, int ldc, int bc) { KB_NI; }
-int kblasZgemm_batch(kblasHandle_t h, char ta, char tb, const int m, const int n, const int k, const hipDoubleComplex alpha, const hipDoubleComplex** A, int lda, const hipDoubleComplex** B, int ldb, const hipDoubleComplex beta, hipDoubleComplex** C, int ldc, int bc) { KB_NI; }
-int kblasCgemm_batch_strided(kblasHandle_t h, char ta, char tb, const int m, const int n, const int k, const hipFloatComplex alpha, const hipFloatComplex* A, int lda, long sA, const hipFloatComplex* B, int ldb, long sB, const hipFloatComplex beta, hipFloatComplex* C, int ldc, long sC, int bc) { KB_NI; }
-int kblasZgemm_batch_strided(kblasHandle_t h, char ta, char tb, const int m, const int n, const int k, const hipDoubleComplex alpha, const hipDoubleComplex* A, int lda, long sA, const hipDoubleComplex* B, int ldb, long sB, const hipDoubleComplex beta, hipDoubleComplex* C, int ldc, long sC, int bc) { KB_NI; }
-int kblasCsyrk_batch(kblasHandle_t h, char uplo, char trans, const int m, const int n, const hipFloatComplex alpha, const hipFloatComplex** A, int lda, const hipFloatComplex beta, hipFloatComplex** B, int ldb, int bc) { KB_NI; }
-int kblasZsyrk_batch(kblasHandle_t h, char uplo, char trans, const int m, const int n, const hipDoubleComplex alpha, const hipDoubleComplex** A, int lda, const hipDoubleComplex beta, hipDoubleComplex** B, int ldb, int bc) { KB_NI; }
-int kblasCsyrk_batch_strided(kblasHandle_t h, char uplo, char trans, const int m, const int n, const hipFloatComplex alpha, const hipFloatComplex* A, int lda, long sA, const hipFloatComplex beta, hipFloatComplex* B, int ldb, long sB, int bc) { KB_NI; }
-int kblasZsyrk_batch_strided(kblasHandle_t h, char uplo, char trans, const int m, const int n, const hipDoubleComplex alpha, const hipDoubleComplex* A, int lda, long sA, const hipDoubleComplex beta, hipDoubleComplex* B, int ldb, long sB, int bc) { KB_NI; }
-int kblasCtrsm_batch(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n, const hipFloatComplex alpha, const hipFloatComplex** A, int lda, hipFloatComplex** B, int ldb, int bc) { KB_NI; }
-int kblasZtrsm_batch(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n, const hipDoubleComplex alpha, const hipDoubleComplex** A, int lda, hipDoubleComplex** B, int ldb, int bc) { KB_NI; }
-int kblasCtrsm_batch_strided(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n, const hipFloatComplex alpha, const hipFloatComplex* A, int lda, long sA, hipFloatComplex* B, int ldb, long sB, int bc) { KB_NI; }
-int kblasZtrsm_batch_strided(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n, const hipDoubleComplex alpha, const hipDoubleComplex* A, int lda, long sA, hipDoubleComplex* B, int ldb, long sB, int bc) { KB_NI; }
-int kblasCpotrf_batch(kblasHandle_t h, char uplo, const int n, hipFloatComplex** A, int lda, int bc, int* info) { KB_NI; }
-int kblasZpotrf_batch(kblasHandle_t h, char uplo, const int n, hipDoubleComplex** A, int lda, int bc, int* info) { KB_NI; }
-int kblasCpotrf_batch_strided(kblasHandle_t h, char uplo, const int n, hipFloatComplex* A, int lda, long sA, int bc, int* info) { KB_NI; }
-int kblasZpotrf_batch_strided(kblasHandle_t h, char uplo, const int n, hipDoubleComplex* A, int lda, long sA, int bc, int* info) { KB_NI; }
-int kblasCpotrs_batch(kblasHandle_t h, char side, char uplo, const int m, const int n, const hipFloatComplex** A, int lda, hipFloatComplex** B, int ldb, int bc) { KB_NI; }
-int kblasZpotrs_batch(kblasHandle_t h, char side, char uplo, const int m, const int n, const hipDoubleComplex** A, int lda, hipDoubleComplex** B, int ldb, int bc) { KB_NI; }
-int kblasCpotrs_batch_strided(kblasHandle_t h, char side, char uplo, const int m, const int n, const hipFloatComplex* A, int lda, long sA, hipFloatComplex* B, int ldb, long sB, int bc) { KB_NI; }
-int kblasZpotrs_batch_strided(kblasHandle_t h, char side, char uplo, const int m, const int n, const hipDoubleComplex* A, int lda, long sA, hipDoubleComplex* B, int ldb, long sB, int bc) { KB_NI; }
-int kblasCposv_batch(kblasHandle_t h, char side, char uplo, const int m, const int n, hipFloatComplex** A, int lda, hipFloatComplex** B, int ldb, int bc, int* info) { KB_NI; }
-int kblasZposv_batch(kblasHandle_t h, char side, char uplo, const int m, const int n, hipDoubleComplex** A, int lda, hipDoubleComplex** B, int ldb, int bc, int* info) { KB_NI; }
-int kblasCposv_batch_strided(kblasHandle_t h, char side, char uplo, const int m, const int n, hipFloatComplex* A, int lda, long sA, hipFloatComplex* B, int ldb, long sB, int bc, int* info) { KB_NI; }
-int kblasZposv_batch_strided(kblasHandle_t h, char side, char uplo, const int m, const int n, hipDoubleComplex* A, int lda, long sA, hipDoubleComplex* B, int ldb, long sB, int bc, int* info) { KB_NI; }
-int kblasClauum_batch(kblasHandle_t h, char uplo, const int n, hipFloatComplex** A, int lda, int bc, int* info) { KB_NI; }
-int kblasClauum_batch_strided(kblasHandle_t h, char uplo, const int n, hipFloatComplex* A, int lda, long sA, int bc, int* info) { KB_NI; }
-int kblasCpotri_batch(kblasHandle_t h, char uplo, const int n, hipFloatComplex** A, int lda, int bc, int* info) { KB_NI; }
-int kblasCpotri_batch_strided(kblasHandle_t h, char uplo, const int n, hipFloatComplex* A, int lda, long sA, int bc, int* info) { KB_NI; }
-int kblasCpoti_batch(kblasHandle_t h, char uplo, const int n, hipFloatComplex** A, int lda, int bc, int* info) { KB_NI; }
-int kblasCpoti_batch_strided(kblasHandle_t h, char uplo, const int n, hipFloatComplex* A, int lda, long sA, int bc, int* info) { KB_NI; }
-int kblasCtrtri_batch(kblasHandle_t h, char uplo, char diag, const int n, hipFloatComplex** A, int lda, int bc, int* info) { KB_NI; }
-int kblasCtrtri_batch_strided(kblasHandle_t h, char uplo, char diag, const int n, hipFloatComplex* A, int lda, long sA, int bc, int* info) { KB_NI; }
-int kblasCtrmm_batch(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n, const hipFloatComplex alpha, const hipFloatComplex** A, int lda, hipFloatComplex** B, int ldb, int bc) { KB_NI; }
-int kblasCtrmm_batch_strided(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n, const hipFloatComplex alpha, const hipFloatComplex* A, int lda, long sA, hipFloatComplex* B, int ldb, long sB, int bc) { KB_NI; }
-int kblasZlauum_batch(kblasHandle_t h, char uplo, const int n, hipDoubleComplex** A, int lda, int bc, int* info) { KB_NI; }
-int kblasZlauum_batch_strided(kblasHandle_t h, char uplo, const int n, hipDoubleComplex* A, int lda, long sA, int bc, int* info) { KB_NI; }
-int kblasZpotri_batch(kblasHandle_t h, char uplo, const int n, hipDoubleComplex** A, int lda, int bc, int* info) { KB_NI; }
-int kblasZpotri_batch_strided(kblasHandle_t h, char uplo, const int n, hipDoubleComplex* A, int lda, long sA, int bc, int* info) { KB_NI; }
-int kblasZpoti_batch(kblasHandle_t h, char uplo, const int n, hipDoubleComplex** A, int lda, int bc, int* info) { KB_NI; }
-int kblasZpoti_batch_strided(kblasHandle_t h, char uplo, const int n, hipDoubleComplex* A, int lda, long sA, int bc, int* info) { KB_NI; }
-int kblasZtrtri_batch(kblasHandle_t h, char uplo, char diag, const int n, hipDoubleComplex** A, int lda, int bc, int* info) { KB_NI; }
-int kblasZtrtri_batch_strided(kblasHandle_t h, char uplo, char diag, const int n, hipDoubleComplex* A, int lda, long sA, int bc, int* info) { KB_NI; }
-int kblasZtrmm_batch(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n, const hipDoubleComplex alpha, const hipDoubleComplex** A, int lda, hipDoubleComplex** B, int ldb, int bc) { KB_NI; }
-int kblasZtrmm_batch_strided(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n, const hipDoubleComplex alpha, const hipDoubleComplex* A, int lda, long sA, hipDoubleComplex* B, int ldb, long sB, int bc) { KB_NI; }
+int kblasCgemm_batch(kblasHandle_t h, char ta, char tb, const int m, const int n, const int k, const hipFloatComplex alpha, const hipFloatComplex** A, int lda, const hipFloatComplex** B, int ldb, const hipFloatComplex beta, hipFloatComplex** C, int ldc, int bc)
+{ return gemm_core(h, 'c', ta, tb, m, n, k, CS(alpha), mat_ptrs(A, lda), 0, mat_ptrs(B, ldb), 0, CS(beta), mat_ptrs(C, ldc), 0, 0, bc); }
+int kblasZgemm_batch(kblasHandle_t h, char ta, char tb, const int m, const int n, const int k, const hipDoubleComplex alpha, const hipDoubleComplex** A, int lda, const hipDoubleComplex** B, int ldb, const hipDoubleComplex beta, hipDoubleComplex** C, int ldc, int bc)
+{ return gemm_core(h, 'z', ta, tb, m, n, k, CS(alpha), mat_ptrs(A, lda), 0, mat_ptrs(B, ldb), 0, CS(beta), mat_ptrs(C, ldc), 0, 0, bc); }
+int kblasCgemm_batch_strided(kblasHandle_t h, char ta, char tb, const int m, const int n, const int k, const hipFloatComplex alpha, const hipFloatComplex* A, int lda, long sA, const hipFloatComplex* B, int ldb, long sB, const hipFloatComplex beta, hipFloatComplex* C, int ldc, long sC, int bc)
+{ return gemm_core(h, 'c', ta, tb, m, n, k, CS(alpha), mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB, CS(beta), mat_strided(C, ldc, sC), sC, 1, bc); }
+int kblasZgemm_batch_strided(kblasHandle_t h, char ta, char tb, const int m, const int n, const int k, const hipDoubleComplex alpha, const hipDoubleComplex* A, int lda, long sA, const hipDoubleComplex* B, int ldb, long sB, const hipDoubleComplex beta, hipDoubleComplex* C, int ldc, long sC, int bc)
+{ return gemm_core(h, 'z', ta, tb, m, n, k, CS(alpha), mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB, CS(beta), mat_strided(C, ldc, sC), sC, 1, bc); }
+int kblasCsyrk_batch(kblasHandle_t h, char uplo, char trans, const int m, const int n, const hipFloatComplex alpha, const hipFloatComplex** A, int lda, const hipFloatComplex beta, hipFloatComplex** B, int ldb, int bc)
+{ return syrk_core(h, 'c', uplo, trans, m, n, CS(alpha), mat_ptrs(A, lda), 0, CS(beta), mat_ptrs(B, ldb), 0, 0, bc); }
+int kblasZsyrk_batch(kblasHandle_t h, char uplo, char trans, const int m, const int n, const hipDoubleComplex alpha, const hipDoubleComplex** A, int lda, const hipDoubleComplex beta, hipDoubleComplex** B, int ldb, int bc)
+{ return syrk_core(h, 'z', uplo, trans, m, n, CS(alpha), mat_ptrs(A, lda), 0, CS(beta), mat_ptrs(B, ldb), 0, 0, bc); }
+int kblasCsyrk_batch_strided(kblasHandle_t h, char uplo, char trans, const int m, const int n, const hipFloatComplex alpha, const hipFloatComplex* A, int lda, long sA, const hipFloatComplex beta, hipFloatComplex* B, int ldb, long sB, int bc)
+{ return syrk_core(h, 'c', uplo, trans, m, n, CS(alpha), mat_strided(A, lda, sA), sA, CS(beta), mat_strided(B, ldb, sB), sB, 1, bc); }
+int kblasZsyrk_batch_strided(kblasHandle_t h, char uplo, char trans, const int m, const int n, const hipDoubleComplex alpha, const hipDoubleComplex* A, int lda, long sA, const hipDoubleComplex beta, hipDoubleComplex* B, int ldb, long sB, int bc)
+{ return syrk_core(h, 'z', uplo, trans, m, n, CS(alpha), mat_strided(A, lda, sA), sA, CS(beta), mat_strided(B, ldb, sB), sB, 1, bc); }
+int kblasCtrsm_batch(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n, const hipFloatComplex alpha, const hipFloatComplex** A, int lda, hipFloatComplex** B, int ldb, int bc)
+{ return trsm_core(h, 'c', side, uplo, trans, diag, m, n, CS(alpha), mat_ptrs(A, lda), 0, mat_ptrs(B, ldb), 0, 0, bc); }
+int kblasZtrsm_batch(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n, const hipDoubleComplex alpha, const hipDoubleComplex** A, int lda, hipDoubleComplex** B, int ldb, int bc)
+{ return trsm_core(h, 'z', side, uplo, trans, diag, m, n, CS(alpha), mat_ptrs(A, lda), 0, mat_ptrs(B, ldb), 0, 0, bc); }
+int kblasCtrsm_batch_strided(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n, const hipFloatComplex alpha, const hipFloatComplex* A, int lda, long sA, hipFloatComplex* B, int ldb, long sB, int bc)
+{ return trsm_core(h, 'c', side, uplo, trans, diag, m, n, CS(alpha), mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB, 1, bc); }
+int kblasZtrsm_batch_strided(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n, const hipDoubleComplex alpha, const hipDoubleComplex* A, int lda, long sA, hipDoubleComplex* B, int ldb, long sB, int bc)
+{ return trsm_core(h, 'z', side, uplo, trans, diag, m, n, CS(alpha), mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB, 1, bc); }
+int kblasCpotrf_batch(kblasHandle_t h, char uplo, const int n, hipFloatComplex** A, int lda, int bc, int* info)
+{ return potrf_core(h, 'c', uplo, n, mat_ptrs(A, lda), 0, 0, bc, info); }
+int kblasZpotrf_batch(kblasHandle_t h, char uplo, const int n, hipDoubleComplex** A, int lda, int bc, int* info)
+{ return potrf_core(h, 'z', uplo, n, mat_ptrs(A, lda), 0, 0, bc, info); }
+int kblasCpotrf_batch_strided(kblasHandle_t h, char uplo, const int n, hipFloatComplex* A, int lda, long sA, int bc, int* info)
+{ return potrf_core(h, 'c', uplo, n, mat_strided(A, lda, sA), sA, 1, bc, info); }
+int kblasZpotrf_batch_strided(kblasHandle_t h, char uplo, const int n, hipDoubleComplex* A, int lda, long sA, int bc, int* info)
+{ return potrf_core(h, 'z', uplo, n, mat_strided(A, lda, sA), sA, 1, bc, info); }
+int kblasCpotrs_batch(kblasHandle_t h, char side, char uplo, const int m, const int n, const hipFloatComplex** A, int lda, hipFloatComplex** B, int ldb, int bc)
+{ return solve_core(h, 'c', KB_MODE_POTRS, side, uplo, m, n, mat_ptrs(A, lda), 0, mat_ptrs(B, ldb), 0, 0, bc, NULL); }
+int kblasZpotrs_batch(kblasHandle_t h, char side, char uplo, const int m, const int n, const hipDoubleComplex** A, int lda, hipDoubleComplex** B, int ldb, int bc)
+{ return solve_core(h, 'z', KB_MODE_POTRS, side, uplo, m, n, mat_ptrs(A, lda), 0, mat_ptrs(B, ldb), 0, 0, bc, NULL); }
+int kblasCpotrs_batch_strided(kblasHandle_t h, char side, char uplo, const int m, const int n, const hipFloatComplex* A, int lda, long sA, hipFloatComplex* B, int ldb, long sB, int bc)
+{ return solve_core(h, 'c', KB_MODE_POTRS, side, uplo, m, n, mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB, 1, bc, NULL); }
+int kblasZpotrs_batch_strided(kblasHandle_t h, char side, char uplo, const int m, const int n, const hipDoubleComplex* A, int lda, long sA, hipDoubleComplex* B, int ldb, long sB, int bc)
+{ return solve_core(h, 'z', KB_MODE_POTRS, side, uplo, m, n, mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB, 1, bc, NULL); }
+int kblasCposv_batch(kblasHandle_t h, char side, char uplo, const int m, const int n, hipFloatComplex** A, int lda, hipFloatComplex** B, int ldb, int bc, int* info)
+{ return solve_core(h, 'c', KB_MODE_POSV, side, uplo, m, n, mat_ptrs(A, lda), 0, mat_ptrs(B, ldb), 0, 0, bc, info); }
+int kblasZposv_batch(kblasHandle_t h, char side, char uplo, const int m, const int n, hipDoubleComplex** A, int lda, hipDoubleComplex** B, int ldb, int bc, int* info)
+{ return solve_core(h, 'z', KB_MODE_POSV, side, uplo, m, n, mat_ptrs(A, lda), 0, mat_ptrs(B, ldb), 0, 0, bc, info); }
+int kblasCposv_batch_strided(kblasHandle_t h, char side, char uplo, const int m, const int n, hipFloatComplex* A, int lda, long sA, hipFloatComplex* B, int ldb, long sB, int bc, int* info)
+{ return solve_core(h, 'c', KB_MODE_POSV, side, uplo, m, n, mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB, 1, bc, info); }
+int kblasZposv_batch_strided(kblasHandle_t h, char side, char uplo, const int m, const int n, hipDoubleComplex* A, int lda, long sA, hipDoubleComplex* B, int ldb, long sB, int bc, int* info)
+{ return solve_core(h, 'z', KB_MODE_POSV, side, uplo, m, n, mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB, 1, bc, info); }
+int kblasClauum_batch(kblasHandle_t h, char uplo, const int n, hipFloatComplex** A, int lda, int bc, int* info)
+{ return inv_core(h, 'c', INV_LAUUM, uplo, 'N', n, mat_ptrs(A, lda), 0, 0, bc, info); }
+int kblasClauum_batch_strided(kblasHandle_t h, char uplo, const int n, hipFloatComplex* A, int lda, long sA, int bc, int* info)
+{ return inv_core(h, 'c', INV_LAUUM, uplo, 'N', n, mat_strided(A, lda, sA), sA, 1, bc, info); }
+int kblasCpotri_batch(kblasHandle_t h, char uplo, const int n, hipFloatComplex** A, int lda, int bc, int* info)
+{ return inv_core(h, 'c', INV_POTRI, uplo, 'N', n, mat_ptrs(A, lda), 0, 0, bc, info); }
+int kblasCpotri_batch_strided(kblasHandle_t h, char uplo, const int n, hipFloatComplex* A, int lda, long sA, int bc, int* info)
+{ return inv_core(h, 'c', INV_POTRI, uplo, 'N', n, mat_strided(A, lda, sA), sA, 1, bc, info); }
+int kblasCpoti_batch(kblasHandle_t h, char uplo, const int n, hipFloatComplex** A, int lda, int bc, int* info)
+{ return inv_core(h, 'c', INV_POTI, uplo, 'N', n, mat_ptrs(A, lda), 0, 0, bc, info); }
+int kblasCpoti_batch_strided(kblasHandle_t h, char uplo, const int n, hipFloatComplex* A, int lda, long sA, int bc, int* info)
+{ return inv_core(h, 'c', INV_POTI, uplo, 'N', n, mat_strided(A, lda, sA), sA, 1, bc, info); }
+int kblasCtrtri_batch(kblasHandle_t h, char uplo, char diag, const int n, hipFloatComplex** A, int lda, int bc, int* info)
+{ return inv_core(h, 'c', INV_TRTRI, uplo, diag, n, mat_ptrs(A, lda), 0, 0, bc, info); }
+int kblasCtrtri_batch_strided(kblasHandle_t h, char uplo, char diag, const int n, hipFloatComplex* A, int lda, long sA, int bc, int* info)
+{ return inv_core(h, 'c', INV_TRTRI, uplo, diag, n, mat_strided(A, lda, sA), sA, 1, bc, info); }
+int kblasCtrmm_batch(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n, const hipFloatComplex alpha, const hipFloatComplex** A, int lda, hipFloatComplex** B, int ldb, int bc)
+{ return trmm_core(h, 'c', side, uplo, trans, diag, m, n, CS(alpha), mat_ptrs(A, lda), 0, mat_ptrs(B, ldb), 0, 0, bc); }
+int kblasCtrmm_batch_strided(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n, const hipFloatComplex alpha, const hipFloatComplex* A, int lda, long sA, hipFloatComplex* B, int ldb, long sB, int bc)
+{ return trmm_core(h, 'c', side, uplo, trans, diag, m, n, CS(alpha), mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB, 1, bc); }
+int kblasZlauum_batch(kblasHandle_t h, char uplo, const int n, hipDoubleComplex** A, int lda, int bc, int* info)
+{ return inv_core(h, 'z', INV_LAUUM, uplo, 'N', n, mat_ptrs(A, lda), 0, 0, bc, info); }
+int kblasZlauum_batch_strided(kblasHandle_t h, char uplo, const int n, hipDoubleComplex* A, int lda, long sA, int bc, int* info)
+{ return inv_core(h, 'z', INV_LAUUM, uplo, 'N', n, mat_strided(A, lda, sA), sA, 1, bc, info); }
+int kblasZpotri_batch(kblasHandle_t h, char uplo, const int n, hipDoubleComplex** A, int lda, int bc, int* info)
+{ return inv_core(h, 'z', INV_POTRI, uplo, 'N', n, mat_ptrs(A, lda), 0, 0, bc, info); }
+int kblasZpotri_batch_strided(kblasHandle_t h, char uplo, const int n, hipDoubleComplex* A, int lda, long sA, int bc, int* info)
+{ return inv_core(h, 'z', INV_POTRI, uplo, 'N', n, mat_strided(A, lda, sA), sA, 1, bc, info); }
+int kblasZpoti_batch(kblasHandle_t h, char uplo, const int n, hipDoubleComplex** A, int lda, int bc, int* info)
+{ return inv_core(h, 'z', INV_POTI, uplo, 'N', n, mat_ptrs(A, lda), 0, 0, bc, info); }
+int kblasZpoti_batch_strided(kblasHandle_t h, char uplo, const int n, hipDoubleComplex* A, int lda, long sA, int bc, int* info)
+{ return inv_core(h, 'z', INV_POTI, uplo, 'N', n, mat_strided(A, lda, sA), sA, 1, bc, info); }
+int kblasZtrtri_batch(kblasHandle_t h, char uplo, char diag, const int n, hipDoubleComplex** A, int lda, int bc, int* info)
+{ return inv_core(h, 'z', INV_TRTRI, uplo, diag, n, mat_ptrs(A, lda), 0, 0, bc, info); }
+int kblasZtrtri_batch_strided(kblasHandle_t h, char uplo, char diag, const int n, hipDoubleComplex* A, int lda, long sA, int bc, int* info)
+{ return inv_core(h, 'z', INV_TRTRI, uplo, diag, n, mat_strided(A, lda, sA), sA, 1, bc, info); }
+int kblasZtrmm_batch(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n, const hipDoubleComplex alpha, const hipDoubleComplex** A, int lda, hipDoubleComplex** B, int ldb, int bc)
+{ return trmm_core(h, 'z', side, uplo, trans, diag, m, n, CS(alpha), mat_ptrs(A, lda), 0, mat_ptrs(B, ldb), 0, 0, bc); }
+int kblasZtrmm_batch_strided(kblasHandle_t h, char side, char uplo, char trans, char diag, const int m, const int n, const hipDoubleComplex alpha, const hipDoubleComplex* A, int lda, long sA, hipDoubleComplex* B, int ldb, long sB, int bc)
+{ return trmm_core(h, 'z', side, uplo, trans, diag, m, n, CS(alpha), mat_strided(A, lda, sA), sA, mat_strided(B, ldb, sB), sB, 1, bc); }

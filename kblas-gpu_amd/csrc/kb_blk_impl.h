@@ -32,12 +32,20 @@ constexpr int KB_BLK_WG = 256;
 constexpr int KB_BLK_TILE = 64;
 constexpr int KB_BLK_RC = 64;   // right-hand sides per pass of the solve
 
+// The trailing update's staged slice is never live at the same time as the
+// diagonal block, the panel transposes or the solve's staging, so they share
+// its space (33 KB for FP64 instead of 44 KB).
 template <typename T>
 struct BlkSmem {
-    T sD[KB_BLK_NBK * (KB_BLK_NBK + 1)];
+    union {
+        T sB[KB_BLK_TILE * (KB_BLK_KW + 1)];   // a 64-row slice of the panels (trailing update)
+        struct {
+            T sY[KB_BLK_RC * (KB_BLK_NBK + 1)];   // the solve's staging of a block of the solution
+            T sD[KB_BLK_NBK * (KB_BLK_NBK + 1)];  // the diagonal block
+            T tpan[4][16][4];                     // panel transposes of the diagonal-block factorization
+        };
+    };
     T sInv[KB_BLK_NBK];
-    T tpan[4][16][4];   // panel transposes of the diagonal-block factorization
-    T sB[KB_BLK_TILE * (KB_BLK_KW + 1)];   // a 64-row slice of the panels; the solve's staging
     int flag;
 };
 
@@ -411,7 +419,7 @@ KB_DEV void blk_fwd_step(const T* L, int lda, int n, int k0, int kb, T* B, int l
                          BlkSmem<T>& S)
 {
     constexpr int RC = KB_BLK_RC;
-    T* sY = S.sB;   // RC x (NBK + 1) staging of the block's solution
+    T* sY = S.sY;   // RC x (NBK + 1) staging of the block's solution
     for (int c0 = 0; c0 < nrhs; c0 += RC) {
         const int rc = min(RC, nrhs - c0);
         const int ncb = (rc + 15) / 16;
@@ -444,7 +452,7 @@ template <typename T>
 KB_DEV void blk_solve_bwd(const T* L, int lda, int n, T* B, int ldb, int side_left, int nrhs, BlkSmem<T>& S)
 {
     constexpr int NBK = KB_BLK_NBK, RC = KB_BLK_RC;
-    T* sY = S.sB;
+    T* sY = S.sY;
     T dnext[KB_BLK_DPT];
     const int nblk = (n + NBK - 1) / NBK;
     blk_diag_fetch<T>(dnext, L, lda, (nblk - 1) * NBK, n - (nblk - 1) * NBK);

@@ -22,8 +22,14 @@
 // left side, whole rows for the right side) are copied to LDS, then every
 // thread forms output elements from the copies and writes them to B, so the
 // in-place update needs no ordering between elements.
+#include "kb_cplx.h"
 #include "kb_device.h"
 #include "kb_launch.h"
+
+// The kernels are templates over the element type; the complex (C, Z)
+// entry points instantiate them with the types of kb_cplx.h (arithmetic
+// without conjugation, as the reference's complex instantiations: LAUUM is
+// L^T L, POTRI (L L^T)^-1; TRMM's trans 2 conjugates the triangle).
 
 #define KB_INV_LDS (60 * 1024)    // k_inv: dynamic part (a few static bytes beside it)
 #define KB_TRMM_LDS (64 * 1024)
@@ -218,9 +224,16 @@ k_inv_reg(int op, int unit, int n, BatchArg<T> A, int* info, const int* skip, in
 // Element (r, c) of alpha op(L) B (left, L of order m) or alpha B op(L)
 // (right, L of order n), read from the panel copy P (ldp rows) of B.
 template <typename T>
-KB_DEV T trmm_elem(int side_left, int trans, int unit, const T* L, int ldl, int ord, const T* P, int ldp,
+KB_DEV T trmm_elem(int side_left, int trans, int unit, const T* L0, int ldl, int ord, const T* P, int ldp,
                    int r, int c)
 {
+    // trans 2: the conjugate of the triangle, read through this view
+    struct View {
+        const T* p;
+        bool cj;
+        KB_DEV T operator[](long i) const { return cj ? kb_conj(p[i]) : p[i]; }
+    };
+    const View L{L0, trans == 2};
     T s = T(0);
     if (side_left) {
         const T* pc = P + (long)c * ldp;
@@ -229,14 +242,14 @@ KB_DEV T trmm_elem(int side_left, int trans, int unit, const T* L, int ldl, int 
             for (int k = 0; k < r; ++k) s = fma(L[r + (long)k * ldl], pc[k], s);
             s = unit ? s + pc[r] : fma(L[r + (long)r * ldl], pc[r], s);
         } else {        // sum_{k>=r} L(k,r) B(k,c)
-            const T* lr = L + (long)r * ldl;
+            const View lr{L0 + (long)r * ldl, trans == 2};
             s = unit ? pc[r] : lr[r] * pc[r];
 #pragma unroll 8
             for (int k = r + 1; k < ord; ++k) s = fma(lr[k], pc[k], s);
         }
     } else {
         if (!trans) {   // sum_{k>=c} B(r,k) L(k,c)
-            const T* lc = L + (long)c * ldl;
+            const View lc{L0 + (long)c * ldl, false};
             s = unit ? P[r + (long)c * ldp] : lc[c] * P[r + (long)c * ldp];
 #pragma unroll 8
             for (int k = c + 1; k < ord; ++k) s = fma(P[r + (long)k * ldp], lc[k], s);
@@ -299,20 +312,27 @@ template <typename T>
 static int launch_inv(int op, int unit, int n, kb_mat_t A, int* info, const int* skip, int batch, hipStream_t s)
 {
     if (n > KB_INV_MAX_N) return (int)hipErrorNotSupported;
-    if (n <= 16) {
-        hipLaunchKernelGGL((k_inv_reg<T, 16>), dim3((batch + 3) / 4), dim3(64), 0, s, op, unit, n, to_arg<T>(A), info,
-                           skip, batch);
-        return (int)hipGetLastError();
+    // register form: real types up to 64, single complex up to 16 (above, its
+    // rows would not fit in registers), double complex never
+    constexpr int REG_MAX = __is_same(T, kb_cd) ? 0 : __is_same(T, kb_cf) ? 16 : 64;
+    if constexpr (REG_MAX >= 16) {
+        if (n <= 16) {
+            hipLaunchKernelGGL((k_inv_reg<T, 16>), dim3((batch + 3) / 4), dim3(64), 0, s, op, unit, n, to_arg<T>(A),
+                               info, skip, batch);
+            return (int)hipGetLastError();
+        }
     }
-    if (n <= 32) {
-        hipLaunchKernelGGL((k_inv_reg<T, 32>), dim3((batch + 1) / 2), dim3(64), 0, s, op, unit, n, to_arg<T>(A), info,
-                           skip, batch);
-        return (int)hipGetLastError();
-    }
-    if (n <= 64) {
-        hipLaunchKernelGGL((k_inv_reg<T, 64>), dim3(batch), dim3(64), 0, s, op, unit, n, to_arg<T>(A), info, skip,
-                           batch);
-        return (int)hipGetLastError();
+    if constexpr (REG_MAX >= 64) {
+        if (n <= 32) {
+            hipLaunchKernelGGL((k_inv_reg<T, 32>), dim3((batch + 1) / 2), dim3(64), 0, s, op, unit, n, to_arg<T>(A),
+                               info, skip, batch);
+            return (int)hipGetLastError();
+        }
+        if (n <= 64) {
+            hipLaunchKernelGGL((k_inv_reg<T, 64>), dim3(batch), dim3(64), 0, s, op, unit, n, to_arg<T>(A), info, skip,
+                               batch);
+            return (int)hipGetLastError();
+        }
     }
     const size_t need = (size_t)n * (n + 1) * sizeof(T);
     const int staged = need <= KB_INV_LDS;
@@ -323,7 +343,7 @@ static int launch_inv(int op, int unit, int n, kb_mat_t A, int* info, const int*
 }
 
 template <typename T>
-static int launch_trmm(int side_left, int trans, int unit, int m, int n, double alpha, kb_mat_t A, kb_mat_t B,
+static int launch_trmm(int side_left, int trans, int unit, int m, int n, T alpha, kb_mat_t A, kb_mat_t B,
                        int batch, hipStream_t s)
 {
     const int ord = side_left ? m : n;
@@ -336,7 +356,7 @@ static int launch_trmm(int side_left, int trans, int unit, int m, int n, double 
     long panel = (long)(left / line);
     if (panel > total) panel = total;
     const size_t bytes = (stageA ? tri : 0) + (size_t)panel * line;
-    hipLaunchKernelGGL(k_trmm<T>, dim3(batch), dim3(256), bytes, s, side_left, trans, unit, m, n, (T)alpha,
+    hipLaunchKernelGGL(k_trmm<T>, dim3(batch), dim3(256), bytes, s, side_left, trans, unit, m, n, alpha,
                        to_arg<T>(A), to_arg<T>(B), stageA, (int)panel, batch);
     return (int)hipGetLastError();
 }
@@ -344,13 +364,25 @@ static int launch_trmm(int side_left, int trans, int unit, int m, int n, double 
 extern "C" int kb_launch_inv(char prec, int op, int unit, int n, kb_mat_t A, int* info, const int* skip, int batch,
                              hipStream_t s)
 {
-    return prec == 'd' ? launch_inv<double>(op, unit, n, A, info, skip, batch, s)
-                       : launch_inv<float>(op, unit, n, A, info, skip, batch, s);
+    switch (prec) {
+    case 'd': return launch_inv<double>(op, unit, n, A, info, skip, batch, s);
+    case 's': return launch_inv<float>(op, unit, n, A, info, skip, batch, s);
+    case 'z': return launch_inv<kb_cd>(op, unit, n, A, info, skip, batch, s);
+    case 'c': return launch_inv<kb_cf>(op, unit, n, A, info, skip, batch, s);
+    }
+    return (int)hipErrorInvalidValue;
 }
 
-extern "C" int kb_launch_trmm(char prec, int side_left, int trans, int unit, int m, int n, double alpha,
+// alpha as a (real, imaginary) pair; trans 0 / 1 / 2 = none / transpose / conjugate transpose
+extern "C" int kb_launch_trmm(char prec, int side_left, int trans, int unit, int m, int n, const double* alpha,
                               kb_mat_t A, kb_mat_t B, int batch, hipStream_t s)
 {
-    return prec == 'd' ? launch_trmm<double>(side_left, trans, unit, m, n, alpha, A, B, batch, s)
-                       : launch_trmm<float>(side_left, trans, unit, m, n, alpha, A, B, batch, s);
+    switch (prec) {
+    case 'd': return launch_trmm<double>(side_left, trans, unit, m, n, alpha[0], A, B, batch, s);
+    case 's': return launch_trmm<float>(side_left, trans, unit, m, n, (float)alpha[0], A, B, batch, s);
+    case 'z': return launch_trmm<kb_cd>(side_left, trans, unit, m, n, kb_cd(alpha[0], alpha[1]), A, B, batch, s);
+    case 'c': return launch_trmm<kb_cf>(side_left, trans, unit, m, n, kb_cf((float)alpha[0], (float)alpha[1]), A, B,
+                                        batch, s);
+    }
+    return (int)hipErrorInvalidValue;
 }

@@ -120,17 +120,33 @@ int kb_launch_vsposv(const int* vm, const int* vn, int side_left, kb_mat_t A, kb
 enum { KB_INV_OP_TRTRI = 0, KB_INV_OP_LAUUM = 1, KB_INV_OP_POTRI = 2 };
 int kb_launch_inv(char prec, int op, int unit, int n, kb_mat_t A, int* info, const int* skip, int batch,
                   hipStream_t s);
-/* Batched lower TRMM: B := alpha op(L) B (side_left, L of order m) or
+/* Batched lower TRMM (prec s, d, c, z; trans 2 conjugates L): B := alpha op(L) B (side_left, L of order m) or
  * B := alpha B op(L) (L of order n); B is m x n.  One line of B (a column
  * for side_left, else a row) is staged in 64 KB of LDS, so the order is at
  * most 16384 (FP32) / 8192 (FP64) (hipErrorNotSupported above). */
-int kb_launch_trmm(char prec, int side_left, int trans, int unit, int m, int n, double alpha,
+int kb_launch_trmm(char prec, int side_left, int trans, int unit, int m, int n, const double* alpha,
                    kb_mat_t A, kb_mat_t B, int batch, hipStream_t s);
 
 /* Batched pivoted Cholesky P^T A P = R^T R (upper), n <= 64 (hipErrorNotSupported
  * above).  Pivots: pivp[b] (pointer array) or pivs + b * spiv (strided), 0-based. */
 int kb_launch_pstrf(char prec, int n, kb_mat_t A, int* const* pivp, int* pivs, long spiv, int* ranks,
                     int batch, hipStream_t s);
+
+/* Any precision ('s', 'd', 'c', 'z'): GEMM / SYRK / TRSM with scalars as
+ * (real, imaginary) pairs, op 0 / 1 / 2 = none / transpose / conjugate
+ * transpose, either triangle (upper), TRSM with a unit diagonal or not. */
+int kb_launch_gemm_x(char prec, int transA, int transB, int m, int n, int k, const double* alpha,
+                     kb_mat_t A, kb_mat_t B, const double* beta, kb_mat_t C, const int* vm,
+                     const int* vn, const int* vk, int batch, hipStream_t s);
+int kb_launch_syrk_x(char prec, int upper, int trans, int m, int k, const double* alpha, kb_mat_t A,
+                     const double* beta, kb_mat_t B, const int* vm, const int* vk, int batch, hipStream_t s);
+int kb_launch_trsm_x(char prec, int side_left, int upper, int trans, int unit, int m, int n,
+                     const double* alpha, kb_mat_t A, kb_mat_t B, const int* vm, const int* vn, int batch,
+                     hipStream_t s);
+/* Generic POTRF / POTRS / POSV (mode KB_MODE_*) for any precision and either
+ * triangle; info per matrix for POTRF / POSV. */
+int kb_launch_gchol(char prec, int mode, int upper, int n, int nrhs, int side_left, kb_mat_t A, kb_mat_t B,
+                    int* info, int batch, hipStream_t s);
 
 /* Randomized-range-finder SPD helpers (all operands per matrix through device
  * arrays).  Rank-revealing Cholesky of dims[b] x dims[b] Gram matrices

@@ -2,8 +2,10 @@
 #pragma once
 #include "kb_rg_impl.h"
 
-// four wavefronts (16 matrices) per workgroup
-constexpr int KB_RG_WPB = 4;
+// one wavefront (four matrices) per workgroup: the most even spread of a
+// batch over the CUs (the n = 16 POSV of 100,000 systems ran 4 % faster than
+// with four wavefronts per workgroup, the n = 32 POTRF the same)
+constexpr int KB_RG_WPB = 1;
 
 template <typename T, int NB, int MODE, int RC>
 static int rg_launch(int n, int nrhs, int side_left, kb_mat_t A, kb_mat_t B, int* info, int batch,

@@ -122,7 +122,7 @@ KB_DEV void sb_residual(const double* A, int lda, int n, const double* B, int ld
     const int li = lane & 15, lg = lane >> 4;
     const int ncb = (nrhs + 15) / 16;
     const int nb = (n + 15) / 16;
-    double* sY = S.d.sB;
+    double* sY = S.d.sB;   // 64 x 33: within the slice area, nothing else is live
     for (int rb0 = 0; rb0 < nb; rb0 += KB_BLK_WG / 64) {
         const int r0 = 16 * (rb0 + wave);
         kb_d4 acc[4];

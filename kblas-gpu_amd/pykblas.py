@@ -124,7 +124,31 @@ _TSIG["pstrf_batch"] = (c_int, [P, c_int, P, c_int, P, P, c_int])
 _TSIG["pstrf_batch_strided"] = (c_int, [P, c_int, P, c_int, c_int, P, c_int, P, c_int])
 
 
+class CF(ctypes.Structure):
+    """hipFloatComplex passed by value"""
+    _fields_ = [("x", c_float), ("y", c_float)]
+
+
+class CD(ctypes.Structure):
+    """hipDoubleComplex passed by value"""
+    _fields_ = [("x", c_double), ("y", c_double)]
+
+
+# complex entry points with the real signatures above, T = the complex scalar
+_CPLX = ("gemm_batch", "gemm_batch_strided", "syrk_batch", "syrk_batch_strided", "trsm_batch", "trsm_batch_strided",
+         "potrf_batch", "potrf_batch_strided", "potrs_batch", "potrs_batch_strided", "posv_batch",
+         "posv_batch_strided", "trmm_batch", "trmm_batch_strided", "trtri_batch", "trtri_batch_strided",
+         "lauum_batch", "lauum_batch_strided", "potri_batch", "potri_batch_strided", "poti_batch",
+         "poti_batch_strided")
+
+
 def _declare(lib):
+    for suffix in _CPLX:
+        res, args = _TSIG[suffix]
+        for prec, t in (("C", CF), ("Z", CD)):
+            f = getattr(lib, "kblas%s%s" % (prec, suffix))
+            f.restype = res
+            f.argtypes = [t if a == "T" else a for a in args]
     for name, (res, args) in _SIG.items():
         f = getattr(lib, name)
         f.restype = res

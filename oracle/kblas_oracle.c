@@ -1083,3 +1083,275 @@ void orc_ara_trsm(int rows, int cols, int bs, double* B, int ldb, const double* 
             B[r + (long)j * ldb] = s / R[j + (long)j * ldr];
         }
 }
+
+/* ======================================================================= */
+/* Complex precisions (C / Z)                                               */
+/* ======================================================================= */
+/* The reference instantiates its batched templates for cuFloatComplex /
+ * cuDoubleComplex (src/batch_triangular/Makefile:30-57) with the operators of
+ * include/kblas_operators.h: a * b and a + b without conjugation (:150-200),
+ * a / b = (a conj(b)) / |b|^2 written out (:237-241), sqrt(x) from the modulus
+ * and cos(arg) with the sign of Im(x) (:243-255).  So its complex POTRF is
+ * the factorization A = L L^T of a complex symmetric matrix (its tests build
+ * such matrices: testing/testing_helper.cu:404-414).  Restated here over
+ * interleaved (real, imaginary) doubles; the fused multiply-add a b + c is
+ * two FMA chains per part.  'C' (conjugate transpose) conjugates the operand
+ * in GEMM and TRSM.  "Lower view": Lv(i,j) = A(i,j) (lower) or A(j,i) (upper). */
+typedef struct { double x, y; } zc;
+
+static zc zmk(double x, double y) { zc r; r.x = x; r.y = y; return r; }
+static zc zmul(zc a, zc b) { return zmk(a.x * b.x - a.y * b.y, a.y * b.x + a.x * b.y); }
+static zc zdiv(zc a, zc b)
+{
+    const double d = b.x * b.x + b.y * b.y;
+    return zmk((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+static zc zfma(zc a, zc b, zc c) { return zmk(fma(a.x, b.x, fma(-a.y, b.y, c.x)), fma(a.x, b.y, fma(a.y, b.x, c.y))); }
+static zc zneg(zc a) { return zmk(-a.x, -a.y); }
+static zc zconj(zc a) { return zmk(a.x, -a.y); }
+
+/* exported for the tests: the reference's principal square root */
+void orc_zsqrt(double re, double im, double* out)
+{
+    const double r = hypot(re, im);
+    if (r == 0.0) { out[0] = 0.0; out[1] = 0.0; return; }
+    const double c = re / r;
+    double y = sqrt(r * (1.0 - c) / 2.0);
+    if (signbit(im)) y = -y;
+    out[0] = sqrt(r * (c + 1.0) / 2.0);
+    out[1] = y;
+}
+
+static zc* lv(zc* A, int lda, int upper, int i, int j)
+{
+    return upper ? A + j + (long)i * lda : A + i + (long)j * lda;
+}
+
+/* Column step of src/batch_triangular/Xpotrf_batch_kernels.cuh:52-69 on the
+ * lower view; info = j + 1 at a zero or non-finite pivot (the reference has
+ * no check).  Returns info. */
+int orc_zpotrf(int upper, int n, double* Ad, int lda)
+{
+    zc* A = (zc*)Ad;
+    for (int j = 0; j < n; ++j) {
+        zc d = *lv(A, lda, upper, j, j);
+        const double m = d.x * d.x + d.y * d.y;
+        if (!(m > 0.0) || !(m < INFINITY)) return j + 1;
+        double s2[2];
+        orc_zsqrt(d.x, d.y, s2);
+        const zc s = zmk(s2[0], s2[1]);
+        *lv(A, lda, upper, j, j) = s;
+        for (int i = j + 1; i < n; ++i) *lv(A, lda, upper, i, j) = zdiv(*lv(A, lda, upper, i, j), s);
+        for (int i = j + 1; i < n; ++i) {
+            const zc lij = *lv(A, lda, upper, i, j);
+            for (int k = j + 1; k <= i; ++k)
+                *lv(A, lda, upper, i, k) = zfma(zneg(lij), *lv(A, lda, upper, k, j), *lv(A, lda, upper, i, k));
+        }
+    }
+    return 0;
+}
+
+/* (Lv Lv^T) X = B for the nrhs right-hand sides: side L -> columns of the
+ * n x nrhs B, side R -> rows of the nrhs x n B (X A = B). */
+void orc_zpotrs(int upper, int side_left, int n, int nrhs, const double* Ad, int lda, double* Bd, int ldb)
+{
+    zc* A = (zc*)Ad;
+    zc* B = (zc*)Bd;
+#define ZB(i, c) (side_left ? B + (i) + (long)(c) * ldb : B + (c) + (long)(i) * ldb)
+    for (int c = 0; c < nrhs; ++c) {
+        for (int i = 0; i < n; ++i) {
+            zc v = *ZB(i, c);
+            for (int k = 0; k < i; ++k) v = zfma(zneg(*lv(A, lda, upper, i, k)), *ZB(k, c), v);
+            *ZB(i, c) = zdiv(v, *lv(A, lda, upper, i, i));
+        }
+        for (int i = n - 1; i >= 0; --i) {
+            zc v = *ZB(i, c);
+            for (int k = i + 1; k < n; ++k) v = zfma(zneg(*lv(A, lda, upper, k, i)), *ZB(k, c), v);
+            *ZB(i, c) = zdiv(v, *lv(A, lda, upper, i, i));
+        }
+    }
+#undef ZB
+}
+
+/* op(X)(i, j) for op 'N', 'T', 'C' */
+static zc zop(const zc* X, int ld, char op, int i, int j)
+{
+    if (op == 'N' || op == 'n') return X[i + (long)j * ld];
+    const zc v = X[j + (long)i * ld];
+    return (op == 'C' || op == 'c') ? zconj(v) : v;
+}
+
+/* C := alpha op(A) op(B) + beta C (beta == 0: C not read) */
+void orc_zgemm(char ta, char tb, int m, int n, int k, const double* alpha, const double* Ad, int lda,
+               const double* Bd, int ldb, const double* beta, double* Cd, int ldc)
+{
+    const zc* A = (const zc*)Ad;
+    const zc* B = (const zc*)Bd;
+    zc* C = (zc*)Cd;
+    const zc al = zmk(alpha[0], alpha[1]), be = zmk(beta[0], beta[1]);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < m; ++i) {
+            zc s = zmk(0.0, 0.0);
+            for (int p = 0; p < k; ++p) s = zfma(zop(A, lda, ta, i, p), zop(B, ldb, tb, p, j), s);
+            zc r = zmul(al, s);
+            if (be.x != 0.0 || be.y != 0.0) r = zfma(be, C[i + (long)j * ldc], r);
+            C[i + (long)j * ldc] = r;
+        }
+}
+
+/* The lower (or upper) part of C := alpha op(A) op(A)^T + beta C (no conjugation). */
+void orc_zsyrk(int upper, char trans, int n, int k, const double* alpha, const double* Ad, int lda,
+               const double* beta, double* Cd, int ldc)
+{
+    const zc* A = (const zc*)Ad;
+    zc* C = (zc*)Cd;
+    const zc al = zmk(alpha[0], alpha[1]), be = zmk(beta[0], beta[1]);
+    const char tt = (trans == 'N' || trans == 'n') ? 'T' : 'N';
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            if (upper ? i > j : i < j) continue;
+            zc s = zmk(0.0, 0.0);
+            for (int p = 0; p < k; ++p) s = zfma(zop(A, lda, trans, i, p), zop(A, lda, tt, p, j), s);
+            zc r = zmul(al, s);
+            if (be.x != 0.0 || be.y != 0.0) r = zfma(be, C[i + (long)j * ldc], r);
+            C[i + (long)j * ldc] = r;
+        }
+}
+
+/* op(T) X = alpha B (side L) or X op(T) = alpha B (side R), T triangular
+ * (lower or upper storage), op 'N' / 'T' / 'C', unit diagonal or not; B is
+ * m x n.  Substitution over the dense op(T), one right-hand side at a time. */
+void orc_ztrsm(char side, int upper, char trans, int unit, int m, int n, const double* alpha,
+               const double* Ad, int lda, double* Bd, int ldb)
+{
+    const zc* A = (const zc*)Ad;
+    zc* B = (zc*)Bd;
+    const int left = side == 'L' || side == 'l';
+    const int ord = left ? m : n, nv = left ? n : m;
+    const zc al = zmk(alpha[0], alpha[1]);
+    /* dense op(T) (ord x ord), zeros outside the triangle */
+    zc* T = (zc*)calloc((size_t)ord * ord + 1, sizeof(zc));
+    for (int j = 0; j < ord; ++j)
+        for (int i = 0; i < ord; ++i) {
+            const int in = upper ? i <= j : i >= j;
+            if (!in) continue;
+            zc v = (unit && i == j) ? zmk(1.0, 0.0) : A[i + (long)j * lda];
+            T[i + (long)j * ord] = v;
+        }
+    /* M := op(T) */
+    zc* M = (zc*)calloc((size_t)ord * ord + 1, sizeof(zc));
+    for (int j = 0; j < ord; ++j)
+        for (int i = 0; i < ord; ++i) M[i + (long)j * ord] = zop(T, ord, trans, i, j);
+    /* side L: M x = al b (x a column of B); side R: x M = al b (x a row) -> M^T x^T = al b^T */
+    const int lowerM = upper ? (trans == 'T' || trans == 't' || trans == 'C' || trans == 'c')
+                             : !(trans == 'T' || trans == 't' || trans == 'C' || trans == 'c');
+    /* effective system S y = r with S = M (side L) or M^T (side R) */
+    const int lowerS = left ? lowerM : !lowerM;
+    for (int v = 0; v < nv; ++v) {
+#define ZX(i) (left ? B + (i) + (long)v * ldb : B + v + (long)(i) * ldb)
+#define SS(i, j) (left ? M[(i) + (long)(j) * ord] : M[(j) + (long)(i) * ord])
+        for (int i = 0; i < ord; ++i) *ZX(i) = zmul(al, *ZX(i));
+        if (lowerS) {
+            for (int i = 0; i < ord; ++i) {
+                zc s = *ZX(i);
+                for (int k = 0; k < i; ++k) s = zfma(zneg(SS(i, k)), *ZX(k), s);
+                *ZX(i) = zdiv(s, SS(i, i));
+            }
+        } else {
+            for (int i = ord - 1; i >= 0; --i) {
+                zc s = *ZX(i);
+                for (int k = i + 1; k < ord; ++k) s = zfma(zneg(SS(i, k)), *ZX(k), s);
+                *ZX(i) = zdiv(s, SS(i, i));
+            }
+        }
+#undef ZX
+#undef SS
+    }
+    free(T);
+    free(M);
+}
+
+/* Complex inverse family and TRMM on the lower triangle (the reference's
+ * templates for C / Z, src/batch_triangular/X{trtri,lauum,potri,trmm}_batch*,
+ * without conjugation; TRMM's 'C' conjugates the triangle): TRTRI A := L^-1
+ * (info: first exactly-zero diagonal element of a non-unit triangle, A left
+ * unchanged), LAUUM A := L^T L, POTRI A := (L L^T)^-1 = LAUUM(TRTRI(L)). */
+int orc_ztrtri(int unit, int n, double* Ad, int lda)
+{
+    zc* A = (zc*)Ad;
+    if (!unit)
+        for (int j = 0; j < n; ++j) {
+            const zc d = A[j + (long)j * lda];
+            if (d.x == 0.0 && d.y == 0.0) return j + 1;
+        }
+    zc* X = (zc*)calloc((size_t)n * n + 1, sizeof(zc));
+    for (int j = 0; j < n; ++j) {
+        const zc djj = unit ? zmk(1.0, 0.0) : A[j + (long)j * lda];
+        X[j + (long)j * n] = zdiv(zmk(1.0, 0.0), djj);
+        for (int i = j + 1; i < n; ++i) {
+            zc s = zmk(0.0, 0.0);
+            for (int k = j; k < i; ++k) {
+                const zc lik = A[i + (long)k * lda];
+                s = zfma(lik, X[k + (long)j * n], s);
+            }
+            const zc dii = unit ? zmk(1.0, 0.0) : A[i + (long)i * lda];
+            X[i + (long)j * n] = zdiv(zneg(s), dii);
+        }
+    }
+    for (int j = 0; j < n; ++j)
+        for (int i = j; i < n; ++i)
+            if (!(unit && i == j)) A[i + (long)j * lda] = X[i + (long)j * n];
+    free(X);
+    return 0;
+}
+
+void orc_zlauum(int n, double* Ad, int lda)
+{
+    zc* A = (zc*)Ad;
+    zc* X = (zc*)calloc((size_t)n * n + 1, sizeof(zc));
+    for (int j = 0; j < n; ++j)
+        for (int i = j; i < n; ++i) {
+            zc s = zmk(0.0, 0.0);
+            for (int k = i; k < n; ++k) s = zfma(A[k + (long)i * lda], A[k + (long)j * lda], s);
+            X[i + (long)j * n] = s;
+        }
+    for (int j = 0; j < n; ++j)
+        for (int i = j; i < n; ++i) A[i + (long)j * lda] = X[i + (long)j * n];
+    free(X);
+}
+
+int orc_zpotri(int n, double* A, int lda)
+{
+    const int info = orc_ztrtri(0, n, A, lda);
+    if (info) return info;
+    orc_zlauum(n, A, lda);
+    return 0;
+}
+
+/* B := alpha op(L) B (side L, L of order m) or alpha B op(L) (side R, order n) */
+void orc_ztrmm(char side, char trans, int unit, int m, int n, const double* alpha, const double* Ad, int lda,
+               double* Bd, int ldb)
+{
+    const zc* A = (const zc*)Ad;
+    zc* B = (zc*)Bd;
+    const int left = side == 'L' || side == 'l';
+    const int ord = left ? m : n;
+    const zc al = zmk(alpha[0], alpha[1]);
+    zc* T = (zc*)calloc((size_t)ord * ord + 1, sizeof(zc));
+    for (int j = 0; j < ord; ++j)
+        for (int i = j; i < ord; ++i) T[i + (long)j * ord] = (unit && i == j) ? zmk(1.0, 0.0) : A[i + (long)j * lda];
+    zc* R = (zc*)calloc((size_t)m * n + 1, sizeof(zc));
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < m; ++i) {
+            zc s = zmk(0.0, 0.0);
+            if (left)
+                for (int k = 0; k < ord; ++k) s = zfma(zop(T, ord, trans, i, k), B[k + (long)j * ldb], s);
+            else
+                for (int k = 0; k < ord; ++k) s = zfma(B[i + (long)k * ldb], zop(T, ord, trans, k, j), s);
+            R[i + (long)j * m] = zmul(al, s);
+        }
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < m; ++i) B[i + (long)j * ldb] = R[i + (long)j * m];
+    free(T);
+    free(R);
+}

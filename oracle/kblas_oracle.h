@@ -18,6 +18,8 @@
  *   - TRTRI / LAUUM / POTRI / TRMM     src/batch_triangular/X{trtri,lauum,trmm}_batch_kernels.cuh,
  *                                       Xpotri_batch_drivers.cuh:78-110
  *   - PSTRF (pivoted Cholesky)         src/batch_svd/batch_pstrf.cu:44-158
+ *   - complex (C / Z) arithmetic        include/kblas_operators.h:150-255 (the
+ *                                       reference's complex instantiations)
  *   - ARA rank-revealing POTRF, Gram   src/batch_svd/ara_util.cuh:163-275,345-408,
  *     product, right TRSM                560-640,651-764
  *   - FLOP model                       testing/flops.h:74-150
@@ -99,6 +101,24 @@ int  orc_posv(char side, int m, int n, double* A, int lda, double* B, int ldb);
  * range, -3: FP32 factor failed, -31: no convergence).                      */
 int  orc_sposv(char side, int m, int n, const double* A, int lda,
                const double* B, int ldb, double* X, int ldx, int* iter);
+
+/* Complex (C / Z) restatement, complex-symmetric arithmetic as the reference's
+ * complex instantiations (interleaved real/imaginary doubles; scalars as
+ * pairs; upper selects the upper triangle, read as the lower view).        */
+void orc_zsqrt(double re, double im, double* out);
+int  orc_zpotrf(int upper, int n, double* A, int lda);
+void orc_zpotrs(int upper, int side_left, int n, int nrhs, const double* A, int lda, double* B, int ldb);
+void orc_zgemm(char ta, char tb, int m, int n, int k, const double* alpha, const double* A, int lda,
+               const double* B, int ldb, const double* beta, double* C, int ldc);
+void orc_zsyrk(int upper, char trans, int n, int k, const double* alpha, const double* A, int lda,
+               const double* beta, double* C, int ldc);
+void orc_ztrsm(char side, int upper, char trans, int unit, int m, int n, const double* alpha,
+               const double* A, int lda, double* B, int ldb);
+int  orc_ztrtri(int unit, int n, double* A, int lda);
+void orc_zlauum(int n, double* A, int lda);
+int  orc_zpotri(int n, double* A, int lda);
+void orc_ztrmm(char side, char trans, int unit, int m, int n, const double* alpha, const double* A, int lda,
+               double* B, int ldb);
 
 /* single-precision register/recursive restatement (S API checks).          */
 int  orc_spotrf(int n, float* A, int lda);

@@ -39,6 +39,16 @@ _SIG = {
     "orc_trtri": (_i, [_c, _i, _p, _i]),
     "orc_pstrf": (_i, [_i, _p, _i, _p]),
     "orc_ara_potrf": (_i, [_i, _i, _p, _i, _p, _i, _p]),
+    "orc_zsqrt": (None, [_d, _d, _p]),
+    "orc_zpotrf": (_i, [_i, _i, _p, _i]),
+    "orc_zpotrs": (None, [_i, _i, _i, _i, _p, _i, _p, _i]),
+    "orc_zgemm": (None, [_c, _c, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
+    "orc_zsyrk": (None, [_i, _c, _i, _i, _p, _p, _i, _p, _p, _i]),
+    "orc_ztrsm": (None, [_c, _i, _c, _i, _i, _i, _p, _p, _i, _p, _i]),
+    "orc_ztrtri": (_i, [_i, _i, _p, _i]),
+    "orc_zlauum": (None, [_i, _p, _i]),
+    "orc_zpotri": (_i, [_i, _p, _i]),
+    "orc_ztrmm": (None, [_c, _c, _i, _i, _i, _p, _p, _i, _p, _i]),
     "orc_ara_syrk": (None, [_i, _i, _p, _i, _p, _i]),
     "orc_ara_trsm": (None, [_i, _i, _i, _p, _i, _p, _i]),
     "orc_lauum": (None, [_i, _p, _i]),
@@ -183,6 +193,92 @@ def ara_trsm(B, R, bs):
     R = np.asfortranarray(R, dtype=np.float64)
     rows, cols = X.shape
     lib().orc_ara_trsm(rows, cols, bs, ptr(X), max(rows, 1), ptr(R), max(R.shape[0], 1))
+    return X
+
+
+def _zpair(v):
+    v = complex(v)
+    return np.array([v.real, v.imag], dtype=np.float64)
+
+
+def zsqrt(v):
+    out = np.zeros(2)
+    lib().orc_zsqrt(complex(v).real, complex(v).imag, ptr(out))
+    return complex(out[0], out[1])
+
+
+def zpotrf(A, upper=False):
+    """complex symmetric L L^T (or U^T U) of the lower (upper) triangle; returns (factor array, info)"""
+    X = np.array(A, dtype=np.complex128, order="F", copy=True)
+    n = X.shape[0]
+    info = lib().orc_zpotrf(int(upper), n, ptr(X), max(n, 1))
+    return X, info
+
+
+def zpotrs(L, B, side="L", upper=False):
+    """(Lv Lv^T) X = B: side L -> B is n x nrhs, side R -> B is nrhs x n"""
+    L = np.asfortranarray(L, dtype=np.complex128)
+    X = np.array(B, dtype=np.complex128, order="F", copy=True)
+    n = L.shape[0]
+    nrhs = X.shape[1] if side == "L" else X.shape[0]
+    lib().orc_zpotrs(int(upper), int(side == "L"), n, nrhs, ptr(L), max(n, 1), ptr(X), max(X.shape[0], 1))
+    return X
+
+
+def zgemm(ta, tb, alpha, A, B, beta, C):
+    A = np.asfortranarray(A, dtype=np.complex128)
+    B = np.asfortranarray(B, dtype=np.complex128)
+    X = np.array(C, dtype=np.complex128, order="F", copy=True)
+    m, n = X.shape
+    k = A.shape[1] if ta == "N" else A.shape[0]
+    lib().orc_zgemm(ch(ta), ch(tb), m, n, k, ptr(_zpair(alpha)), ptr(A), max(A.shape[0], 1), ptr(B),
+                    max(B.shape[0], 1), ptr(_zpair(beta)), ptr(X), max(m, 1))
+    return X
+
+
+def zsyrk(upper, trans, alpha, A, beta, C):
+    A = np.asfortranarray(A, dtype=np.complex128)
+    X = np.array(C, dtype=np.complex128, order="F", copy=True)
+    n = X.shape[0]
+    k = A.shape[1] if trans == "N" else A.shape[0]
+    lib().orc_zsyrk(int(upper), ch(trans), n, k, ptr(_zpair(alpha)), ptr(A), max(A.shape[0], 1),
+                    ptr(_zpair(beta)), ptr(X), max(n, 1))
+    return X
+
+
+def ztrsm(side, upper, trans, unit, alpha, A, B):
+    A = np.asfortranarray(A, dtype=np.complex128)
+    X = np.array(B, dtype=np.complex128, order="F", copy=True)
+    m, n = X.shape
+    lib().orc_ztrsm(ch(side), int(upper), ch(trans), int(unit), m, n, ptr(_zpair(alpha)), ptr(A),
+                    max(A.shape[0], 1), ptr(X), max(m, 1))
+    return X
+
+
+def ztrtri(L, unit=False):
+    X = np.array(L, dtype=np.complex128, order="F", copy=True)
+    info = lib().orc_ztrtri(int(unit), X.shape[0], ptr(X), max(X.shape[0], 1))
+    return X, info
+
+
+def zlauum(L):
+    X = np.array(L, dtype=np.complex128, order="F", copy=True)
+    lib().orc_zlauum(X.shape[0], ptr(X), max(X.shape[0], 1))
+    return X
+
+
+def zpotri(L):
+    X = np.array(L, dtype=np.complex128, order="F", copy=True)
+    info = lib().orc_zpotri(X.shape[0], ptr(X), max(X.shape[0], 1))
+    return X, info
+
+
+def ztrmm(side, trans, unit, alpha, L, B):
+    L = np.asfortranarray(L, dtype=np.complex128)
+    X = np.array(B, dtype=np.complex128, order="F", copy=True)
+    m, n = X.shape
+    lib().orc_ztrmm(ch(side), ch(trans), int(unit), m, n, ptr(_zpair(alpha)), ptr(L), max(L.shape[0], 1), ptr(X),
+                    max(m, 1))
     return X
 
 
