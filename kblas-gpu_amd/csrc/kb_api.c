@@ -420,7 +420,7 @@ static int vsplit(char prec, int mode, int max_n)
     return (v >= 0 && v <= KB_REG_MAX_N) ? v : KB_REG_MAX_N;
 }
 
-static int vchol(kblasHandle_t h, char prec, int mode, char side, char uplo, int* m, int* n, int max_n,
+static int vchol(kblasHandle_t h, char prec, int mode, char side, char uplo, int* m, int* n, int max_m, int max_n,
                  void* A, int* lda, void* B, int* ldb, void* X, int* ldx, int bc, int* info, int* iter)
 {
     if (!h) return KBLAS_Error_NotInitialized;
@@ -437,23 +437,25 @@ static int vchol(kblasHandle_t h, char prec, int mode, char side, char uplo, int
     kb_mat_t Am = mat_vptrs(A, lda);
     kb_mat_t Bm = B ? mat_vptrs(B, ldb) : Am;
     kb_mat_t Xm = X ? mat_vptrs(X, ldx) : Bm;
-    return launched(kb_launch_vchol_split(prec, mode, m, n, sd == 0, Am, Bm, Xm, info, iter, bc, max_n,
+    /* right-hand sides: the columns of B (side L) or its rows (side R) */
+    const int max_nrhs = mode == KB_MODE_POTRF ? 0 : (sd == 0 ? max_n : max_m);
+    return launched(kb_launch_vchol_split(prec, mode, m, n, sd == 0, Am, Bm, Xm, info, iter, bc, max_nrhs,
                                           h->d_data, vsplit(prec, mode, max_n), h->stream));
 }
 
 int kblasDpotrf_vbatch(kblasHandle_t h, char uplo, int* n, int max_n, double** A, int* lda, int bc, int* info)
-{ return vchol(h, 'd', KB_MODE_POTRF, 'R', uplo, NULL, n, max_n, A, lda, NULL, NULL, NULL, NULL, bc, info, NULL); }
+{ return vchol(h, 'd', KB_MODE_POTRF, 'R', uplo, NULL, n, 0, max_n, A, lda, NULL, NULL, NULL, NULL, bc, info, NULL); }
 int kblasSpotrf_vbatch(kblasHandle_t h, char uplo, int* n, int max_n, float** A, int* lda, int bc, int* info)
-{ return vchol(h, 's', KB_MODE_POTRF, 'R', uplo, NULL, n, max_n, A, lda, NULL, NULL, NULL, NULL, bc, info, NULL); }
+{ return vchol(h, 's', KB_MODE_POTRF, 'R', uplo, NULL, n, 0, max_n, A, lda, NULL, NULL, NULL, NULL, bc, info, NULL); }
 int kblasDpotrs_vbatch(kblasHandle_t h, char side, char uplo, int* m, int* n, int max_m, int max_n,
                        double** A, int* lda, double** B, int* ldb, int bc)
-{ (void)max_m; return vchol(h, 'd', KB_MODE_POTRS, side, uplo, m, n, max_n, A, lda, B, ldb, NULL, NULL, bc, NULL, NULL); }
+{ return vchol(h, 'd', KB_MODE_POTRS, side, uplo, m, n, max_m, max_n, A, lda, B, ldb, NULL, NULL, bc, NULL, NULL); }
 int kblasDposv_vbatch(kblasHandle_t h, char side, char uplo, int* m, int* n, int max_m, int max_n,
                       double** A, int* lda, double** B, int* ldb, int bc, int* info)
-{ (void)max_m; return vchol(h, 'd', KB_MODE_POSV, side, uplo, m, n, max_n, A, lda, B, ldb, NULL, NULL, bc, info, NULL); }
+{ return vchol(h, 'd', KB_MODE_POSV, side, uplo, m, n, max_m, max_n, A, lda, B, ldb, NULL, NULL, bc, info, NULL); }
 int kblasSposv_vbatch(kblasHandle_t h, char side, char uplo, int* m, int* n, int max_m, int max_n,
                       float** A, int* lda, float** B, int* ldb, int bc, int* info)
-{ (void)max_m; return vchol(h, 's', KB_MODE_POSV, side, uplo, m, n, max_n, A, lda, B, ldb, NULL, NULL, bc, info, NULL); }
+{ return vchol(h, 's', KB_MODE_POSV, side, uplo, m, n, max_m, max_n, A, lda, B, ldb, NULL, NULL, bc, info, NULL); }
 /* Mixed non-uniform POSV: the queue is required; when the workspace also
  * holds the FP32 slots (kblas_sposv_vbatch_wsquery) the orders above 64 run
  * the mixed method, else dsposv's full-precision path (ITER = -1). */
@@ -491,7 +493,6 @@ int kblasDsposv_vbatch(kblasHandle_t h, char side, char uplo, int* m, int* n, in
 int kblasDposv_vbatch_noalloc(kblasHandle_t h, char side, char uplo, int* m, int* n, int max_m, int max_n,
                               double** A, int* lda, double** B, int* ldb, int bc, int* info)
 {
-    (void)max_m;
     if (!h) return KBLAS_Error_NotInitialized;
     int u = upl(uplo);
     int sd = sideof(side);
@@ -508,8 +509,8 @@ int kblasDposv_vbatch_noalloc(kblasHandle_t h, char side, char uplo, int* m, int
     if (!ws) return kb_hip_status(err ? err : 1);
     kb_mat_t Am = mat_vptrs(A, lda);
     kb_mat_t Bm = mat_vptrs(B, ldb);
-    return launched(kb_launch_vchol_split('d', KB_MODE_POSV, m, n, sd == 0, Am, Bm, Bm, info, NULL, bc, max_n, ws,
-                                          vsplit('d', KB_MODE_POSV, max_n), h->stream));
+    return launched(kb_launch_vchol_split('d', KB_MODE_POSV, m, n, sd == 0, Am, Bm, Bm, info, NULL, bc,
+                                          sd == 0 ? max_n : max_m, ws, vsplit('d', KB_MODE_POSV, max_n), h->stream));
 }
 
 /* ============ inverse family: TRTRI / LAUUM / POTRI / POTI ============== */

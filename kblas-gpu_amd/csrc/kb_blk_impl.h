@@ -323,15 +323,19 @@ KB_DEV void blk_diag_subst(T* B, int ldb, int side_left, int k0, int kb, int c0,
     }
 }
 
-template <typename T, bool FWD>
+// ns groups of 16 right-hand sides, ns <= CB (the widest pass instantiated)
+template <typename T, bool FWD, int CB>
 KB_DEV void blk_diag_subst_n(int ns, T* B, int ldb, int side_left, int k0, int kb, int c0, int rc, BlkSmem<T>& S,
                              T* sY)
 {
-    switch (ns) {
-    case 1: blk_diag_subst<T, 1, FWD>(B, ldb, side_left, k0, kb, c0, rc, S, sY); break;
-    case 2: blk_diag_subst<T, 2, FWD>(B, ldb, side_left, k0, kb, c0, rc, S, sY); break;
-    case 3: blk_diag_subst<T, 3, FWD>(B, ldb, side_left, k0, kb, c0, rc, S, sY); break;
-    default: blk_diag_subst<T, 4, FWD>(B, ldb, side_left, k0, kb, c0, rc, S, sY); break;
+    if (CB == 2 || ns <= 2) {
+        if (ns == 1) blk_diag_subst<T, 1, FWD>(B, ldb, side_left, k0, kb, c0, rc, S, sY);
+        else blk_diag_subst<T, 2, FWD>(B, ldb, side_left, k0, kb, c0, rc, S, sY);
+        return;
+    }
+    if constexpr (CB > 2) {
+        if (ns == 3) blk_diag_subst<T, 3, FWD>(B, ldb, side_left, k0, kb, c0, rc, S, sY);
+        else blk_diag_subst<T, 4, FWD>(B, ldb, side_left, k0, kb, c0, rc, S, sY);
     }
 }
 
@@ -399,40 +403,45 @@ KB_DEV void blk_update_rows(const T* L, int lda, T* B, int ldb, int side_left, i
     }
 }
 
-template <typename T, bool TRANS>
+template <typename T, bool TRANS, int CB>
 KB_DEV void blk_update_rows_n(int ncb, const T* L, int lda, T* B, int ldb, int side_left, int k0, int kb, int r_lo,
                               int r_hi, int c0, int rc, const T* sY)
 {
-    switch (ncb) {
-    case 1: blk_update_rows<T, TRANS, 1>(L, lda, B, ldb, side_left, k0, kb, r_lo, r_hi, c0, rc, sY); break;
-    case 2: blk_update_rows<T, TRANS, 2>(L, lda, B, ldb, side_left, k0, kb, r_lo, r_hi, c0, rc, sY); break;
-    case 3: blk_update_rows<T, TRANS, 3>(L, lda, B, ldb, side_left, k0, kb, r_lo, r_hi, c0, rc, sY); break;
-    default: blk_update_rows<T, TRANS, 4>(L, lda, B, ldb, side_left, k0, kb, r_lo, r_hi, c0, rc, sY); break;
+    if (CB == 2 || ncb <= 2) {
+        if (ncb == 1) blk_update_rows<T, TRANS, 1>(L, lda, B, ldb, side_left, k0, kb, r_lo, r_hi, c0, rc, sY);
+        else blk_update_rows<T, TRANS, 2>(L, lda, B, ldb, side_left, k0, kb, r_lo, r_hi, c0, rc, sY);
+        return;
+    }
+    if constexpr (CB > 2) {
+        if (ncb == 3) blk_update_rows<T, TRANS, 3>(L, lda, B, ldb, side_left, k0, kb, r_lo, r_hi, c0, rc, sY);
+        else blk_update_rows<T, TRANS, 4>(L, lda, B, ldb, side_left, k0, kb, r_lo, r_hi, c0, rc, sY);
     }
 }
 
 // One block step of the forward sweep L Y = B: substitution with the
 // diagonal block staged in sD / sInv, then the rows below updated with the
-// block column of L (all right-hand sides, in passes of up to 64).
-template <typename T>
+// block column of L (all right-hand sides, in passes of up to 16 CB: a
+// narrower pass needs fewer registers, a wider one streams L fewer times).
+template <typename T, int CB = 4>
 KB_DEV void blk_fwd_step(const T* L, int lda, int n, int k0, int kb, T* B, int ldb, int side_left, int nrhs,
                          BlkSmem<T>& S)
 {
-    constexpr int RC = KB_BLK_RC;
+    constexpr int RC = 16 * CB;
+    static_assert(RC <= KB_BLK_RC, "the staging holds KB_BLK_RC right-hand sides");
     T* sY = S.sY;   // RC x (NBK + 1) staging of the block's solution
     for (int c0 = 0; c0 < nrhs; c0 += RC) {
         const int rc = min(RC, nrhs - c0);
         const int ncb = (rc + 15) / 16;
-        blk_diag_subst_n<T, true>(ncb, B, ldb, side_left, k0, kb, c0, rc, S, sY);
+        blk_diag_subst_n<T, true, CB>(ncb, B, ldb, side_left, k0, kb, c0, rc, S, sY);
         __syncthreads();
-        blk_update_rows_n<T, false>(ncb, L, lda, B, ldb, side_left, k0, kb, k0 + kb, n, c0, rc, sY);
+        blk_update_rows_n<T, false, CB>(ncb, L, lda, B, ldb, side_left, k0, kb, k0 + kb, n, c0, rc, sY);
         __syncthreads();
     }
 }
 
 // Forward sweep L Y = B over 32-row blocks (right-looking), the next
 // diagonal block fetched while the current one is used.
-template <typename T>
+template <typename T, int CB = 4>
 KB_DEV void blk_solve_fwd(const T* L, int lda, int n, T* B, int ldb, int side_left, int nrhs, BlkSmem<T>& S)
 {
     constexpr int NBK = KB_BLK_NBK;
@@ -442,16 +451,16 @@ KB_DEV void blk_solve_fwd(const T* L, int lda, int n, T* B, int ldb, int side_le
         const int kb = min(NBK, n - k0);
         blk_diag_put<T>(dnext, kb, S);
         if (k0 + NBK < n) blk_diag_fetch<T>(dnext, L, lda, k0 + NBK, min(NBK, n - k0 - NBK));
-        blk_fwd_step<T>(L, lda, n, k0, kb, B, ldb, side_left, nrhs, S);
+        blk_fwd_step<T, CB>(L, lda, n, k0, kb, B, ldb, side_left, nrhs, S);
     }
 }
 
 // Backward sweep L^T X = Y over 32-row blocks (right-looking: after each
 // block's substitution the rows above it are updated).
-template <typename T>
+template <typename T, int CB = 4>
 KB_DEV void blk_solve_bwd(const T* L, int lda, int n, T* B, int ldb, int side_left, int nrhs, BlkSmem<T>& S)
 {
-    constexpr int NBK = KB_BLK_NBK, RC = KB_BLK_RC;
+    constexpr int NBK = KB_BLK_NBK, RC = 16 * CB;
     T* sY = S.sY;
     T dnext[KB_BLK_DPT];
     const int nblk = (n + NBK - 1) / NBK;
@@ -464,20 +473,20 @@ KB_DEV void blk_solve_bwd(const T* L, int lda, int n, T* B, int ldb, int side_le
         for (int c0 = 0; c0 < nrhs; c0 += RC) {
             const int rc = min(RC, nrhs - c0);
             const int ncb = (rc + 15) / 16;
-            blk_diag_subst_n<T, false>(ncb, B, ldb, side_left, k0, kb, c0, rc, S, sY);
+            blk_diag_subst_n<T, false, CB>(ncb, B, ldb, side_left, k0, kb, c0, rc, S, sY);
             __syncthreads();
-            blk_update_rows_n<T, true>(ncb, L, lda, B, ldb, side_left, k0, kb, 0, k0, c0, rc, sY);
+            blk_update_rows_n<T, true, CB>(ncb, L, lda, B, ldb, side_left, k0, kb, 0, k0, c0, rc, sY);
             __syncthreads();
         }
     }
 }
 
 // Solve (L L^T) X = B for nrhs vectors stored by rhs_at(); L lower in A.
-template <typename T>
+template <typename T, int CB = 4>
 KB_DEV void blk_solve(const T* L, int lda, int n, T* B, int ldb, int side_left, int nrhs, BlkSmem<T>& S)
 {
-    blk_solve_fwd<T>(L, lda, n, B, ldb, side_left, nrhs, S);
-    blk_solve_bwd<T>(L, lda, n, B, ldb, side_left, nrhs, S);
+    blk_solve_fwd<T, CB>(L, lda, n, B, ldb, side_left, nrhs, S);
+    blk_solve_bwd<T, CB>(L, lda, n, B, ldb, side_left, nrhs, S);
 }
 
 // In-place lower Cholesky of an n x n matrix by the whole workgroup.
@@ -486,7 +495,7 @@ KB_DEV void blk_solve(const T* L, int lda, int n, T* B, int ldb, int side_left, 
 // follows the panel, while its diagonal block is still in LDS and the panel
 // in cache; a factorization that fails at panel k leaves B forward-solved
 // through the panels before k.
-template <typename T>
+template <typename T, int CB = 4>
 KB_DEV int blk_potrf(T* A, int lda, int n, BlkSmem<T>& S, T* B = nullptr, int ldb = 0, int side_left = 0,
                      int nrhs = 0)
 {
@@ -495,7 +504,7 @@ KB_DEV int blk_potrf(T* A, int lda, int n, BlkSmem<T>& S, T* B = nullptr, int ld
         const int kb0 = min(NBK, n - k0);
         int f = blk_panel<T>(A, lda, n, k0, kb0, S);
         if (f) return f;
-        if (B) blk_fwd_step<T>(A, lda, n, k0, kb0, B, ldb, side_left, nrhs, S);
+        if (B) blk_fwd_step<T, CB>(A, lda, n, k0, kb0, B, ldb, side_left, nrhs, S);
         const int k1 = k0 + kb0;
         if (k1 >= n) break;
         const int kb1 = min(NBK, n - k1);
@@ -504,7 +513,7 @@ KB_DEV int blk_potrf(T* A, int lda, int n, BlkSmem<T>& S, T* B = nullptr, int ld
         __syncthreads();
         f = blk_panel<T>(A, lda, n, k1, kb1, S);
         if (f) return f;
-        if (B) blk_fwd_step<T>(A, lda, n, k1, kb1, B, ldb, side_left, nrhs, S);
+        if (B) blk_fwd_step<T, CB>(A, lda, n, k1, kb1, B, ldb, side_left, nrhs, S);
         // the trailing matrix, with both panels at once (depth kb0 + kb1)
         const int t0 = k1 + kb1;
         if (t0 < n) blk_syrk_update<T>(A + t0 + (long)k0 * lda, A + t0 + (long)t0 * lda, lda, n - t0, kb0 + kb1, S, n - t0);
@@ -513,8 +522,10 @@ KB_DEV int blk_potrf(T* A, int lda, int n, BlkSmem<T>& S, T* B = nullptr, int ld
     return 0;
 }
 
-template <typename T, int MODE>
-__global__ void __launch_bounds__(KB_BLK_WG) __attribute__((amdgpu_waves_per_eu(2)))
+// CB = 2: passes of 32 right-hand sides, few enough registers for four
+// waves per SIMD; CB = 4: passes of 64 (L streamed half as often), three.
+template <typename T, int MODE, int CB>
+__global__ void __launch_bounds__(KB_BLK_WG) __attribute__((amdgpu_waves_per_eu(CB == 2 ? 4 : 2)))
 k_blk_chol(int n, int nrhs, int side_left, BatchArg<T> A, BatchArg<T> B, int* info, int batch)
 {
     __shared__ BlkSmem<T> S;
@@ -523,19 +534,34 @@ k_blk_chol(int n, int nrhs, int side_left, BatchArg<T> A, BatchArg<T> B, int* in
         const int lda = A.ldim(mat);
         int inf = 0;
         if constexpr (MODE == KB_MODE_POTRF) {
-            inf = blk_potrf<T>(Ap, lda, n, S);
+            inf = blk_potrf<T, CB>(Ap, lda, n, S);
             if (threadIdx.x == 0 && info) info[mat] = inf;
         } else if constexpr (MODE == KB_MODE_POSV) {
             T* Bp = B.get(mat);
             const int ldb = B.ldim(mat);
-            inf = blk_potrf<T>(Ap, lda, n, S, Bp, ldb, side_left, nrhs);
+            inf = blk_potrf<T, CB>(Ap, lda, n, S, Bp, ldb, side_left, nrhs);
             if (threadIdx.x == 0 && info) info[mat] = inf;
-            if (inf == 0) blk_solve_bwd<T>(Ap, lda, n, Bp, ldb, side_left, nrhs, S);
+            if (inf == 0) blk_solve_bwd<T, CB>(Ap, lda, n, Bp, ldb, side_left, nrhs, S);
         } else {
-            blk_solve<T>(Ap, lda, n, B.get(mat), B.ldim(mat), side_left, nrhs, S);
+            blk_solve<T, CB>(Ap, lda, n, B.get(mat), B.ldim(mat), side_left, nrhs, S);
         }
         __syncthreads();
     }
+}
+
+// Passes of 32 right-hand sides when the count allows (POTRF: none), else 64.
+constexpr int KB_BLK_NARROW = 32;
+
+template <typename T, int MODE>
+static void launch_blk_mode(int n, int nrhs, int side_left, BatchArg<T> a, BatchArg<T> b, int* info, int batch,
+                            int grid, hipStream_t s)
+{
+    if (MODE == KB_MODE_POTRF || nrhs <= KB_BLK_NARROW)
+        hipLaunchKernelGGL((k_blk_chol<T, MODE, 2>), dim3(grid), dim3(KB_BLK_WG), 0, s, n, nrhs, side_left, a, b, info,
+                           batch);
+    else
+        hipLaunchKernelGGL((k_blk_chol<T, MODE, 4>), dim3(grid), dim3(KB_BLK_WG), 0, s, n, nrhs, side_left, a, b, info,
+                           batch);
 }
 
 template <typename T>
@@ -545,10 +571,10 @@ static int launch_blk(int mode, int n, int nrhs, int side_left, kb_mat_t A, kb_m
     const int grid = batch < 2048 ? batch : 2048;
     BatchArg<T> a = to_arg<T>(A), b = to_arg<T>(B);
     if (mode == KB_MODE_POTRF)
-        hipLaunchKernelGGL((k_blk_chol<T, KB_MODE_POTRF>), dim3(grid), dim3(KB_BLK_WG), 0, s, n, nrhs, side_left, a, b, info, batch);
+        launch_blk_mode<T, KB_MODE_POTRF>(n, nrhs, side_left, a, b, info, batch, grid, s);
     else if (mode == KB_MODE_POSV)
-        hipLaunchKernelGGL((k_blk_chol<T, KB_MODE_POSV>), dim3(grid), dim3(KB_BLK_WG), 0, s, n, nrhs, side_left, a, b, info, batch);
+        launch_blk_mode<T, KB_MODE_POSV>(n, nrhs, side_left, a, b, info, batch, grid, s);
     else
-        hipLaunchKernelGGL((k_blk_chol<T, KB_MODE_POTRS>), dim3(grid), dim3(KB_BLK_WG), 0, s, n, nrhs, side_left, a, b, info, batch);
+        launch_blk_mode<T, KB_MODE_POTRS>(n, nrhs, side_left, a, b, info, batch, grid, s);
     return (int)hipGetLastError();
 }

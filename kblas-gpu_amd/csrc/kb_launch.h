@@ -75,7 +75,8 @@ int kb_launch_trsm(char prec, int side_left, int trans, int unit, int m, int n, 
  * vn[i] (right side); mode POTRF passes vm = NULL and vn[i] = order.
  * prec 'd', 's' or 'm' (mode SPOSV: FP32 factor + FP64 refinement, X receives
  * the solution, iter as in dsposv).  ws: device workspace of
- * kb_vchol_ws_bytes(batch) bytes. */
+ * kb_vchol_ws_bytes(batch) bytes.  max_nrhs: a bound on the right-hand-side
+ * counts (0: unknown); up to 32 the workgroup phase runs passes of 32. */
 enum { KB_MODE_SPOSV = 3 };
 /* ints of the queue header (histogram, offsets, counters) before the order array */
 #define KB_VCHOL_WS_HDR_INTS (2 * 4096 + 64)
@@ -90,13 +91,13 @@ int kb_noalloc_init(void);
 void* kb_noalloc_scratch(int* err);
 int kb_launch_vchol(char prec, int mode, const int* vm, const int* vn, int side_left,
                     kb_mat_t A, kb_mat_t B, kb_mat_t X, int* info, int* iter, int batch,
-                    int max_n, void* ws, hipStream_t s);
+                    int max_nrhs, void* ws, hipStream_t s);
 /* The same with the order at and below which a matrix goes to the wavefront
  * phase (0..KB_REG_MAX_N; orders above it go to the workgroup phase).  The
  * mixed mode always splits at KB_REG_MAX_N. */
 int kb_launch_vchol_split(char prec, int mode, const int* vm, const int* vn, int side_left,
                           kb_mat_t A, kb_mat_t B, kb_mat_t X, int* info, int* iter, int batch,
-                          int max_n, void* ws, int split, hipStream_t s);
+                          int max_nrhs, void* ws, int split, hipStream_t s);
 
 /* Non-uniform mixed POSV.  ws holds ws_bytes bytes: the queue, then (when
  * ws_bytes >= kb_vsposv_ws_bytes(batch, max_order, max_nrhs)) one FP32 slot

@@ -95,8 +95,10 @@ extern "C" int kb_launch_blk_chol(char prec, int mode, int n, int nrhs, int side
     return kb_blk_chol_s(mode, n, nrhs, side_left, A, B, info, batch, s);
 }
 
-extern "C" int kb_vchol_d(int, const int*, const int*, int, kb_mat_t, kb_mat_t, kb_mat_t, int*, int*, int, void*, int, hipStream_t);
-extern "C" int kb_vchol_s(int, const int*, const int*, int, kb_mat_t, kb_mat_t, kb_mat_t, int*, int*, int, void*, int, hipStream_t);
+extern "C" int kb_vchol_d(int, const int*, const int*, int, kb_mat_t, kb_mat_t, kb_mat_t, int*, int*, int, void*, int, int,
+                          hipStream_t);
+extern "C" int kb_vchol_s(int, const int*, const int*, int, kb_mat_t, kb_mat_t, kb_mat_t, int*, int*, int, void*, int, int,
+                          hipStream_t);
 extern "C" int kb_vchol_m(int, const int*, const int*, int, kb_mat_t, kb_mat_t, kb_mat_t, int*, int*, int, void*, hipStream_t);
 
 // hist + offsets + counters + the order array (see kb_vchol_impl.h)
@@ -107,20 +109,19 @@ extern "C" size_t kb_vchol_ws_bytes(int batch)
 
 extern "C" int kb_launch_vchol_split(char prec, int mode, const int* vm, const int* vn, int side_left,
                                      kb_mat_t A, kb_mat_t B, kb_mat_t X, int* info, int* iter, int batch,
-                                     int max_n, void* ws, int split, hipStream_t s)
+                                     int max_nrhs, void* ws, int split, hipStream_t s)
 {
-    (void)max_n;
     if (batch < 1) return 0;
     if (!ws || split < 0 || split > KB_REG_MAX_N) return (int)hipErrorInvalidValue;
     if (prec == 'm') return kb_vchol_m(mode, vm, vn, side_left, A, B, X, info, iter, batch, ws, s);
-    if (prec == 'd') return kb_vchol_d(mode, vm, vn, side_left, A, B, X, info, iter, batch, ws, split, s);
-    return kb_vchol_s(mode, vm, vn, side_left, A, B, X, info, iter, batch, ws, split, s);
+    if (prec == 'd') return kb_vchol_d(mode, vm, vn, side_left, A, B, X, info, iter, batch, ws, split, max_nrhs, s);
+    return kb_vchol_s(mode, vm, vn, side_left, A, B, X, info, iter, batch, ws, split, max_nrhs, s);
 }
 
 extern "C" int kb_launch_vchol(char prec, int mode, const int* vm, const int* vn, int side_left,
                                kb_mat_t A, kb_mat_t B, kb_mat_t X, int* info, int* iter, int batch,
-                               int max_n, void* ws, hipStream_t s)
+                               int max_nrhs, void* ws, hipStream_t s)
 {
-    return kb_launch_vchol_split(prec, mode, vm, vn, side_left, A, B, X, info, iter, batch, max_n, ws,
+    return kb_launch_vchol_split(prec, mode, vm, vn, side_left, A, B, X, info, iter, batch, max_nrhs, ws,
                                  KB_REG_MAX_N, s);
 }

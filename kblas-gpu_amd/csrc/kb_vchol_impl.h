@@ -139,9 +139,10 @@ KB_DEV void v_small(long mat, int n, int nrhs, int side_left, BatchArg<T>& A, Ba
     }
 }
 
-// Workgroup phase: the matrices above the register range, one workgroup each.
-template <typename T, int MODE>
-__global__ void __launch_bounds__(KB_BLK_WG) __attribute__((amdgpu_waves_per_eu(2)))
+// Workgroup phase: the matrices above the register range, one workgroup each
+// (CB: passes of 16 CB right-hand sides, as k_blk_chol).
+template <typename T, int MODE, int CB>
+__global__ void __launch_bounds__(KB_BLK_WG) __attribute__((amdgpu_waves_per_eu(CB == 2 ? 4 : 2)))
 k_vchol_blk(const int* vm, const int* vn, int side_left, BatchArg<T> A, BatchArg<T> B, BatchArg<T> X,
             int* info, int* iter, int batch, void* ws)
 {
@@ -169,23 +170,23 @@ k_vchol_blk(const int* vm, const int* vn, int side_left, BatchArg<T> A, BatchArg
             const int ldb = B.ldim(mat), ldx = X.ldim(mat);
             copy_rhs<T>(Bp, ldb, Xp, ldx, side_left, n, nrhs, threadIdx.x, KB_BLK_WG);
             __syncthreads();
-            const int inf = blk_potrf<T>(Ap, lda, n, S, Xp, ldx, side_left, nrhs);
+            const int inf = blk_potrf<T, CB>(Ap, lda, n, S, Xp, ldx, side_left, nrhs);
             if (threadIdx.x == 0) { if (info) info[mat] = inf; if (iter) iter[mat] = -1; }
-            if (inf == 0) blk_solve_bwd<T>(Ap, lda, n, Xp, ldx, side_left, nrhs, S);
+            if (inf == 0) blk_solve_bwd<T, CB>(Ap, lda, n, Xp, ldx, side_left, nrhs, S);
         } else {
             T* Ap = A.get(mat);
             const int lda = A.ldim(mat);
             if constexpr (MODE == KB_MODE_POTRF) {
-                const int inf = blk_potrf<T>(Ap, lda, n, S);
+                const int inf = blk_potrf<T, CB>(Ap, lda, n, S);
                 if (threadIdx.x == 0 && info) info[mat] = inf;
             } else if constexpr (MODE == KB_MODE_POSV) {
                 T* Bp = B.get(mat);
                 const int ldb = B.ldim(mat);
-                const int inf = blk_potrf<T>(Ap, lda, n, S, Bp, ldb, side_left, nrhs);
+                const int inf = blk_potrf<T, CB>(Ap, lda, n, S, Bp, ldb, side_left, nrhs);
                 if (threadIdx.x == 0 && info) info[mat] = inf;
-                if (inf == 0) blk_solve_bwd<T>(Ap, lda, n, Bp, ldb, side_left, nrhs, S);
+                if (inf == 0) blk_solve_bwd<T, CB>(Ap, lda, n, Bp, ldb, side_left, nrhs, S);
             } else {
-                blk_solve<T>(Ap, lda, n, B.get(mat), B.ldim(mat), side_left, nrhs, S);
+                blk_solve<T, CB>(Ap, lda, n, B.get(mat), B.ldim(mat), side_left, nrhs, S);
             }
         }
         __syncthreads();
@@ -234,7 +235,7 @@ static int vchol_grid(K kernel, int batch)
 
 template <typename T, int MODE>
 static int launch_vchol(const int* vm, const int* vn, int side_left, kb_mat_t A, kb_mat_t B, kb_mat_t X,
-                        int* info, int* iter, int batch, void* ws, int split, hipStream_t s)
+                        int* info, int* iter, int batch, void* ws, int split, int max_nrhs, hipStream_t s)
 {
     hipLaunchKernelGGL(k_vq_clear, dim3(8), dim3(1024), 0, s, ws);
     const int gb = (batch + 255) / 256 < 1024 ? (batch + 255) / 256 : 1024;
@@ -242,8 +243,13 @@ static int launch_vchol(const int* vm, const int* vn, int side_left, kb_mat_t A,
     hipLaunchKernelGGL(k_vq_scan, dim3(1), dim3(1024), 0, s, ws, batch, split);
     hipLaunchKernelGGL(k_vq_scatter, dim3(gb), dim3(256), 0, s, ws, vm, vn, side_left, batch);
     const BatchArg<T> a = to_arg<T>(A), b = to_arg<T>(B), x = to_arg<T>(X);
-    hipLaunchKernelGGL((k_vchol_blk<T, MODE>), dim3(vchol_grid(k_vchol_blk<T, MODE>, batch)), dim3(KB_BLK_WG), 0, s,
-                       vm, vn, side_left, a, b, x, info, iter, batch, ws);
+    // passes of 32 right-hand sides (four waves per SIMD) when the bound allows
+    if (MODE == KB_MODE_POTRF || (max_nrhs > 0 && max_nrhs <= KB_BLK_NARROW))
+        hipLaunchKernelGGL((k_vchol_blk<T, MODE, 2>), dim3(vchol_grid(k_vchol_blk<T, MODE, 2>, batch)),
+                           dim3(KB_BLK_WG), 0, s, vm, vn, side_left, a, b, x, info, iter, batch, ws);
+    else
+        hipLaunchKernelGGL((k_vchol_blk<T, MODE, 4>), dim3(vchol_grid(k_vchol_blk<T, MODE, 4>, batch)),
+                           dim3(KB_BLK_WG), 0, s, vm, vn, side_left, a, b, x, info, iter, batch, ws);
     hipLaunchKernelGGL((k_vchol_small<T, MODE>), dim3(vchol_grid(k_vchol_small<T, MODE>, (batch + 3) / 4)),
                        dim3(KB_BLK_WG), 0, s, vm, vn, side_left, a, b, x, info, iter, batch, ws);
     return (int)hipGetLastError();
