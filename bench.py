@@ -72,6 +72,10 @@ CONFIGS = {
     # inverse family (SPD inverse), sized like c1; not part of the default run
     "poti": dict(op="poti", n=32, batch=10000, nrhs=0, gen="shift", seed=1240,
                  workload="dpoti_batch_strided n=32 x 10000 (A := A^-1, POTRF then POTRI), A=B*B^T+32I"),
+    # the shapes of the reference's published batch charts (batch 10240, m = n)
+    "sweep": dict(op="sweep", batch=10240, sizes=[8, 16, 32, 64, 128, 256], seed=6,
+                  workload="batch 10240, n = m = 8..256: dposv_batch_strided (R, nrhs = n), dtrsm_batch_strided "
+                           "(L, L, N, N), dsyrk_batch_strided (L, N), dgemm_batch_strided (N, N)"),
 }
 
 
@@ -798,6 +802,86 @@ def run_poti(torch, pykblas, h, cfg, K, W, with_cpu):
     return out
 
 
+# GFLOP/s read off the reference's published "BATCH DPOSV" / "BATCH DTRSM" charts (docs/KBLAS_handout.png,
+# batch 10240, m = n, unlabelled NVIDIA GPU, +-10 %): context for the sweep, not a target
+POSTER_GFS = {"dposv": {8: 54, 16: 93, 32: 113, 64: 142, 128: 204, 256: 267},
+              "dtrsm": {8: 44, 16: 85, 32: 110, 64: 140, 128: 199, 256: 259}}
+
+
+def flops_of(op, n):
+    """testing/flops.h: POTRF n^3/3 + n^2/2 + n/6; POTRS 2 nrhs n^2 (here nrhs = n);
+    TRSM (left, m = n) n m^2; SYRK k n (n + 1); GEMM 2 m n k."""
+    potrf = n ** 3 / 3.0 + n ** 2 / 2.0 + n / 6.0
+    return {"dposv": potrf + 2.0 * n ** 3, "dtrsm": float(n ** 3), "dsyrk": float(n * n * (n + 1)),
+            "dgemm": 2.0 * n ** 3}[op]
+
+
+def run_sweep(torch, pykblas, h, cfg, K, W):
+    """The reference's own benchmark shapes: batch 10240, m = n = 8..256,
+    DPOSV (side R, nrhs = n), DTRSM (L, L, N, N), DSYRK (L, N) and DGEMM (N, N),
+    each step a graph replay behind a spin gate, inputs restored between steps
+    (outside the timed region) where the call overwrites them."""
+    bc = cfg["batch"]
+    lib = h.lib
+    g = torch.Generator(device=DEV)
+    g.manual_seed(1234 + cfg["seed"])
+    info = torch.zeros(bc, dtype=torch.int32, device=DEV)
+    res = {}
+    for n in cfg["sizes"]:
+        NA = n * n
+        A0 = gen_shift(torch, g, bc, n)
+        B0 = torch.randn((bc, n, n), generator=g, device=DEV, dtype=torch.float64)
+        A, B = A0.clone(), B0.clone()
+        L = A0.clone()
+        assert lib.kblasDpotrf_batch_strided(h.h, b"L", n, L.data_ptr(), n, NA, bc,
+                                             info.data_ptr()) == pykblas.KBLAS_Success
+        torch.cuda.synchronize()
+        row = {}
+
+        def restore_ab(i):
+            A.copy_(A0)
+            B.copy_(B0)
+
+        def restore_b(i):
+            B.copy_(B0)
+
+        legs = {
+            "dposv": (lambda s: lib.kblasDposv_batch_strided(h.h, b"R", b"L", n, n, A.data_ptr(), n, NA,
+                                                             B.data_ptr(), n, NA, bc, info.data_ptr()),
+                      restore_ab),
+            "dtrsm": (lambda s: lib.kblasDtrsm_batch_strided(h.h, b"L", b"L", b"N", b"N", n, n, 1.0,
+                                                             L.data_ptr(), n, NA, B.data_ptr(), n, NA, bc),
+                      restore_b),
+            "dsyrk": (lambda s: lib.kblasDsyrk_batch_strided(h.h, b"L", b"N", n, n, -1.0, A0.data_ptr(), n, NA,
+                                                             1.0, B.data_ptr(), n, NA, bc), restore_b),
+            "dgemm": (lambda s: lib.kblasDgemm_batch_strided(h.h, b"N", b"N", n, n, n, -1.0, A0.data_ptr(), n,
+                                                             NA, L.data_ptr(), n, NA, 1.0, B.data_ptr(), n, NA,
+                                                             bc), restore_b),
+        }
+        for op, (call, restore) in legs.items():
+            def issue(s, call=call):
+                assert call(s) == pykblas.KBLAS_Success
+            r = Stepper(torch, h, issue, 1).run(K, W, lambda i: 0, restore=restore)
+            if op == "dposv":
+                assert int((info != 0).sum().item()) == 0
+            t = r["total_s"] / K
+            e = {"ms_per_step": 1e3 * t, "median_ms": r["median_ms"], "min_ms": r["min_ms"],
+                 "gflops": flops_of(op, n) * bc / t / 1e9}
+            if op in POSTER_GFS:
+                e["reference_published_gflops"] = POSTER_GFS[op][n]
+            row[op] = e
+        res[str(n)] = row
+        log("sweep n=%d: %s" % (n, ", ".join("%s %.0f GF/s" % (k, v["gflops"]) for k, v in row.items())))
+        del A0, B0, A, B, L
+        torch.cuda.empty_cache()
+    top = res[str(cfg["sizes"][-1])]["dposv"]
+    return {"workload": cfg["workload"], "batch": bc, "n": cfg["sizes"][-1], "nrhs": cfg["sizes"][-1],
+            "value": top["gflops"], "unit": "GFLOP/s", "ms_per_step": top["ms_per_step"],
+            "fp64_peak_frac": top["gflops"] / FP64_PEAK_GFS, "sizes": res,
+            "flop_model": "testing/flops.h (POSV = POTRF + 2 TRSM with nrhs = n; TRSM left m^2 n; SYRK k n (n+1); "
+                          "GEMM 2mnk)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -829,6 +913,8 @@ def main():
             results[name] = run_vbatch(torch, pykblas, h, name, cfg, K, W, not args.no_cpu)
         elif cfg["op"] == "poti":
             results[name] = run_poti(torch, pykblas, h, cfg, K, W, not args.no_cpu)
+        elif cfg["op"] == "sweep":
+            results[name] = run_sweep(torch, pykblas, h, cfg, K, W)
         else:
             results[name] = run_uniform(torch, pykblas, h, name, cfg, K, W, not args.no_cpu, args.cpu_sample)
         torch.cuda.synchronize()
@@ -850,18 +936,19 @@ def main():
     cfg = CONFIGS[head_name]
     metric = {
         "sposv": "solves/s (batched POSV n=64 mixed FP32+IR, bwd err<=1e-13)",
-        "potrf": "factorizations/s (batched DPOTRF n=%d)" % cfg["n"],
-        "posv": "solves/s (batched DPOSV n=%d, %d RHS)" % (cfg["n"], cfg["nrhs"]),
+        "potrf": "factorizations/s (batched DPOTRF n=%d)" % cfg.get("n", 0),
+        "posv": "solves/s (batched DPOSV n=%d, %d RHS)" % (cfg.get("n", 0), cfg.get("nrhs", 0)),
         "vposv": "solves/s (non-uniform batched DPOSV n~U{8..256}, 4 RHS)",
         "vsposv": "solves/s (non-uniform mixed DSPOSV n~U{128..512}, 64 RHS)",
-        "poti": "inversions/s (batched DPOTI n=%d)" % cfg["n"],
+        "poti": "inversions/s (batched DPOTI n=%d)" % cfg.get("n", 0),
+        "sweep": "GFLOP/s (batched DPOSV n=nrhs=256 x 10240, with the DPOSV/DTRSM/DSYRK/DGEMM sweep n=8..256)",
     }[cfg["op"]]
     line = {
         "metric": metric, "value": head["value"], "unit": head["unit"], "n_gpus": 1, "steps": K, "warmup": W,
         "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+f64" if cfg["op"] in ("sposv", "vsposv") else "f64",
         "data": "synthetic (seeded, generated on the GPU)",
-        "config": {"workload": cfg["workload"], "batch": cfg["batch"], "n": cfg["n"], "nrhs": cfg["nrhs"]},
+        "config": {"workload": cfg["workload"], "batch": cfg["batch"], "n": head["n"], "nrhs": head.get("nrhs", 0)},
     }
     for k, v in head.items():
         if k not in ("workload", "batch", "n", "nrhs", "value", "unit", "ms_per_step"):

@@ -254,6 +254,13 @@ template <typename T>
 int trsm_t(int side_left, int trans, int m, int n, T alpha, kb_mat_t A, kb_mat_t B,
            const int* vm, const int* vn, int batch, hipStream_t s, int upper = 0, int unit = 0)
 {
+    if constexpr (__is_same(T, double) || __is_same(T, float)) {
+        if (!vm && !vn) {   // uniform real batches: the register / blocked kernels of kb_trsm.hip
+            const int e = kb_launch_trsm_fast(sizeof(T) == 8 ? 'd' : 's', side_left, upper, trans, unit, m, n,
+                                              (double)alpha, A, B, batch, s);
+            if (e != (int)hipErrorNotSupported) return e;
+        }
+    }
     int gy;
     if (vm) {
         gy = 4;

@@ -7,6 +7,12 @@ extern "C" int kb_blk_chol_d(int mode, int n, int nrhs, int side_left, kb_mat_t 
     return launch_blk<double>(mode, n, nrhs, side_left, A, B, info, batch, s);
 }
 
+extern "C" int kb_blk_trsm_d(int fwd, int side_left, int unit, int n, int nrhs, double alpha, kb_mat_t A,
+                              kb_mat_t B, int batch, hipStream_t s)
+{
+    return launch_blk_trsm<double>(fwd, side_left, unit, n, nrhs, (double)alpha, A, B, batch, s);
+}
+
 // Mixed-precision POSV entry for orders above the register range: LAPACK
 // dsposv's full-precision path (ITER = -1, "fell back to full precision for
 // implementation reasons"): X := B (and ITER := -1) here, then the FP64 POSV

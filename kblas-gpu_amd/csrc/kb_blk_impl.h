@@ -149,7 +149,7 @@ KB_DEV void blk_diag_fetch(T (&v)[KB_BLK_DPT], const T* L, int lda, int k0, int 
 }
 
 template <typename T>
-KB_DEV void blk_diag_put(const T (&v)[KB_BLK_DPT], int kb, BlkSmem<T>& S)
+KB_DEV void blk_diag_put(const T (&v)[KB_BLK_DPT], int kb, BlkSmem<T>& S, int unit = 0)
 {
     constexpr int NBK = KB_BLK_NBK;
     int tid = threadIdx.x;
@@ -159,7 +159,7 @@ KB_DEV void blk_diag_put(const T (&v)[KB_BLK_DPT], int kb, BlkSmem<T>& S)
         const int e = tid + KB_BLK_WG * m;
         const int r = e % NBK, c = e / NBK;
         S.sD[r * (NBK + 1) + c] = v[m];
-        if (r == c) S.sInv[r] = (r < kb) ? T(1) / v[m] : T(1);
+        if (r == c) S.sInv[r] = (r < kb && !unit) ? T(1) / v[m] : T(1);
     }
     __syncthreads();
 }
@@ -265,9 +265,10 @@ KB_DEV int blk_drow(int q, int g)
 // li and 16 + li, and y(j) comes from lane j by DPP; the NS independent
 // substitutions are interleaved and share the LDS reads of L.
 // FWD: L_D y = b, else L_D^T x = y; the result also goes to sY (right-hand
-// side major) for the off-diagonal update that follows.
+// side major) for the off-diagonal update that follows.  b is read as sc * b.
 template <typename T, int NS, bool FWD>
-KB_DEV void blk_diag_subst(T* B, int ldb, int side_left, int k0, int kb, int c0, int rc, BlkSmem<T>& S, T* sY)
+KB_DEV void blk_diag_subst(T* B, int ldb, int side_left, int k0, int kb, int c0, int rc, BlkSmem<T>& S, T* sY,
+                           T sc = T(1))
 {
     constexpr int NBK = KB_BLK_NBK;
     int tid = threadIdx.x;
@@ -287,7 +288,7 @@ KB_DEV void blk_diag_subst(T* B, int ldb, int side_left, int k0, int kb, int c0,
         for (int h = 0; h < 2; ++h) {
             const int r = 16 * h + li;
             const bool in = c < rc && r < kb;
-            y[s][h] = in ? gld(rhs_elem(B, ldb, side_left, in ? k0 + r : 0, in ? c0 + c : 0)) : T(0);
+            y[s][h] = in ? sc * gld(rhs_elem(B, ldb, side_left, in ? k0 + r : 0, in ? c0 + c : 0)) : T(0);
         }
     }
 #pragma unroll
@@ -326,20 +327,20 @@ KB_DEV void blk_diag_subst(T* B, int ldb, int side_left, int k0, int kb, int c0,
 // ns groups of 16 right-hand sides, ns <= CB (the widest pass instantiated)
 template <typename T, bool FWD, int CB>
 KB_DEV void blk_diag_subst_n(int ns, T* B, int ldb, int side_left, int k0, int kb, int c0, int rc, BlkSmem<T>& S,
-                             T* sY)
+                             T* sY, T sc = T(1))
 {
     if (CB == 2 || ns <= 2) {
-        if (ns == 1) blk_diag_subst<T, 1, FWD>(B, ldb, side_left, k0, kb, c0, rc, S, sY);
-        else blk_diag_subst<T, 2, FWD>(B, ldb, side_left, k0, kb, c0, rc, S, sY);
+        if (ns == 1) blk_diag_subst<T, 1, FWD>(B, ldb, side_left, k0, kb, c0, rc, S, sY, sc);
+        else blk_diag_subst<T, 2, FWD>(B, ldb, side_left, k0, kb, c0, rc, S, sY, sc);
         return;
     }
     if constexpr (CB > 2) {
-        if (ns == 3) blk_diag_subst<T, 3, FWD>(B, ldb, side_left, k0, kb, c0, rc, S, sY);
-        else blk_diag_subst<T, 4, FWD>(B, ldb, side_left, k0, kb, c0, rc, S, sY);
+        if (ns == 3) blk_diag_subst<T, 3, FWD>(B, ldb, side_left, k0, kb, c0, rc, S, sY, sc);
+        else blk_diag_subst<T, 4, FWD>(B, ldb, side_left, k0, kb, c0, rc, S, sY, sc);
     }
 }
 
-// B(r, c) -= sum_k Lk(r, k) Y(k, c) for the rows r in [r_lo, r_hi), k < kb and
+// B(r, c) := sc B(r, c) - sum_k Lk(r, k) Y(k, c) for the rows r in [r_lo, r_hi), k < kb and
 // the pass's right-hand sides, on the matrix cores.  Lk(r, k) = L(r, k0 + k)
 // (TRANS false: the block column below the diagonal block, forward sweep) or
 // L(k0 + k, r) (TRANS true: the block row left of it, backward sweep); Y is
@@ -348,7 +349,7 @@ KB_DEV void blk_diag_subst_n(int ns, T* B, int ldb, int side_left, int k0, int k
 // 16 right-hand sides are loaded together, then NCB independent MFMA chains.
 template <typename T, bool TRANS, int NCB>
 KB_DEV void blk_update_rows(const T* L, int lda, T* B, int ldb, int side_left, int k0, int kb, int r_lo,
-                            int r_hi, int c0, int rc, const T* sY)
+                            int r_hi, int c0, int rc, const T* sY, T sc = T(1))
 {
     constexpr int NBK = KB_BLK_NBK;
     int tid = threadIdx.x;
@@ -397,7 +398,7 @@ KB_DEV void blk_update_rows(const T* L, int lda, T* B, int ldb, int side_left, i
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int row = r0 + blk_drow<T>(q, lg);
-                if (row < r_hi && cc < rc) gst(rhs_elem(B, ldb, side_left, row, c0 + cc), bv[cb][q] - acc[cb][q]);
+                if (row < r_hi && cc < rc) gst(rhs_elem(B, ldb, side_left, row, c0 + cc), sc * bv[cb][q] - acc[cb][q]);
             }
         }
     }
@@ -405,16 +406,16 @@ KB_DEV void blk_update_rows(const T* L, int lda, T* B, int ldb, int side_left, i
 
 template <typename T, bool TRANS, int CB>
 KB_DEV void blk_update_rows_n(int ncb, const T* L, int lda, T* B, int ldb, int side_left, int k0, int kb, int r_lo,
-                              int r_hi, int c0, int rc, const T* sY)
+                              int r_hi, int c0, int rc, const T* sY, T sc = T(1))
 {
     if (CB == 2 || ncb <= 2) {
-        if (ncb == 1) blk_update_rows<T, TRANS, 1>(L, lda, B, ldb, side_left, k0, kb, r_lo, r_hi, c0, rc, sY);
-        else blk_update_rows<T, TRANS, 2>(L, lda, B, ldb, side_left, k0, kb, r_lo, r_hi, c0, rc, sY);
+        if (ncb == 1) blk_update_rows<T, TRANS, 1>(L, lda, B, ldb, side_left, k0, kb, r_lo, r_hi, c0, rc, sY, sc);
+        else blk_update_rows<T, TRANS, 2>(L, lda, B, ldb, side_left, k0, kb, r_lo, r_hi, c0, rc, sY, sc);
         return;
     }
     if constexpr (CB > 2) {
-        if (ncb == 3) blk_update_rows<T, TRANS, 3>(L, lda, B, ldb, side_left, k0, kb, r_lo, r_hi, c0, rc, sY);
-        else blk_update_rows<T, TRANS, 4>(L, lda, B, ldb, side_left, k0, kb, r_lo, r_hi, c0, rc, sY);
+        if (ncb == 3) blk_update_rows<T, TRANS, 3>(L, lda, B, ldb, side_left, k0, kb, r_lo, r_hi, c0, rc, sY, sc);
+        else blk_update_rows<T, TRANS, 4>(L, lda, B, ldb, side_left, k0, kb, r_lo, r_hi, c0, rc, sY, sc);
     }
 }
 
@@ -422,9 +423,10 @@ KB_DEV void blk_update_rows_n(int ncb, const T* L, int lda, T* B, int ldb, int s
 // diagonal block staged in sD / sInv, then the rows below updated with the
 // block column of L (all right-hand sides, in passes of up to 16 CB: a
 // narrower pass needs fewer registers, a wider one streams L fewer times).
+// sc scales the right-hand sides as they are first read (the first step).
 template <typename T, int CB = 4>
 KB_DEV void blk_fwd_step(const T* L, int lda, int n, int k0, int kb, T* B, int ldb, int side_left, int nrhs,
-                         BlkSmem<T>& S)
+                         BlkSmem<T>& S, T sc = T(1))
 {
     constexpr int RC = 16 * CB;
     static_assert(RC <= KB_BLK_RC, "the staging holds KB_BLK_RC right-hand sides");
@@ -432,33 +434,36 @@ KB_DEV void blk_fwd_step(const T* L, int lda, int n, int k0, int kb, T* B, int l
     for (int c0 = 0; c0 < nrhs; c0 += RC) {
         const int rc = min(RC, nrhs - c0);
         const int ncb = (rc + 15) / 16;
-        blk_diag_subst_n<T, true, CB>(ncb, B, ldb, side_left, k0, kb, c0, rc, S, sY);
+        blk_diag_subst_n<T, true, CB>(ncb, B, ldb, side_left, k0, kb, c0, rc, S, sY, sc);
         __syncthreads();
-        blk_update_rows_n<T, false, CB>(ncb, L, lda, B, ldb, side_left, k0, kb, k0 + kb, n, c0, rc, sY);
+        blk_update_rows_n<T, false, CB>(ncb, L, lda, B, ldb, side_left, k0, kb, k0 + kb, n, c0, rc, sY, sc);
         __syncthreads();
     }
 }
 
-// Forward sweep L Y = B over 32-row blocks (right-looking), the next
-// diagonal block fetched while the current one is used.
+// Forward sweep L Y = alpha B over 32-row blocks (right-looking), the next
+// diagonal block fetched while the current one is used; unit: the diagonal
+// of L is taken as 1 (not read).
 template <typename T, int CB = 4>
-KB_DEV void blk_solve_fwd(const T* L, int lda, int n, T* B, int ldb, int side_left, int nrhs, BlkSmem<T>& S)
+KB_DEV void blk_solve_fwd(const T* L, int lda, int n, T* B, int ldb, int side_left, int nrhs, BlkSmem<T>& S,
+                          T alpha = T(1), int unit = 0)
 {
     constexpr int NBK = KB_BLK_NBK;
     T dnext[KB_BLK_DPT];
     blk_diag_fetch<T>(dnext, L, lda, 0, min(NBK, n));
     for (int k0 = 0; k0 < n; k0 += NBK) {
         const int kb = min(NBK, n - k0);
-        blk_diag_put<T>(dnext, kb, S);
+        blk_diag_put<T>(dnext, kb, S, unit);
         if (k0 + NBK < n) blk_diag_fetch<T>(dnext, L, lda, k0 + NBK, min(NBK, n - k0 - NBK));
-        blk_fwd_step<T, CB>(L, lda, n, k0, kb, B, ldb, side_left, nrhs, S);
+        blk_fwd_step<T, CB>(L, lda, n, k0, kb, B, ldb, side_left, nrhs, S, k0 == 0 ? alpha : T(1));
     }
 }
 
-// Backward sweep L^T X = Y over 32-row blocks (right-looking: after each
-// block's substitution the rows above it are updated).
+// Backward sweep L^T X = alpha Y over 32-row blocks (right-looking: after
+// each block's substitution the rows above it are updated); unit as above.
 template <typename T, int CB = 4>
-KB_DEV void blk_solve_bwd(const T* L, int lda, int n, T* B, int ldb, int side_left, int nrhs, BlkSmem<T>& S)
+KB_DEV void blk_solve_bwd(const T* L, int lda, int n, T* B, int ldb, int side_left, int nrhs, BlkSmem<T>& S,
+                          T alpha = T(1), int unit = 0)
 {
     constexpr int NBK = KB_BLK_NBK, RC = 16 * CB;
     T* sY = S.sY;
@@ -468,14 +473,15 @@ KB_DEV void blk_solve_bwd(const T* L, int lda, int n, T* B, int ldb, int side_le
     for (int bk = nblk - 1; bk >= 0; --bk) {
         const int k0 = bk * NBK;
         const int kb = min(NBK, n - k0);
-        blk_diag_put<T>(dnext, kb, S);
+        blk_diag_put<T>(dnext, kb, S, unit);
         if (bk > 0) blk_diag_fetch<T>(dnext, L, lda, k0 - NBK, NBK);
+        const T sc = bk == nblk - 1 ? alpha : T(1);
         for (int c0 = 0; c0 < nrhs; c0 += RC) {
             const int rc = min(RC, nrhs - c0);
             const int ncb = (rc + 15) / 16;
-            blk_diag_subst_n<T, false, CB>(ncb, B, ldb, side_left, k0, kb, c0, rc, S, sY);
+            blk_diag_subst_n<T, false, CB>(ncb, B, ldb, side_left, k0, kb, c0, rc, S, sY, sc);
             __syncthreads();
-            blk_update_rows_n<T, true, CB>(ncb, L, lda, B, ldb, side_left, k0, kb, 0, k0, c0, rc, sY);
+            blk_update_rows_n<T, true, CB>(ncb, L, lda, B, ldb, side_left, k0, kb, 0, k0, c0, rc, sY, sc);
             __syncthreads();
         }
     }
@@ -562,6 +568,46 @@ static void launch_blk_mode(int n, int nrhs, int side_left, BatchArg<T> a, Batch
     else
         hipLaunchKernelGGL((k_blk_chol<T, MODE, 4>), dim3(grid), dim3(KB_BLK_WG), 0, s, n, nrhs, side_left, a, b, info,
                            batch);
+}
+
+// Batched lower TRSM for orders above the register range: one sweep of the
+// blocked solve, FWD: L X = alpha B, else L^T X = alpha B, for the nrhs
+// vectors of B (side_left: columns, else rows, i.e. X op(L) = alpha B as
+// op(L)^T X^T = alpha B^T).
+template <typename T, bool FWD, int CB>
+__global__ void __launch_bounds__(KB_BLK_WG) __attribute__((amdgpu_waves_per_eu(CB == 2 ? 4 : 2)))
+k_blk_trsm(int n, int nrhs, int side_left, int unit, T alpha, BatchArg<T> A, BatchArg<T> B, int batch)
+{
+    __shared__ BlkSmem<T> S;
+    for (long mat = blockIdx.x; mat < batch; mat += gridDim.x) {
+        if constexpr (FWD)
+            blk_solve_fwd<T, CB>(A.get(mat), A.ldim(mat), n, B.get(mat), B.ldim(mat), side_left, nrhs, S, alpha, unit);
+        else
+            blk_solve_bwd<T, CB>(A.get(mat), A.ldim(mat), n, B.get(mat), B.ldim(mat), side_left, nrhs, S, alpha, unit);
+        __syncthreads();
+    }
+}
+
+template <typename T>
+static int launch_blk_trsm(int fwd, int side_left, int unit, int n, int nrhs, T alpha, kb_mat_t A, kb_mat_t B,
+                           int batch, hipStream_t s)
+{
+    const int grid = batch < 2048 ? batch : 2048;
+    BatchArg<T> a = to_arg<T>(A), b = to_arg<T>(B);
+    const bool narrow = nrhs <= KB_BLK_NARROW;
+    if (fwd && narrow)
+        hipLaunchKernelGGL((k_blk_trsm<T, true, 2>), dim3(grid), dim3(KB_BLK_WG), 0, s, n, nrhs, side_left, unit,
+                           alpha, a, b, batch);
+    else if (fwd)
+        hipLaunchKernelGGL((k_blk_trsm<T, true, 4>), dim3(grid), dim3(KB_BLK_WG), 0, s, n, nrhs, side_left, unit,
+                           alpha, a, b, batch);
+    else if (narrow)
+        hipLaunchKernelGGL((k_blk_trsm<T, false, 2>), dim3(grid), dim3(KB_BLK_WG), 0, s, n, nrhs, side_left, unit,
+                           alpha, a, b, batch);
+    else
+        hipLaunchKernelGGL((k_blk_trsm<T, false, 4>), dim3(grid), dim3(KB_BLK_WG), 0, s, n, nrhs, side_left, unit,
+                           alpha, a, b, batch);
+    return (int)hipGetLastError();
 }
 
 template <typename T>

@@ -144,6 +144,11 @@ int kb_launch_syrk_x(char prec, int upper, int trans, int m, int k, const double
 int kb_launch_trsm_x(char prec, int side_left, int upper, int trans, int unit, int m, int n,
                      const double* alpha, kb_mat_t A, kb_mat_t B, const int* vm, const int* vn, int batch,
                      hipStream_t s);
+/* Real uniform TRSM (prec 'd' / 's', trans 0 = N, else T): orders up to 64 by
+ * register substitution with the triangle in LDS, lower orders above 64 by the
+ * blocked solve; hipErrorNotSupported for an upper triangle above 64. */
+int kb_launch_trsm_fast(char prec, int side_left, int upper, int trans, int unit, int m, int n, double alpha,
+                        kb_mat_t A, kb_mat_t B, int batch, hipStream_t s);
 /* Generic POTRF / POTRS / POSV (mode KB_MODE_*) for any precision and either
  * triangle; info per matrix for POTRF / POSV. */
 int kb_launch_gchol(char prec, int mode, int upper, int n, int nrhs, int side_left, kb_mat_t A, kb_mat_t B,

@@ -1,0 +1,131 @@
+"""GPU parity tests of the batched real TRSM kernels (kb_trsm.hip): the
+register kernel for orders up to 64 and one sweep of the blocked solve above
+it, against the oracle's TRSM (orc_trsm for lower non-unit triangles,
+orc_ztrsm on real data for upper and unit ones).
+
+The shapes cross every order bucket of the register kernel (8, 16, 32, 64,
+orders just above and below each), vector counts below, at and above one
+wavefront (one to 130), both sides and transpositions, alpha != 1, strided
+and pointer-array operands with leading dimensions above the order, NaN in
+the unreferenced triangle and in the rows between the order and the leading
+dimension (never read, never written), and odd batch counts so that the last
+wavefront holds groups with no matrix.  Bar: relative max-norm error within
+64 * order * eps of the working precision."""
+import numpy as np
+import pytest
+
+import kbtest as kt
+import pykblas
+import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+PREC = {"D": (np.float64, kt.EPS64), "S": (np.float32, kt.EPS32)}
+
+
+@pytest.fixture(scope="module")
+def h():
+    hh = kt.handle()
+    yield hh
+    hh.close()
+
+
+def triangle(rng, k, upper, unit):
+    T = rng.standard_normal((k, k)) / k + np.diag(1.0 + rng.uniform(0.0, 1.0, k))
+    T = np.triu(T) if upper else np.tril(T)
+    T[np.tril_indices(k, -1) if upper else np.triu_indices(k, 1)] = np.nan
+    if unit:
+        T[np.diag_indices(k)] = np.nan     # a unit diagonal is not referenced
+    return T
+
+
+def stage(mats, ld, dt):
+    """column-major matrices with leading dimension ld, NaN padding"""
+    r, c = mats[0].shape
+    buf = np.full((len(mats), c, ld), np.nan)
+    for i, M in enumerate(mats):
+        buf[i, :, :r] = M.T
+    return kt.to_dev(buf.astype(dt)), c * ld
+
+
+def unstage(t, b, r):
+    return kt.to_host(t)[b].T[:r].astype(np.float64)
+
+
+def run_case(h, p, side, upper, trans, unit, m, n, batch, seed, ptrs):
+    dt, eps = PREC[p]
+    rng = np.random.default_rng(seed)
+    k = m if side == "L" else n
+    Ts = [triangle(rng, k, upper, unit) for _ in range(batch)]
+    Bs = [rng.standard_normal((m, n)) for _ in range(batch)]
+    if dt == np.float32:
+        Ts = [T.astype(np.float32).astype(np.float64) for T in Ts]
+        Bs = [B.astype(np.float32).astype(np.float64) for B in Bs]
+    lda, ldb = k + 3, m + 2
+    dT, sT = stage(Ts, lda, dt)
+    dB, sB = stage(Bs, ldb, dt)
+    alpha = 0.75
+    if ptrs:
+        pT, pB = kt.ptr_array(dT, batch, sT), kt.ptr_array(dB, batch, sB)
+        st = getattr(h.lib, "kblas%strsm_batch" % p)(h.h, side.encode(), b"U" if upper else b"L", trans.encode(),
+                                                     b"U" if unit else b"N", m, n, alpha, kt.P(pT), lda, kt.P(pB),
+                                                     ldb, batch)
+    else:
+        st = getattr(h.lib, "kblas%strsm_batch_strided" % p)(h.h, side.encode(), b"U" if upper else b"L",
+                                                             trans.encode(), b"U" if unit else b"N", m, n, alpha,
+                                                             kt.P(dT), lda, sT, kt.P(dB), ldb, sB, batch)
+    assert st == pykblas.KBLAS_Success
+    Xall = kt.to_host(dB)
+    for b in range(batch):
+        X = Xall[b].T.astype(np.float64)
+        T = np.nan_to_num(Ts[b])
+        if upper or unit:
+            Xo = po.ztrsm(side, upper, trans, unit, alpha, T.astype(np.complex128), Bs[b]).real
+        else:
+            Xo = po.trsm(side, trans, alpha, T, Bs[b])
+        err = np.max(np.abs(X[:m] - Xo)) / max(np.max(np.abs(Xo)), 1e-300)
+        assert err <= 64 * k * eps, (b, err)
+        assert np.all(np.isnan(X[m:])), b
+
+
+# order -> vector count: below, at and above one wavefront across the buckets
+REG_SHAPES = [(1, 5), (3, 64), (8, 8), (9, 65), (16, 3), (17, 16), (31, 1), (32, 130), (33, 2), (48, 32), (63, 7),
+              (64, 64)]
+
+
+@pytest.mark.parametrize("p", ["D", "S"])
+@pytest.mark.parametrize("side", ["L", "R"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("shape", REG_SHAPES)
+def test_register_trsm_lower(h, p, side, trans, shape):
+    k, nv = shape
+    m, n = (k, nv) if side == "L" else (nv, k)
+    run_case(h, p, side, False, trans, False, m, n, batch=11, seed=k * 131 + nv, ptrs=(k % 2 == 1))
+
+
+@pytest.mark.parametrize("side", ["L", "R"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("upper,unit", [(True, False), (False, True), (True, True)])
+@pytest.mark.parametrize("shape", [(5, 9), (24, 70), (64, 3)])
+def test_register_trsm_upper_unit(h, side, trans, upper, unit, shape):
+    k, nv = shape
+    m, n = (k, nv) if side == "L" else (nv, k)
+    run_case(h, "D", side, upper, trans, unit, m, n, batch=5, seed=7 * k + nv + 3 * upper, ptrs=unit)
+
+
+@pytest.mark.parametrize("p", ["D", "S"])
+@pytest.mark.parametrize("side", ["L", "R"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("shape", [(65, 1), (96, 33), (130, 64), (257, 100)])
+def test_blocked_trsm_lower(h, p, side, trans, shape):
+    k, nv = shape
+    m, n = (k, nv) if side == "L" else (nv, k)
+    run_case(h, p, side, False, trans, False, m, n, batch=3, seed=k + nv, ptrs=(nv % 2 == 1))
+
+
+@pytest.mark.parametrize("side", ["L", "R"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+def test_blocked_trsm_unit_and_upper(h, side, trans):
+    # lower unit: the blocked sweep with a unit diagonal; upper above 64: the generic kernel
+    run_case(h, "D", side, False, trans, True, *((100, 20) if side == "L" else (20, 100)), batch=3, seed=5, ptrs=False)
+    run_case(h, "D", side, True, trans, False, *((90, 12) if side == "L" else (12, 90)), batch=2, seed=6, ptrs=True)
