@@ -232,6 +232,13 @@ int gemm_t(int ta, int tb, int m, int n, int k, T alpha, kb_mat_t A, kb_mat_t B,
            kb_mat_t C, const int* vm, const int* vn, const int* vk, int batch, hipStream_t s, bool syrk,
            int upper = 0)
 {
+    if constexpr (__is_same(T, double) || __is_same(T, float)) {
+        if (!vm && !vn && !vk) {   // uniform real batches: the matrix-core kernel of kb_gemm.hip
+            const int e = kb_launch_gemm_fast(sizeof(T) == 8 ? 'd' : 's', syrk, upper, ta, syrk ? !ta : tb, m, n, k,
+                                              (double)alpha, A, B, (double)beta, C, batch, s);
+            if (e != (int)hipErrorNotSupported) return e;
+        }
+    }
     int gy;
     if (vm) {
         gy = 16;

@@ -1,0 +1,182 @@
+// kb_gemm.hip -- batched real GEMM and SYRK on the matrix cores (uniform
+// batches), the products the reference delegates to cuBLAS
+// (src/batch_triangular/Xgemm_batch_core.cuh:170-313,492-634) and its SYRK
+// recursion (Xsyrk_batch_drivers.cuh:32-119,125-433):
+//   GEMM  C := alpha op(A) op(B) + beta C
+//   SYRK  the lower (or upper) triangle of C := alpha op(A) op(A)^T + beta C
+//
+// One wavefront per TM x TM output tile (TM = 16 or 32), the tile built from
+// 16 x 16 x 4 matrix-core products (FP64 or FP32).  The product is formed
+// transposed, D = op(B)^T op(A)^T, so that the accumulator's lanes run down
+// the columns of C and the read-modify-write of C is coalesced.  Operand
+// fragments are loaded straight from memory, 4 or 8 K steps deep, with the next
+// chunk's loads issued before the current chunk's products.
+//
+// Launch order: workgroup w goes to XCD w mod 8, so the tiles of one matrix
+// are numbered 8 apart (matrix b on XCD b mod 8) and dispatched together:
+// every tile of a matrix reads its A and B panels through the same L2 while
+// they are still there.
+#include "kb_device.h"
+#include "kb_launch.h"
+
+namespace {
+
+// K steps of 4 per chunk: two chunks of fragments in flight fit in about 128
+// registers (four waves per SIMD) at 8 steps, except FP64 32 x 32 tiles (4)
+template <typename T, int TM>
+constexpr int gm_ku() { return (TM == 32 && sizeof(T) == 8) ? 4 : 8; }
+
+template <typename T, int NB, int KU, bool TA, bool TB>
+KB_DEV void gm_load(const T* Ap, int lda, const T* Bp, int ldb, int m, int n, int k, int I0, int J0, int p0, int li,
+                    int lg, T (&a)[KU][NB], T (&b)[KU][NB])
+{
+#pragma unroll
+    for (int s = 0; s < KU; ++s) {
+        const int kk = p0 + 4 * s + lg;
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+            const int r = I0 + 16 * q + li;   // row of op(A)
+            const bool ia = r < m && kk < k;
+            a[s][q] = ia ? gld(Ap + (TA ? kk + (long)r * lda : r + (long)kk * lda)) : T(0);
+            const int c = J0 + 16 * q + li;   // column of op(B)
+            const bool ib = c < n && kk < k;
+            b[s][q] = ib ? gld(Bp + (TB ? c + (long)kk * ldb : kk + (long)c * ldb)) : T(0);
+        }
+    }
+}
+
+template <typename T, int TM, bool TA, bool TB>
+__global__ void __launch_bounds__(64)
+k_gemm_mfma(int m, int n, int k, T alpha, BatchArg<T> A, BatchArg<T> B, T beta, BatchArg<T> C, int batch,
+            int tiles, int tm, int syrk, int upper)
+{
+    constexpr int NB = TM / 16, KU = gm_ku<T, TM>();
+    typedef typename TileV<T>::v4 v4;
+    const int lane = threadIdx.x, li = lane & 15, lg = lane >> 4;
+    // workgroup -> (matrix, tile): matrices in groups of 8, tile-major inside a group
+    const long w = blockIdx.x;
+    const long grp = w / (8L * tiles);
+    const int r8 = (int)(w - grp * 8L * tiles);
+    const int t = r8 >> 3;
+    const long mat = grp * 8 + (r8 & 7);
+    if (mat >= batch) return;
+    int I, J;
+    if (syrk) {
+        // lower tiles (I, J <= I) in row order; upper: the transposed tile
+        I = 0;
+        while ((I + 1) * (I + 2) / 2 <= t) ++I;
+        J = t - I * (I + 1) / 2;
+        if (upper) {
+            const int u = I;
+            I = J;
+            J = u;
+        }
+    } else {
+        I = t % tm;
+        J = t / tm;
+    }
+    const int I0 = I * TM, J0 = J * TM;
+    const T* Ap = A.get(mat);
+    const T* Bp = B.get(mat);
+    T* Cp = C.get(mat);
+    const int lda = A.ldim(mat), ldb = B.ldim(mat), ldc = C.ldim(mat);
+    v4 acc[NB][NB];
+#pragma unroll
+    for (int x = 0; x < NB; ++x)
+#pragma unroll
+        for (int y = 0; y < NB; ++y) acc[x][y] = v4{T(0), T(0), T(0), T(0)};
+    if (alpha != T(0) && k > 0) {
+        T a[KU][NB], b[KU][NB], an[KU][NB], bn[KU][NB];
+        gm_load<T, NB, KU, TA, TB>(Ap, lda, Bp, ldb, m, n, k, I0, J0, 0, li, lg, a, b);
+        for (int p0 = 0; p0 < k; p0 += 4 * KU) {
+            const bool more = p0 + 4 * KU < k;
+            if (more) gm_load<T, NB, KU, TA, TB>(Ap, lda, Bp, ldb, m, n, k, I0, J0, p0 + 4 * KU, li, lg, an, bn);
+#pragma unroll
+            for (int s = 0; s < KU; ++s)
+#pragma unroll
+                for (int x = 0; x < NB; ++x)        // x: block of C rows (op(A) rows)
+#pragma unroll
+                    for (int y = 0; y < NB; ++y)    // y: block of C columns
+                        acc[x][y] = mfma16x16x4(b[s][y], a[s][x], acc[x][y]);
+            if (more) {
+#pragma unroll
+                for (int s = 0; s < KU; ++s)
+#pragma unroll
+                    for (int q = 0; q < NB; ++q) {
+                        a[s][q] = an[s][q];
+                        b[s][q] = bn[s][q];
+                    }
+            }
+        }
+    }
+    // acc[x][y] lane (li, lg) register e: C(I0 + 16x + li, J0 + 16y + col(e, lg))
+#pragma unroll
+    for (int x = 0; x < NB; ++x) {
+        const int r = I0 + 16 * x + li;
+#pragma unroll
+        for (int y = 0; y < NB; ++y) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int c = J0 + 16 * y + ((sizeof(T) == 8) ? 4 * e + lg : 4 * lg + e);
+                const bool in = r < m && c < n && (!syrk || (upper ? c >= r : c <= r));
+                if (in) {
+                    T* cp = Cp + r + (long)c * ldc;
+                    T v = alpha * acc[x][y][e];
+                    if (beta != T(0)) v = beta * gld(cp) + v;
+                    gst(cp, v);
+                }
+            }
+        }
+    }
+}
+
+template <typename T, int TM, bool TA, bool TB>
+int gm_launch(int m, int n, int k, T alpha, kb_mat_t A, kb_mat_t B, T beta, kb_mat_t C, int batch, int syrk,
+              int upper, hipStream_t s)
+{
+    const int tm = (m + TM - 1) / TM, tn = (n + TM - 1) / TM;
+    const int tiles = syrk ? tm * (tm + 1) / 2 : tm * tn;
+    const long groups = (batch + 7) / 8;
+    const long wgs = groups * 8L * tiles;
+    if (wgs >= (1L << 26)) return (int)hipErrorNotSupported;   // grid of at most 2^32 work-items
+    hipLaunchKernelGGL((k_gemm_mfma<T, TM, TA, TB>), dim3((unsigned)wgs), dim3(64), 0, s, m, n, k, alpha,
+                       to_arg<T>(A), to_arg<T>(B), beta, to_arg<T>(C), batch, tiles, tm, syrk, upper);
+    return (int)hipGetLastError();
+}
+
+template <typename T, int TM>
+int gm_trans(int ta, int tb, int m, int n, int k, T alpha, kb_mat_t A, kb_mat_t B, T beta, kb_mat_t C, int batch,
+             int syrk, int upper, hipStream_t s)
+{
+    if (!ta && !tb) return gm_launch<T, TM, false, false>(m, n, k, alpha, A, B, beta, C, batch, syrk, upper, s);
+    if (!ta && tb) return gm_launch<T, TM, false, true>(m, n, k, alpha, A, B, beta, C, batch, syrk, upper, s);
+    if (ta && !tb) return gm_launch<T, TM, true, false>(m, n, k, alpha, A, B, beta, C, batch, syrk, upper, s);
+    return gm_launch<T, TM, true, true>(m, n, k, alpha, A, B, beta, C, batch, syrk, upper, s);
+}
+
+template <typename T>
+int gm_dispatch(int ta, int tb, int m, int n, int k, T alpha, kb_mat_t A, kb_mat_t B, T beta, kb_mat_t C,
+                int batch, int syrk, int upper, hipStream_t s)
+{
+    if (m <= 16 || n <= 16) return gm_trans<T, 16>(ta, tb, m, n, k, alpha, A, B, beta, C, batch, syrk, upper, s);
+    return gm_trans<T, 32>(ta, tb, m, n, k, alpha, A, B, beta, C, batch, syrk, upper, s);
+}
+
+}  // namespace
+
+// Real uniform GEMM / SYRK on the matrix cores ('d' or 's'; op 0 = N, else T).
+// SYRK (syrk != 0): C is m x m, B must be A and tb = !ta; upper selects the
+// triangle.  hipErrorNotSupported when the grid would be too large (the
+// caller then runs the general kernel).
+extern "C" int kb_launch_gemm_fast(char prec, int syrk, int upper, int ta, int tb, int m, int n, int k,
+                                   double alpha, kb_mat_t A, kb_mat_t B, double beta, kb_mat_t C, int batch,
+                                   hipStream_t s)
+{
+    if (batch < 1 || m < 1 || n < 1) return 0;
+    ta = ta != 0;
+    tb = tb != 0;
+    if (prec == 'd') return gm_dispatch<double>(ta, tb, m, n, k, alpha, A, B, beta, C, batch, syrk, upper, s);
+    if (prec == 's')
+        return gm_dispatch<float>(ta, tb, m, n, k, (float)alpha, A, B, (float)beta, C, batch, syrk, upper, s);
+    return (int)hipErrorInvalidValue;
+}

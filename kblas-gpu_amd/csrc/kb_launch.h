@@ -149,6 +149,11 @@ int kb_launch_trsm_x(char prec, int side_left, int upper, int trans, int unit, i
  * blocked solve; hipErrorNotSupported for an upper triangle above 64. */
 int kb_launch_trsm_fast(char prec, int side_left, int upper, int trans, int unit, int m, int n, double alpha,
                         kb_mat_t A, kb_mat_t B, int batch, hipStream_t s);
+/* Real uniform GEMM / SYRK on the matrix cores (prec 'd' / 's', op 0 = N, else
+ * T; syrk: C is m x m, B = A, tb = !ta, upper picks the triangle);
+ * hipErrorNotSupported when the batch is too large for one launch. */
+int kb_launch_gemm_fast(char prec, int syrk, int upper, int ta, int tb, int m, int n, int k, double alpha,
+                        kb_mat_t A, kb_mat_t B, double beta, kb_mat_t C, int batch, hipStream_t s);
 /* Generic POTRF / POTRS / POSV (mode KB_MODE_*) for any precision and either
  * triangle; info per matrix for POTRF / POSV. */
 int kb_launch_gchol(char prec, int mode, int upper, int n, int nrhs, int side_left, kb_mat_t A, kb_mat_t B,

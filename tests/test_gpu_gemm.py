@@ -1,0 +1,133 @@
+"""GPU parity tests of the matrix-core batched GEMM / SYRK kernel (kb_gemm.hip)
+against the oracle (orc_gemm, orc_syrk; orc_zsyrk on real data for the upper
+triangle).
+
+Shapes cross both tile sizes (16 and 32) and their edges, inner dimensions
+around the K chunk (1, 3, 31, 32, 33, 65, 130), every transposition pair,
+alpha / beta combinations including beta = 0 with NaN in C (C must not be
+read), batches that are not a multiple of 8 (the launch numbers matrices in
+groups of 8), leading dimensions above the rows with NaN padding (never read,
+never written) and pointer arrays.  SYRK leaves the other triangle of C
+bit-unchanged.  Bar: relative max-norm error within 16 * k * eps."""
+import numpy as np
+import pytest
+
+import kbtest as kt
+import pykblas
+import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+PREC = {"D": (np.float64, kt.EPS64), "S": (np.float32, kt.EPS32)}
+
+
+@pytest.fixture(scope="module")
+def h():
+    hh = kt.handle()
+    yield hh
+    hh.close()
+
+
+def stage(mats, ld, dt):
+    r, c = mats[0].shape
+    buf = np.full((len(mats), c, ld), np.nan)
+    for i, M in enumerate(mats):
+        buf[i, :, :r] = M.T
+    return kt.to_dev(buf.astype(dt)), c * ld
+
+
+def rnd(rng, shape, dt):
+    return rng.standard_normal(shape).astype(dt).astype(np.float64)
+
+
+GEMM_SHAPES = [(8, 8, 8), (16, 16, 1), (5, 17, 3), (17, 5, 31), (32, 32, 32), (33, 40, 33), (64, 64, 65),
+               (70, 31, 130), (100, 96, 16)]
+
+
+@pytest.mark.parametrize("p", ["D", "S"])
+@pytest.mark.parametrize("ta", ["N", "T"])
+@pytest.mark.parametrize("tb", ["N", "T"])
+@pytest.mark.parametrize("mnk", GEMM_SHAPES)
+def test_gemm(h, p, ta, tb, mnk):
+    dt, eps = PREC[p]
+    m, n, k = mnk
+    batch = 13
+    rng = np.random.default_rng(m * 7 + n * 3 + k)
+    sa = (m, k) if ta == "N" else (k, m)
+    sb = (k, n) if tb == "N" else (n, k)
+    As = [rnd(rng, sa, dt) for _ in range(batch)]
+    Bs = [rnd(rng, sb, dt) for _ in range(batch)]
+    Cs = [rnd(rng, (m, n), dt) for _ in range(batch)]
+    beta = 0.0 if k % 2 else 0.5
+    if beta == 0.0:
+        Cs = [np.full((m, n), np.nan) for _ in range(batch)]
+    lda, ldb, ldc = sa[0] + 1, sb[0] + 3, m + 2
+    dA, sA = stage(As, lda, dt)
+    dB, sB = stage(Bs, ldb, dt)
+    dC, sC = stage(Cs, ldc, dt)
+    alpha = -1.25
+    if m % 2:
+        pA, pB, pC = kt.ptr_array(dA, batch, sA), kt.ptr_array(dB, batch, sB), kt.ptr_array(dC, batch, sC)
+        st = getattr(h.lib, "kblas%sgemm_batch" % p)(h.h, ta.encode(), tb.encode(), m, n, k, alpha, kt.P(pA), lda,
+                                                     kt.P(pB), ldb, beta, kt.P(pC), ldc, batch)
+    else:
+        st = getattr(h.lib, "kblas%sgemm_batch_strided" % p)(h.h, ta.encode(), tb.encode(), m, n, k, alpha, kt.P(dA),
+                                                             lda, sA, kt.P(dB), ldb, sB, beta, kt.P(dC), ldc, sC,
+                                                             batch)
+    assert st == pykblas.KBLAS_Success
+    R = kt.to_host(dC)
+    for b in range(batch):
+        X = R[b].T.astype(np.float64)
+        Co = po.gemm(ta, tb, alpha, As[b], Bs[b], beta, np.nan_to_num(Cs[b]))
+        err = np.max(np.abs(X[:m] - Co)) / max(np.max(np.abs(Co)), 1e-300)
+        assert err <= 16 * max(k, 1) * eps, (b, err)
+        assert np.all(np.isnan(X[m:])), b
+
+
+@pytest.mark.parametrize("p", ["D", "S"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("upper", [False, True])
+@pytest.mark.parametrize("mk", [(8, 8), (16, 3), (17, 33), (32, 32), (45, 64), (96, 130)])
+def test_syrk(h, p, trans, upper, mk):
+    dt, eps = PREC[p]
+    m, k = mk
+    batch = 11
+    rng = np.random.default_rng(m * 5 + k + 100 * upper)
+    sa = (m, k) if trans == "N" else (k, m)
+    As = [rnd(rng, sa, dt) for _ in range(batch)]
+    Cs = [rnd(rng, (m, m), dt) for _ in range(batch)]
+    other = np.tril_indices(m, -1) if upper else np.triu_indices(m, 1)
+    for C in Cs:
+        C[other] = np.nan             # the other triangle is neither read nor written
+    lda, ldc = sa[0] + 2, m + 1
+    dA, sA = stage(As, lda, dt)
+    dC, sC = stage(Cs, ldc, dt)
+    alpha, beta = 0.5, -1.0
+    st = getattr(h.lib, "kblas%ssyrk_batch_strided" % p)(h.h, b"U" if upper else b"L", trans.encode(), m, k, alpha,
+                                                         kt.P(dA), lda, sA, beta, kt.P(dC), ldc, sC, batch)
+    assert st == pykblas.KBLAS_Success
+    R = kt.to_host(dC)
+    tri = np.triu_indices(m) if upper else np.tril_indices(m)
+    for b in range(batch):
+        X = R[b].T.astype(np.float64)[:m]
+        C0 = np.nan_to_num(Cs[b])
+        if upper:
+            Co = po.zsyrk(True, trans, alpha, As[b].astype(np.complex128), beta, C0.astype(np.complex128)).real
+        else:
+            Co = po.syrk(trans, alpha, As[b], beta, C0)
+        err = np.max(np.abs(X[tri] - Co[tri])) / max(np.max(np.abs(Co[tri])), 1e-300)
+        assert err <= 16 * k * eps, (b, err)
+        assert np.all(np.isnan(X[other])), b
+
+
+def test_gemm_alpha_zero_reads_no_operands(h):
+    # alpha = 0: C := beta C, A and B not read (NaN operands leave C finite)
+    m, n, k, batch = 20, 9, 7, 3
+    A = np.full((batch, k, m), np.nan)
+    B = np.full((batch, n, k), np.nan)
+    C = np.random.default_rng(1).standard_normal((batch, n, m))
+    dA, dB, dC = kt.to_dev(A), kt.to_dev(B), kt.to_dev(C)
+    st = h.lib.kblasDgemm_batch_strided(h.h, b"N", b"N", m, n, k, 0.0, kt.P(dA), m, m * k, kt.P(dB), k, k * n,
+                                        2.0, kt.P(dC), m, m * n, batch)
+    assert st == pykblas.KBLAS_Success
+    assert np.array_equal(kt.to_host(dC), 2.0 * C)
