@@ -90,7 +90,8 @@ k_trsm_small(int k, int nvec, int lg, int chunks, long items, int side_left, int
                     const int e = li + ((t0 + u) << lg);
                     const int i = e % K, j = e / K;
                     const bool rd = t0 + u < cnt && i >= j && i < k && !(unit && i == j);
-                    v[u] = rd ? gld(Ap + (upper ? j + (long)i * lda : i + (long)j * lda)) : T(0);
+                    const T x = gld(Ap + (rd ? (upper ? j + (long)i * lda : i + (long)j * lda) : 0));
+                    v[u] = rd ? x : T(0);
                 }
 #pragma unroll
                 for (int u = 0; u < TS_U; ++u) {
@@ -113,9 +114,9 @@ k_trsm_small(int k, int nvec, int lg, int chunks, long items, int side_left, int
         T x[K];
 #pragma unroll
         for (int i = 0; i < K; ++i) {
-            T v = T(0);
-            if (act && i < k) v = alpha * gld(xp + i * es);
-            x[i] = v;
+            const bool in = act && i < k;
+            const T v = gld(xp + (in ? i * es : 0));   // clamped address: every load in flight at once
+            x[i] = (in && alpha != T(0)) ? alpha * v : T(0);
         }
         if constexpr (FWD) {
 #pragma unroll
@@ -144,6 +145,196 @@ k_trsm_small(int k, int nvec, int lg, int chunks, long items, int side_left, int
         }
         __syncthreads();
     }
+}
+
+// ---------------------------------------------------------------------------
+// Orders 17..64 on the matrix cores.  The working triangle W (forward: W = Lv,
+// lower; backward: W = Lv^T, upper) is staged once per workgroup as NBK x NBK
+// blocks of 16 x 16 (the blocks of W's triangle only, each column-major), the
+// order padded to 16 NBK with the identity; then each diagonal block is
+// replaced by its inverse (one column per thread, by substitution).  Every
+// wavefront takes 32 vectors (two column blocks of 16) and solves block row
+// by block row in W's order:
+//     R_i = alpha B_i - sum_j W_ij X_j   (j before i),   X_i = W_ii^-1 R_i,
+// each product a chain of four 16x16x4 products with the W fragment from LDS
+// as the first operand and the right-hand block as the second.  The
+// accumulator layout of a 16 x 16 product (register e of lane (c, g) holds row
+// kmap(e, g), column c) is exactly the second-operand layout of the next
+// product's K step e, so R_i and X_j feed the products from registers.
+// ---------------------------------------------------------------------------
+// row of a 16 x 16 accumulator held in register e of lane group g (FP64 / FP32 layouts)
+template <typename T>
+KB_DEV int tm_kmap(int e, int g) { return sizeof(T) == 8 ? 4 * e + g : 4 * g + e; }
+
+// LDS slot of W's block (i, j) (forward: j <= i; backward: i <= j)
+KB_DEV int tm_slot(int i, int j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; }
+
+constexpr int TM_NCB = 2;   // column blocks of 16 vectors per wavefront
+
+template <typename T, int NBK, bool FWD>
+__global__ void __launch_bounds__(256)
+k_trsm_mfma(int k, int nvec, int chunks, long items, int side_left, int upper, int unit, T alpha,
+            BatchArg<T> A, BatchArg<T> B)
+{
+    typedef typename TileV<T>::v4 v4;
+    constexpr int NSLOT = NBK * (NBK + 1) / 2;
+    __shared__ T sW[NSLOT * 256];
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lg = lane >> 4;
+    const int nb = (k + 15) >> 4;   // row blocks holding the order
+    for (long it = blockIdx.x; it < items; it += gridDim.x) {
+        const long mat = it / chunks;
+        const int chunk = (int)(it - mat * chunks);
+        const T* Ap = A.get(mat);
+        const int lda = A.ldim(mat);
+        // this wavefront's vectors: alpha B, in the accumulator layout, loaded
+        // first so the loads overlap the staging and the inversion below
+        const int v0 = (chunk * (nthr >> 6) + wave) * (16 * TM_NCB);
+        T* Bp = B.get(mat);
+        const long ldb = B.ldim(mat);
+        auto bptr = [&](int r, int vec) {
+            return Bp + (side_left ? r + (long)vec * ldb : vec + (long)r * ldb);
+        };
+        v4 X[NBK][TM_NCB];
+#pragma unroll
+        for (int i = 0; i < NBK; ++i) {
+#pragma unroll
+            for (int cb = 0; cb < TM_NCB; ++cb) {
+                const int vec = v0 + 16 * cb + li;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int r = 16 * i + tm_kmap<T>(e, lg);
+                    // unconditional load from a clamped address (B's first element otherwise), so
+                    // all the loads are in flight together
+                    const bool in = i < nb && r < k && vec < nvec;
+                    const T v = gld(in ? bptr(r, vec) : Bp);
+                    X[i][cb][e] = (in && alpha != T(0)) ? alpha * v : T(0);
+                }
+            }
+        }
+        // stage W's blocks; the loads of TS_U elements per thread in flight together
+        for (int e0 = 0; e0 < NSLOT * 256; e0 += nthr * TS_U) {
+            T v[TS_U];
+#pragma unroll
+            for (int u = 0; u < TS_U; ++u) {
+                const int e = e0 + u * nthr + tid;
+                const int slot = e >> 8, rr = e & 15, cc = (e >> 4) & 15;
+                int bi = 0;
+                while ((bi + 1) * (bi + 2) / 2 <= slot) ++bi;
+                const int bj = slot - bi * (bi + 1) / 2;       // bj <= bi
+                // forward: block (bi, bj) of the lower W; backward: block (bj, bi) of the upper W
+                const int r = 16 * (FWD ? bi : bj) + rr, c = 16 * (FWD ? bj : bi) + cc;
+                // W(r, c) = Lv(r, c) (forward) or Lv(c, r) (backward); Lv(p, q) (p >= q) = A(p, q) or A(q, p)
+                const int p = FWD ? r : c, q = FWD ? c : r;
+                const bool tri = FWD ? r >= c : r <= c;
+                const bool rd = e < NSLOT * 256 && tri && p < k && !(unit && p == q);
+                const T x = gld(Ap + (rd ? (upper ? q + (long)p * lda : p + (long)q * lda) : 0));
+                v[u] = (r == c && (p >= k || unit)) ? T(1) : rd ? x : T(0);
+            }
+#pragma unroll
+            for (int u = 0; u < TS_U; ++u) {
+                const int e = e0 + u * nthr + tid;
+                if (e < NSLOT * 256) sW[e] = v[u];
+            }
+        }
+        __syncthreads();
+        // invert the diagonal blocks: thread t < 16 NBK takes column t % 16 of block t / 16
+        {
+            T x[16];
+            const int b = tid >> 4, c = tid & 15;
+            const bool on = tid < 16 * NBK && b < nb;
+            const T* D = sW + (on ? tm_slot(b, b) : 0) * 256;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) x[r] = T(0);
+            if (on) {
+                if constexpr (FWD) {
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        T sacc = (r == c) ? T(1) : T(0);
+#pragma unroll
+                        for (int q = 0; q < r; ++q) sacc = ts_fma(-D[r + 16 * q], x[q], sacc);
+                        x[r] = (r >= c) ? sacc / D[r + 16 * r] : T(0);
+                    }
+                } else {
+#pragma unroll
+                    for (int r = 15; r >= 0; --r) {
+                        T sacc = (r == c) ? T(1) : T(0);
+#pragma unroll
+                        for (int q = r + 1; q < 16; ++q) sacc = ts_fma(-D[r + 16 * q], x[q], sacc);
+                        x[r] = (r <= c) ? sacc / D[r + 16 * r] : T(0);
+                    }
+                }
+            }
+            __syncthreads();
+            if (on) {
+                T* Dw = sW + tm_slot(b, b) * 256;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) Dw[r + 16 * c] = x[r];
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int ii = 0; ii < NBK; ++ii) {
+            const int i = FWD ? ii : NBK - 1 - ii;
+            if (i < nb) {
+                v4 acc[TM_NCB];
+#pragma unroll
+                for (int cb = 0; cb < TM_NCB; ++cb) acc[cb] = X[i][cb];
+#pragma unroll
+                for (int jj = 0; jj < NBK; ++jj) {
+                    const int j = FWD ? jj : NBK - 1 - jj;
+                    if ((FWD ? j < i : j > i) && j < nb) {
+                        const T* Wb = sW + tm_slot(i, j) * 256;
+#pragma unroll
+                        for (int s2 = 0; s2 < 4; ++s2) {
+                            const T w = -Wb[li + 16 * tm_kmap<T>(s2, lg)];
+#pragma unroll
+                            for (int cb = 0; cb < TM_NCB; ++cb) acc[cb] = mfma16x16x4(w, X[j][cb][s2], acc[cb]);
+                        }
+                    }
+                }
+                const T* Di = sW + tm_slot(i, i) * 256;
+#pragma unroll
+                for (int cb = 0; cb < TM_NCB; ++cb) X[i][cb] = v4{T(0), T(0), T(0), T(0)};
+#pragma unroll
+                for (int s2 = 0; s2 < 4; ++s2) {
+                    const T w = Di[li + 16 * tm_kmap<T>(s2, lg)];
+#pragma unroll
+                    for (int cb = 0; cb < TM_NCB; ++cb) X[i][cb] = mfma16x16x4(w, acc[cb][s2], X[i][cb]);
+                }
+#pragma unroll
+                for (int cb = 0; cb < TM_NCB; ++cb) {
+                    const int vec = v0 + 16 * cb + li;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int r = 16 * i + tm_kmap<T>(e, lg);
+                        if (r < k && vec < nvec) gst(bptr(r, vec), X[i][cb][e]);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T, int NBK>
+int tm_launch(bool fwd, int k, int nvec, int side_left, int upper, int unit, T alpha, kb_mat_t A, kb_mat_t B,
+              int batch, hipStream_t s)
+{
+    const int per_wave = 16 * TM_NCB;
+    const int waves = nvec >= 4 * per_wave ? 4 : (nvec + per_wave - 1) / per_wave;
+    const int chunks = (nvec + waves * per_wave - 1) / (waves * per_wave);
+    const long items = (long)batch * chunks;
+    const unsigned grid = (unsigned)(items < (1L << 22) ? items : (1L << 22));
+    BatchArg<T> a = to_arg<T>(A), b = to_arg<T>(B);
+    if (fwd)
+        hipLaunchKernelGGL((k_trsm_mfma<T, NBK, true>), dim3(grid), dim3(64 * waves), 0, s, k, nvec, chunks, items,
+                           side_left, upper, unit, alpha, a, b);
+    else
+        hipLaunchKernelGGL((k_trsm_mfma<T, NBK, false>), dim3(grid), dim3(64 * waves), 0, s, k, nvec, chunks,
+                           items, side_left, upper, unit, alpha, a, b);
+    return (int)hipGetLastError();
 }
 
 template <typename T, int K>
@@ -176,8 +367,8 @@ int ts_dispatch(bool fwd, int k, int nvec, int side_left, int upper, int unit, T
 {
     if (k <= 8) return ts_launch<T, 8>(fwd, k, nvec, side_left, upper, unit, alpha, A, B, batch, s);
     if (k <= 16) return ts_launch<T, 16>(fwd, k, nvec, side_left, upper, unit, alpha, A, B, batch, s);
-    if (k <= 32) return ts_launch<T, 32>(fwd, k, nvec, side_left, upper, unit, alpha, A, B, batch, s);
-    return ts_launch<T, 64>(fwd, k, nvec, side_left, upper, unit, alpha, A, B, batch, s);
+    if (k <= 32) return tm_launch<T, 2>(fwd, k, nvec, side_left, upper, unit, alpha, A, B, batch, s);
+    return tm_launch<T, 4>(fwd, k, nvec, side_left, upper, unit, alpha, A, B, batch, s);
 }
 
 }  // namespace
