@@ -948,7 +948,8 @@ def main():
         "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+f64" if cfg["op"] in ("sposv", "vsposv") else "f64",
         "data": "synthetic (seeded, generated on the GPU)",
-        "config": {"workload": cfg["workload"], "batch": cfg["batch"], "n": head["n"], "nrhs": head.get("nrhs", 0)},
+        "config": {"workload": cfg["workload"], "batch": cfg["batch"], "n": head.get("n", cfg.get("n")),
+                   "nrhs": head.get("nrhs", cfg.get("nrhs", 0))},
     }
     for k, v in head.items():
         if k not in ("workload", "batch", "n", "nrhs", "value", "unit", "ms_per_step"):

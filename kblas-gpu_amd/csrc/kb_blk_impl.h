@@ -288,7 +288,10 @@ KB_DEV void blk_diag_subst(T* B, int ldb, int side_left, int k0, int kb, int c0,
         for (int h = 0; h < 2; ++h) {
             const int r = 16 * h + li;
             const bool in = c < rc && r < kb;
-            y[s][h] = in ? sc * gld(rhs_elem(B, ldb, side_left, in ? k0 + r : 0, in ? c0 + c : 0)) : T(0);
+            // unconditional load from a clamped address, then a select: every load of the
+            // block is in flight at once (a load under a branch waits before the join)
+            const T v = gld(rhs_elem(B, ldb, side_left, in ? k0 + r : 0, in ? c0 + c : 0));
+            y[s][h] = in ? sc * v : T(0);
         }
     }
 #pragma unroll

@@ -63,6 +63,8 @@ KB_DEV double sb_block_max(double v, SbSmem& S)
 // its upper part as column i below the diagonal (one contiguous run per
 // thread), so every thread does about n reads per row.  big is set when an
 // entry exceeds FP32 range.
+constexpr int SB_U = 4;   // elements per thread whose loads go out together in the element-wise loops
+
 KB_DEV double sb_convert_norm(const double* A, int lda, int n, float* A32, bool& big, SbSmem& S)
 {
     double rmax = 0.0;
@@ -85,23 +87,53 @@ KB_DEV double sb_convert_norm(const double* A, int lda, int n, float* A32, bool&
 // Z := FP32 copy of the right-hand sides B (n x nrhs, ld n); big as above.
 KB_DEV void sb_convert_rhs(const double* B, int ldb, int side_left, int n, int nrhs, float* Z, bool& big)
 {
-    for (int e = threadIdx.x; e < n * nrhs; e += KB_BLK_WG) {
-        const int i = e % n, c = e / n;
-        const double v = gld(rhs_elem(const_cast<double*>(B), ldb, side_left, i, c));
-        big = big || __builtin_fabs(v) > KB_SB_FLT_MAX;
-        gst(Z + i + (long)c * n, (float)v);
+    const int tot = n * nrhs;
+    for (int e0 = threadIdx.x; e0 < tot; e0 += SB_U * KB_BLK_WG) {
+        double v[SB_U];
+#pragma unroll
+        for (int u = 0; u < SB_U; ++u) {
+            const int e = e0 + u * KB_BLK_WG;
+            const bool in = e < tot;
+            v[u] = gld(rhs_elem(const_cast<double*>(B), ldb, side_left, in ? e % n : 0, in ? e / n : 0));
+        }
+#pragma unroll
+        for (int u = 0; u < SB_U; ++u) {
+            const int e = e0 + u * KB_BLK_WG;
+            if (e < tot) {
+                big = big || __builtin_fabs(v[u]) > KB_SB_FLT_MAX;
+                gst(Z + e % n + (long)(e / n) * n, (float)v[u]);
+            }
+        }
     }
 }
 
 // X := (first ? 0 : X) + Z, and S.xn[c] := max_i |x(i, c)|.
 KB_DEV void sb_update_x(double* X, int ldx, int side_left, int n, int nrhs, const float* Z, bool first, SbSmem& S)
 {
-    for (int e = threadIdx.x; e < n * nrhs; e += KB_BLK_WG) {
-        const int i = e % n, c = e / n;
-        double* px = rhs_elem(X, ldx, side_left, i, c);
-        const double x = (first ? 0.0 : gld(px)) + (double)gld(Z + i + (long)c * n);
-        gst(px, x);
-        atomicMax(&S.xn[c], sb_bits(x));
+    // SB_U elements per thread and step, their loads issued together (from
+    // clamped addresses) before any store
+    const int tot = n * nrhs;
+    for (int e0 = threadIdx.x; e0 < tot; e0 += SB_U * KB_BLK_WG) {
+        double xo[SB_U];
+        float z[SB_U];
+#pragma unroll
+        for (int u = 0; u < SB_U; ++u) {
+            const int e = e0 + u * KB_BLK_WG;
+            const bool in = e < tot;
+            const int i = in ? e % n : 0, c = in ? e / n : 0;
+            xo[u] = first ? 0.0 : gld(rhs_elem(X, ldx, side_left, i, c));
+            z[u] = gld(Z + i + (long)c * n);
+        }
+#pragma unroll
+        for (int u = 0; u < SB_U; ++u) {
+            const int e = e0 + u * KB_BLK_WG;
+            if (e < tot) {
+                const int i = e % n, c = e / n;
+                const double x = xo[u] + (double)z[u];
+                gst(rhs_elem(X, ldx, side_left, i, c), x);
+                atomicMax(&S.xn[c], sb_bits(x));
+            }
+        }
     }
 }
 
@@ -130,11 +162,12 @@ KB_DEV void sb_residual(const double* A, int lda, int n, const double* B, int ld
         for (int cb = 0; cb < 4; ++cb) acc[cb] = kb_d4{0.0, 0.0, 0.0, 0.0};
         for (int k0 = 0; k0 < n; k0 += NBK) {
             const int kb = min(NBK, n - k0);
+#pragma unroll
             for (int e = tid; e < NBK * KB_SB_MAX_RHS; e += KB_BLK_WG) {
                 const int k = e % NBK, c = e / NBK;
-                double v = 0.0;
-                if (k < kb && c < nrhs) v = gld(rhs_elem(const_cast<double*>(X), ldx, side_left, k0 + k, c));
-                sY[c * (NBK + 1) + k] = v;
+                const bool in = k < kb && c < nrhs;
+                const double v = gld(rhs_elem(const_cast<double*>(X), ldx, side_left, in ? k0 + k : 0, in ? c : 0));
+                sY[c * (NBK + 1) + k] = in ? v : 0.0;
             }
             __syncthreads();
             if (r0 < n) {
@@ -158,6 +191,17 @@ KB_DEV void sb_residual(const double* A, int lda, int n, const double* B, int ld
             __syncthreads();
         }
         if (r0 < n) {
+            double bv[4][4];   // B's block, loaded together from clamped addresses
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb) {
+                const int col = 16 * cb + li;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int row = r0 + 4 * q + lg;
+                    const bool in = cb < ncb && row < n && col < nrhs;
+                    bv[cb][q] = gld(rhs_elem(const_cast<double*>(B), ldb, side_left, in ? row : 0, in ? col : 0));
+                }
+            }
 #pragma unroll
             for (int cb = 0; cb < 4; ++cb) {
                 const int col = 16 * cb + li;
@@ -165,7 +209,7 @@ KB_DEV void sb_residual(const double* A, int lda, int n, const double* B, int ld
                 for (int q = 0; q < 4; ++q) {
                     const int row = r0 + 4 * q + lg;
                     if (cb < ncb && row < n && col < nrhs) {
-                        const double r = gld(rhs_elem(const_cast<double*>(B), ldb, side_left, row, col)) - acc[cb][q];
+                        const double r = bv[cb][q] - acc[cb][q];
                         ovf = ovf || __builtin_fabs(r) > KB_SB_FLT_MAX;
                         atomicMax(&S.rn[col], sb_bits(r));
                         gst(Z + row + (long)col * n, (float)r);
