@@ -83,6 +83,20 @@ static int chol_dispatch(kblasHandle_t h, char prec, int mode, int n, int nrhs, 
     const int v = kb_tune_variant(prec, mode, n);
     if (v == KB_VAR_BLK || n > KB_REG_MAX_N)
         return kb_launch_blk_chol(prec, mode, n, nrhs, side_left, A, B, info, bc, h->stream);
+    if (mode != KB_MODE_POTRF && n > 32 && nrhs >= KB_SOLVE_SPLIT_NRHS) {
+        /* many right-hand sides: factor, then the matrix-core sweeps (a failed
+         * factorization leaves that matrix's B as it was) */
+        if (mode == KB_MODE_POSV) {
+            const int pv = kb_tune_variant(prec, KB_MODE_POTRF, n);
+            const int e = pv == KB_VAR_BLK
+                              ? kb_launch_blk_chol(prec, KB_MODE_POTRF, n, 0, side_left, A, B, info, bc, h->stream)
+                              : kb_launch_reg_chol_v(prec, KB_MODE_POTRF, n, 0, side_left, A, B, info, bc, pv,
+                                                     h->stream);
+            if (e) return e;
+        }
+        return kb_launch_trsm_solve(prec, side_left, n, nrhs, A, B, mode == KB_MODE_POSV ? info : NULL, bc,
+                                    h->stream);
+    }
     return kb_launch_reg_chol_v(prec, mode, n, nrhs, side_left, A, B, info, bc, v, h->stream);
 }
 

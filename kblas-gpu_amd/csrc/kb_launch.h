@@ -149,6 +149,13 @@ int kb_launch_trsm_x(char prec, int side_left, int upper, int trans, int unit, i
  * blocked solve; hipErrorNotSupported for an upper triangle above 64. */
 int kb_launch_trsm_fast(char prec, int side_left, int upper, int trans, int unit, int m, int n, double alpha,
                         kb_mat_t A, kb_mat_t B, int batch, hipStream_t s);
+/* (L L^T) X = B for 16 < n <= 64 ('d' / 's'): the two sweeps of the matrix-core
+ * TRSM; matrices with skip[b] != 0 are left alone (skip may be NULL). */
+int kb_launch_trsm_solve(char prec, int side_left, int n, int nrhs, kb_mat_t A, kb_mat_t B, const int* skip,
+                         int batch, hipStream_t s);
+/* Orders above 32 with at least this many right-hand sides: POSV / POTRS as a
+ * factorization followed by kb_launch_trsm_solve. */
+#define KB_SOLVE_SPLIT_NRHS 32
 /* Real uniform GEMM / SYRK on the matrix cores (prec 'd' / 's', op 0 = N, else
  * T; syrk: C is m x m, B = A, tb = !ta, upper picks the triangle);
  * hipErrorNotSupported when the batch is too large for one launch. */

@@ -162,8 +162,10 @@ KB_DEV void sb_residual(const double* A, int lda, int n, const double* B, int ld
         for (int cb = 0; cb < 4; ++cb) acc[cb] = kb_d4{0.0, 0.0, 0.0, 0.0};
         for (int k0 = 0; k0 < n; k0 += NBK) {
             const int kb = min(NBK, n - k0);
+            static_assert(NBK * KB_SB_MAX_RHS % KB_BLK_WG == 0, "whole steps");
 #pragma unroll
-            for (int e = tid; e < NBK * KB_SB_MAX_RHS; e += KB_BLK_WG) {
+            for (int u = 0; u < NBK * KB_SB_MAX_RHS / KB_BLK_WG; ++u) {
+                const int e = tid + u * KB_BLK_WG;
                 const int k = e % NBK, c = e / NBK;
                 const bool in = k < kb && c < nrhs;
                 const double v = gld(rhs_elem(const_cast<double*>(X), ldx, side_left, in ? k0 + k : 0, in ? c : 0));

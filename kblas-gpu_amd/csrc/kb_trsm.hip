@@ -174,7 +174,7 @@ constexpr int TM_NCB = 2;   // column blocks of 16 vectors per wavefront
 template <typename T, int NBK, bool FWD>
 __global__ void __launch_bounds__(256)
 k_trsm_mfma(int k, int nvec, int chunks, long items, int side_left, int upper, int unit, T alpha,
-            BatchArg<T> A, BatchArg<T> B)
+            BatchArg<T> A, BatchArg<T> B, const int* skip)
 {
     typedef typename TileV<T>::v4 v4;
     constexpr int NSLOT = NBK * (NBK + 1) / 2;
@@ -185,6 +185,7 @@ k_trsm_mfma(int k, int nvec, int chunks, long items, int side_left, int upper, i
     const int nb = (k + 15) >> 4;   // row blocks holding the order
     for (long it = blockIdx.x; it < items; it += gridDim.x) {
         const long mat = it / chunks;
+        if (skip && skip[mat]) continue;   // (the whole workgroup: one matrix per item)
         const int chunk = (int)(it - mat * chunks);
         const T* Ap = A.get(mat);
         const int lda = A.ldim(mat);
@@ -320,7 +321,7 @@ k_trsm_mfma(int k, int nvec, int chunks, long items, int side_left, int upper, i
 
 template <typename T, int NBK>
 int tm_launch(bool fwd, int k, int nvec, int side_left, int upper, int unit, T alpha, kb_mat_t A, kb_mat_t B,
-              int batch, hipStream_t s)
+              int batch, hipStream_t s, const int* skip = nullptr)
 {
     const int per_wave = 16 * TM_NCB;
     const int waves = nvec >= 4 * per_wave ? 4 : (nvec + per_wave - 1) / per_wave;
@@ -330,10 +331,10 @@ int tm_launch(bool fwd, int k, int nvec, int side_left, int upper, int unit, T a
     BatchArg<T> a = to_arg<T>(A), b = to_arg<T>(B);
     if (fwd)
         hipLaunchKernelGGL((k_trsm_mfma<T, NBK, true>), dim3(grid), dim3(64 * waves), 0, s, k, nvec, chunks, items,
-                           side_left, upper, unit, alpha, a, b);
+                           side_left, upper, unit, alpha, a, b, skip);
     else
         hipLaunchKernelGGL((k_trsm_mfma<T, NBK, false>), dim3(grid), dim3(64 * waves), 0, s, k, nvec, chunks,
-                           items, side_left, upper, unit, alpha, a, b);
+                           items, side_left, upper, unit, alpha, a, b, skip);
     return (int)hipGetLastError();
 }
 
@@ -392,5 +393,32 @@ extern "C" int kb_launch_trsm_fast(char prec, int side_left, int upper, int tran
     }
     if (prec == 'd') return ts_dispatch<double>(fwd, k, nvec, side_left, upper, unit, alpha, A, B, batch, s);
     if (prec == 's') return ts_dispatch<float>(fwd, k, nvec, side_left, upper, unit, (float)alpha, A, B, batch, s);
+    return (int)hipErrorInvalidValue;
+}
+
+namespace {
+template <typename T>
+int tm_solve(int side_left, int n, int nrhs, kb_mat_t A, kb_mat_t B, const int* skip, int batch, hipStream_t s)
+{
+    int e;
+    if (n <= 32) {
+        e = tm_launch<T, 2>(true, n, nrhs, side_left, 0, 0, T(1), A, B, batch, s, skip);
+        return e ? e : tm_launch<T, 2>(false, n, nrhs, side_left, 0, 0, T(1), A, B, batch, s, skip);
+    }
+    e = tm_launch<T, 4>(true, n, nrhs, side_left, 0, 0, T(1), A, B, batch, s, skip);
+    return e ? e : tm_launch<T, 4>(false, n, nrhs, side_left, 0, 0, T(1), A, B, batch, s, skip);
+}
+}  // namespace
+
+// (L L^T) X = B with the lower factor L of order 16 < n <= 64 ('d' or 's'):
+// the forward and the backward sweep of the matrix-core TRSM; matrices with
+// skip[b] != 0 (a failed factorization) are left as they are.
+extern "C" int kb_launch_trsm_solve(char prec, int side_left, int n, int nrhs, kb_mat_t A, kb_mat_t B,
+                                    const int* skip, int batch, hipStream_t s)
+{
+    if (batch < 1 || nrhs < 1) return 0;
+    if (n <= 16 || n > KB_REG_MAX_N) return (int)hipErrorInvalidValue;
+    if (prec == 'd') return tm_solve<double>(side_left, n, nrhs, A, B, skip, batch, s);
+    if (prec == 's') return tm_solve<float>(side_left, n, nrhs, A, B, skip, batch, s);
     return (int)hipErrorInvalidValue;
 }
