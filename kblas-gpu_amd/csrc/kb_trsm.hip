@@ -76,12 +76,36 @@ k_trsm_small(int k, int nvec, int lg, int chunks, long items, int side_left, int
         const long mat = live ? it / chunks : 0;
         const int vec = (int)(live ? it - mat * chunks : 0) * G + li;
         const bool act = live && vec < nvec;
+        // this lane's vector, loaded first so the loads overlap the staging below
+        // both matrices' addresses first (in pointer-array mode they are loads, and a
+        // wait for them would also wait for every load issued before)
+        T* Bp = B.get(mat);
+        const long ldb = B.ldim(mat);
+        const T* Ap = A.get(mat);
+        const int lda = A.ldim(mat);
+        T* xp = Bp + (act ? (side_left ? (long)vec * ldb : (long)vec) : 0);
+        const long es = side_left ? 1 : ldb;
+        // Unconditional loads from clamped addresses, so they are all in flight at once
+        // (a select on each value lets the compiler sink every load into its own
+        // branch).  Rows k..K-1 and idle lanes hold values that no stored row
+        // depends on: the padded triangle is zero below row k and the sweeps never
+        // read those rows into rows < k.
+        // (alpha is applied after the staging, so these loads need not be waited for first)
+        T x[K];
+        if (alpha != T(0)) {
+#pragma unroll
+            for (int i = 0; i < K; ++i) {
+                // the address is opaque to the optimizer: otherwise it sees that a clamped
+                // load repeats the first one and moves each load under a branch again
+                const T* q = xp + (act && i < k ? i * es : 0);
+                asm volatile("" : "+v"(q));
+                x[i] = gld(q);
+            }
+        }
         if (live) {
             // the lower view, packed by columns, rows k..K-1 zero: the group's
             // lanes walk the K x K square (element e = li + G t, row e % K,
             // column e / K), TS_U loads in flight per lane
-            const T* Ap = A.get(mat);
-            const int lda = A.ldim(mat);
             const int cnt = (K * K) >> lg;
             for (int t0 = 0; t0 < cnt; t0 += TS_U) {
                 T v[TS_U];
@@ -107,17 +131,8 @@ k_trsm_small(int k, int nvec, int lg, int chunks, long items, int side_left, int
             for (int d = li; d < k; d += G) Ls[ts_col(K, d)] = unit ? T(1) : T(1) / Ls[ts_col(K, d)];
         }
         __syncthreads();
-        T* Bp = B.get(mat);
-        const long ldb = B.ldim(mat);
-        T* xp = Bp + (act ? (side_left ? (long)vec * ldb : (long)vec) : 0);
-        const long es = side_left ? 1 : ldb;
-        T x[K];
 #pragma unroll
-        for (int i = 0; i < K; ++i) {
-            const bool in = act && i < k;
-            const T v = gld(xp + (in ? i * es : 0));   // clamped address: every load in flight at once
-            x[i] = (in && alpha != T(0)) ? alpha * v : T(0);
-        }
+        for (int i = 0; i < K; ++i) x[i] = alpha != T(0) ? alpha * x[i] : T(0);
         if constexpr (FWD) {
 #pragma unroll
             for (int j = 0; j < K; ++j) {
