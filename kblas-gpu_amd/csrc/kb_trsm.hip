@@ -9,16 +9,18 @@
 // (Lv = T, or T^T for an upper T): forward (Lv x = alpha b) or backward
 // (Lv^T x = alpha b).
 //
-// Orders up to 64 (k_trsm_small): a wavefront works on P = 64 / G groups of
+// Orders up to 16 (k_trsm_small): a wavefront works on P = 64 / G groups of
 // G lanes; each group takes one matrix and G of its vectors, one vector per
-// lane held in registers (K values, K = the order rounded up to a power of
-// two), and stages the matrix's triangle in LDS, packed by columns with the
+// lane held in registers (K values, K = the order rounded up to 8 or 16),
+// and stages the matrix's triangle in LDS, packed by columns with the
 // reciprocal of each diagonal element in its place.  The substitution reads
 // the triangle as LDS broadcasts within a group (the P groups' triangles
 // start on different banks), so the arithmetic is K^2 / 2 FMAs per lane with
 // no cross-lane traffic.  Small orders put many matrices in one wavefront
 // (order 8: eight matrices of eight vectors), so the launch is not dominated
 // by idle lanes.
+//
+// Orders 17..64 (k_trsm_mfma, below): 16 x 16 blocks on the matrix cores.
 //
 // Orders above 64 (lower T): one sweep of the blocked Cholesky solve
 // (kb_blk_impl.h): diagonal 32 x 32 blocks by DPP-row substitution, the
