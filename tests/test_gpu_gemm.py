@@ -40,8 +40,10 @@ def rnd(rng, shape, dt):
     return rng.standard_normal(shape).astype(dt).astype(np.float64)
 
 
-GEMM_SHAPES = [(8, 8, 8), (16, 16, 1), (5, 17, 3), (17, 5, 31), (32, 32, 32), (33, 40, 33), (64, 64, 65),
-               (70, 31, 130), (100, 96, 16)]
+# the last four take the 64 x 64 workgroup tiles (m, n >= 64): K chunks of 16 with a partial last chunk,
+# edge tiles in both directions
+GEMM_SHAPES = [(8, 8, 8), (16, 16, 1), (5, 17, 3), (17, 5, 31), (32, 32, 32), (33, 40, 33), (70, 31, 130),
+               (64, 64, 65), (100, 96, 16), (130, 65, 7), (129, 200, 33)]
 
 
 @pytest.mark.parametrize("p", ["D", "S"])
