@@ -152,13 +152,30 @@ k_inv_reg(int op, int unit, int n, BatchArg<T> A, int* info, const int* skip, in
     const long b = (long)blockIdx.x * MPW + g;
     const bool live = b < batch && !(skip && skip[b] != 0);
     T* S = tile[g];
-    T* G = live ? A.get(b) : nullptr;
-    const int lda = live ? A.ldim(b) : 0;
+    const long bv = b < batch ? b : batch - 1;   // a valid matrix for the clamped loads below
+    T* G = A.get(bv);
+    const int lda = A.ldim(bv);
+    // the triangle in chunks of 16 columns: each chunk's loads issued together
+    // (unconditional, from clamped addresses kept opaque to the optimizer, which
+    // would otherwise put every load under its own branch and wait for it)
 #pragma unroll
-    for (int j = 0; j < NL; ++j) {
-        T v = T(0);
-        if (live && r < n && j < n && r >= j) v = G[r + (long)j * lda];
-        S[r + j * LD] = v;
+    for (int j0 = 0; j0 < NL; j0 += 16) {
+        T v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int j = j0 + u;
+            const bool in = live && r < n && j < n && r >= j;
+            const T* q = G + (in ? r + (long)j * lda : 0);
+            asm volatile("" : "+v"(q));
+            if constexpr (__is_same(T, double) || __is_same(T, float)) v[u] = gld(q);
+            else v[u] = *q;   // the complex types: a plain load
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int j = j0 + u;
+            const bool in = live && r < n && j < n && r >= j;
+            S[r + j * LD] = in ? v[u] : T(0);
+        }
     }
     __syncthreads();
     // first exactly-zero diagonal element of this lane group's matrix (non-unit TRTRI / POTRI)

@@ -48,6 +48,32 @@ KB_DEV void ps_maxdiag(const T* S, int LD, int n, int k, int t, T& val, int& idx
     idx = __shfl(key, 0, width);
 }
 
+// Row t of the n x n block G (ld lda) into S (ld LD), in chunks of 8 columns
+// whose loads go out together: unconditional, from clamped addresses kept
+// opaque to the optimizer (which would otherwise put each load under its own
+// branch and wait for it).  on = false: nothing is stored (G must still be a
+// valid matrix).
+template <typename T>
+KB_DEV void ps_stage(T* S, int LD, const T* G, int lda, int n, int t, bool on)
+{
+    for (int j0 = 0; j0 < n; j0 += 8) {
+        T v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int j = j0 + u;
+            const bool in = on && t < n && j < n;
+            const T* q = G + (in ? t + (long)j * lda : 0);
+            asm volatile("" : "+v"(q));
+            v[u] = gld(q);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int j = j0 + u;
+            if (on && t < n && j < n) S[t + j * LD] = v[u];
+        }
+    }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(64)
 k_pstrf(int n, BatchArg<T> A, int* const* pivp, int* pivs, long spiv, int* ranks, int batch)
@@ -61,8 +87,7 @@ k_pstrf(int n, BatchArg<T> A, int* const* pivp, int* pivs, long spiv, int* ranks
     T* G = A.get(b);
     const int lda = A.ldim(b);
     int* piv = pivp ? pivp[b] : pivs + b * spiv;
-    for (int j = 0; j < n; ++j)
-        if (t < n) S[t + j * LD] = G[t + (long)j * lda];
+    ps_stage<T>(S, LD, G, lda, n, t, true);
     __syncthreads();
     for (int j = 0; j < n; ++j)
         if (t < n && t > j) S[t + j * LD] = S[j + t * LD];   // lower := upper^T
@@ -125,10 +150,10 @@ k_pstrf2(int n, BatchArg<T> A, int* const* pivp, int* pivs, long spiv, int* rank
     const long b = (long)blockIdx.x * 2 + g;
     const bool live = b < batch;
     T* S = reinterpret_cast<T*>(kb_ps_lds) + g * n * LD;
-    T* G = live ? A.get(b) : nullptr;
-    const int lda = live ? A.ldim(b) : 0;
-    for (int j = 0; j < n; ++j)
-        if (live && t < n) S[t + j * LD] = G[t + (long)j * lda];
+    const long bv = live ? b : batch - 1;   // a valid matrix for the clamped loads
+    T* G = A.get(bv);
+    const int lda = A.ldim(bv);
+    ps_stage<T>(S, LD, G, lda, n, t, live);
     __syncthreads();
     for (int j = 0; j < n; ++j)
         if (live && t < n && t > j) S[t + j * LD] = S[j + t * LD];
