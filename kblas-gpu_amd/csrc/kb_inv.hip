@@ -296,10 +296,30 @@ k_trmm(int side_left, int trans, int unit, int m, int n, T alpha, BatchArg<T> A,
     T* SP = SA + (stageA ? (long)ord * (ord + 1) : 0);
     const T* L = GA;
     int ldl = lda;
+    // staging copies: TM_U elements per thread whose loads go out together
+    // (unconditional, from opaque clamped addresses; a load under a branch of its
+    // own would be waited for alone)
+    constexpr int TM_U = 4;
     if (stageA) {
-        for (unsigned e = t; e < (unsigned)(ord * ord); e += nt) {   // staged sizes fit 32 bits
-            const int i = (int)(e % (unsigned)ord), j = (int)(e / (unsigned)ord);
-            if (i >= j) SA[i + (long)j * (ord + 1)] = GA[i + (long)j * lda];
+        const unsigned tot = (unsigned)(ord * ord);   // staged sizes fit 32 bits
+        for (unsigned e0 = t; e0 < tot; e0 += TM_U * nt) {
+            T v[TM_U];
+#pragma unroll
+            for (int u = 0; u < TM_U; ++u) {
+                const unsigned e = e0 + u * nt;
+                const int i = (int)(e % (unsigned)ord), j = (int)(e / (unsigned)ord);
+                const bool in = e < tot && i >= j;
+                const T* q = GA + (in ? i + (long)j * lda : 0);
+                asm volatile("" : "+v"(q));
+                if constexpr (__is_same(T, double) || __is_same(T, float)) v[u] = gld(q);
+                else v[u] = *q;
+            }
+#pragma unroll
+            for (int u = 0; u < TM_U; ++u) {
+                const unsigned e = e0 + u * nt;
+                const int i = (int)(e % (unsigned)ord), j = (int)(e / (unsigned)ord);
+                if (e < tot && i >= j) SA[i + (long)j * (ord + 1)] = v[u];
+            }
         }
         L = SA;
         ldl = ord + 1;
@@ -310,9 +330,22 @@ k_trmm(int side_left, int trans, int unit, int m, int n, T alpha, BatchArg<T> A,
         const int w = total - p0 < panel ? total - p0 : panel;
         const int prow = side_left ? m : w, pcol = side_left ? w : n;
         __syncthreads();   // previous panel's reads done (and the triangle staged)
-        for (unsigned e = t; e < (unsigned)(prow * pcol); e += nt) {   // staged sizes fit 32 bits
-            const int i = (int)(e % (unsigned)prow), j = (int)(e / (unsigned)prow);
-            SP[e] = side_left ? GB[i + (long)(p0 + j) * ldb] : GB[p0 + i + (long)j * ldb];
+        const unsigned tot = (unsigned)(prow * pcol);   // staged sizes fit 32 bits
+        for (unsigned e0 = t; e0 < tot; e0 += TM_U * nt) {
+            T v[TM_U];
+#pragma unroll
+            for (int u = 0; u < TM_U; ++u) {
+                const unsigned e = e0 + u * nt;
+                const int i = (int)(e % (unsigned)prow), j = (int)(e / (unsigned)prow);
+                const bool in = e < tot;
+                const T* q = GB + (in ? (side_left ? i + (long)(p0 + j) * ldb : p0 + i + (long)j * ldb) : 0);
+                asm volatile("" : "+v"(q));
+                if constexpr (__is_same(T, double) || __is_same(T, float)) v[u] = gld(q);
+                else v[u] = *q;
+            }
+#pragma unroll
+            for (int u = 0; u < TM_U; ++u)
+                if (e0 + u * nt < tot) SP[e0 + u * nt] = v[u];
         }
         __syncthreads();
         for (unsigned e = t; e < (unsigned)(prow * pcol); e += nt) {   // staged sizes fit 32 bits
