@@ -14,6 +14,8 @@ same inputs timed beside it.  The other workloads go under "configs":
   c2  DPOSV  n=16 x 100,000, 1 RHS
   c3  DPOSV  non-uniform n ~ U{8..256} x 20,000, 4 RHS, pointer arrays
   c5  DSPOSV non-uniform n ~ U{128..512} x 2,000, 64 RHS, cond in [1e2, 1e4]
+  sweep  the reference's published chart shapes: batch 10,240, n = m = 8..256,
+         DPOSV (nrhs = n), DTRSM, DSYRK, DGEMM in GFLOP/s (testing/flops.h)
 
 Every step is one library call captured as a HIP graph (torch.cuda.CUDAGraph
 on a side stream) and replayed; inputs are generated on the GPU with torch
@@ -37,7 +39,7 @@ inputs, OpenMP over the batch on every host core, warmed up, median of 5;
 a single-thread figure and the oracle's own C restatement are reported
 beside it.
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config all|c1|c2|c3|c4|c5|poti]
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config all|c1|c2|c3|c4|c5|poti|sweep]
 """
 import argparse
 import json
@@ -904,7 +906,7 @@ def main():
 
     K, W = args.steps, args.warmup
     h = pykblas.Handle(torch.cuda.current_stream().cuda_stream)
-    names = ["c4", "c1", "c2", "c3", "c5"] if args.config == "all" else [args.config]
+    names = ["c4", "c1", "c2", "c3", "c5", "sweep"] if args.config == "all" else [args.config]
     results = {}
     t_start = time.perf_counter()
     for name in names:
