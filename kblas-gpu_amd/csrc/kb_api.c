@@ -350,9 +350,11 @@ int kblasDgemm_vbatch(kblasHandle_t h, char ta, char tb, int* m, int* n, int* k,
     int a = transof(ta), b = transof(tb);
     if (a < 0 || b < 0 || !m || !n || !k || !lda || !ldb || !ldc || !B) return KBLAS_Error_WrongInput;
     if (bc == 0) return KBLAS_Success;
-    (void)max_m; (void)max_n; (void)max_k;
-    return launched(kb_launch_gemm('d', a, b, 0, 0, 0, alpha, mat_vptrs(A, lda), mat_vptrs(B, ldb), beta,
-                                   mat_vptrs(C, ldc), m, n, k, bc, h->stream));
+    /* maxima bound the sizes (as the reference's MAGMA path assumes); 0 when unknown */
+    const int bounded = max_m > 0 && max_n > 0 && max_k > 0;
+    return launched(kb_launch_gemm('d', a, b, bounded ? max_m : 0, bounded ? max_n : 0, bounded ? max_k : 0, alpha,
+                                   mat_vptrs(A, lda), mat_vptrs(B, ldb), beta, mat_vptrs(C, ldc), m, n, k, bc,
+                                   h->stream));
 }
 int kblasSgemm_vbatch(kblasHandle_t h, char ta, char tb, int* m, int* n, int* k, int max_m, int max_n,
                       int max_k, const float alpha, const float** A, int* lda, const float** B,
@@ -363,13 +365,15 @@ int kblasSgemm_vbatch(kblasHandle_t h, char ta, char tb, int* m, int* n, int* k,
     int a = transof(ta), b = transof(tb);
     if (a < 0 || b < 0 || !m || !n || !k || !lda || !ldb || !ldc || !B) return KBLAS_Error_WrongInput;
     if (bc == 0) return KBLAS_Success;
-    (void)max_m; (void)max_n; (void)max_k;
-    return launched(kb_launch_gemm('s', a, b, 0, 0, 0, alpha, mat_vptrs(A, lda), mat_vptrs(B, ldb), beta,
-                                   mat_vptrs(C, ldc), m, n, k, bc, h->stream));
+    /* maxima bound the sizes (as the reference's MAGMA path assumes); 0 when unknown */
+    const int bounded = max_m > 0 && max_n > 0 && max_k > 0;
+    return launched(kb_launch_gemm('s', a, b, bounded ? max_m : 0, bounded ? max_n : 0, bounded ? max_k : 0, alpha,
+                                   mat_vptrs(A, lda), mat_vptrs(B, ldb), beta, mat_vptrs(C, ldc), m, n, k, bc,
+                                   h->stream));
 }
 
-static int syrk_v(kblasHandle_t h, char prec, char uplo, char trans, int* m, int* n, double alpha,
-                  void* A, int* lda, double beta, void* B, int* ldb, int bc)
+static int syrk_v(kblasHandle_t h, char prec, char uplo, char trans, int* m, int* n, int max_m, int max_n,
+                  double alpha, void* A, int* lda, double beta, void* B, int* ldb, int bc)
 {
     int st = vcheck(h, bc, A, B);
     if (st != KBLAS_Success) return st;
@@ -377,18 +381,19 @@ static int syrk_v(kblasHandle_t h, char prec, char uplo, char trans, int* m, int
     if (u < 0 || t < 0 || !m || !n || !lda || !ldb) return KBLAS_Error_WrongInput;
     if (u == 1) return KBLAS_NotImplemented;
     if (bc == 0) return KBLAS_Success;
-    return launched(kb_launch_syrk(prec, t, 0, 0, alpha, mat_vptrs(A, lda), beta, mat_vptrs(B, ldb),
-                                   m, n, bc, h->stream));
+    const int bounded = max_m > 0 && max_n > 0;
+    return launched(kb_launch_syrk(prec, t, bounded ? max_m : 0, bounded ? max_n : 0, alpha, mat_vptrs(A, lda), beta,
+                                   mat_vptrs(B, ldb), m, n, bc, h->stream));
 }
 int kblasDsyrk_vbatch(kblasHandle_t h, char uplo, char trans, int* m, int* n, int max_m, int max_n,
                       double alpha, double** A, int* lda, double beta, double** B, int* ldb, int bc)
-{ (void)max_m; (void)max_n; return syrk_v(h, 'd', uplo, trans, m, n, alpha, A, lda, beta, B, ldb, bc); }
+{ return syrk_v(h, 'd', uplo, trans, m, n, max_m, max_n, alpha, A, lda, beta, B, ldb, bc); }
 int kblasSsyrk_vbatch(kblasHandle_t h, char uplo, char trans, int* m, int* n, int max_m, int max_n,
                       float alpha, float** A, int* lda, float beta, float** B, int* ldb, int bc)
-{ (void)max_m; (void)max_n; return syrk_v(h, 's', uplo, trans, m, n, alpha, A, lda, beta, B, ldb, bc); }
+{ return syrk_v(h, 's', uplo, trans, m, n, max_m, max_n, alpha, A, lda, beta, B, ldb, bc); }
 
 static int trsm_v(kblasHandle_t h, char prec, char side, char uplo, char trans, char diag, int* m, int* n,
-                  double alpha, void* A, int* lda, void* B, int* ldb, int bc)
+                  int max_m, int max_n, double alpha, void* A, int* lda, void* B, int* ldb, int bc)
 {
     int st = vcheck(h, bc, A, B);
     if (st != KBLAS_Success) return st;
@@ -396,15 +401,16 @@ static int trsm_v(kblasHandle_t h, char prec, char side, char uplo, char trans, 
     if (sd < 0 || u < 0 || t < 0 || dg < 0 || !m || !n || !lda || !ldb) return KBLAS_Error_WrongInput;
     if (u == 1 || dg == 1) return KBLAS_NotImplemented;
     if (bc == 0) return KBLAS_Success;
-    return launched(kb_launch_trsm(prec, sd == 0, t, dg, 0, 0, alpha, mat_vptrs(A, lda), mat_vptrs(B, ldb),
-                                   m, n, bc, h->stream));
+    const int bounded = max_m > 0 && max_n > 0;
+    return launched(kb_launch_trsm(prec, sd == 0, t, dg, bounded ? max_m : 0, bounded ? max_n : 0, alpha,
+                                   mat_vptrs(A, lda), mat_vptrs(B, ldb), m, n, bc, h->stream));
 }
 int kblasDtrsm_vbatch(kblasHandle_t h, char side, char uplo, char trans, char diag, int* m, int* n,
                       int max_m, int max_n, double alpha, double** A, int* lda, double** B, int* ldb, int bc)
-{ (void)max_m; (void)max_n; return trsm_v(h, 'd', side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, bc); }
+{ return trsm_v(h, 'd', side, uplo, trans, diag, m, n, max_m, max_n, alpha, A, lda, B, ldb, bc); }
 int kblasStrsm_vbatch(kblasHandle_t h, char side, char uplo, char trans, char diag, int* m, int* n,
                       int max_m, int max_n, float alpha, float** A, int* lda, float** B, int* ldb, int bc)
-{ (void)max_m; (void)max_n; return trsm_v(h, 's', side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, bc); }
+{ return trsm_v(h, 's', side, uplo, trans, diag, m, n, max_m, max_n, alpha, A, lda, B, ldb, bc); }
 
 /* ---- non-uniform Cholesky family: workspace = the size-sorted queue ---- */
 void kb_ws_need_vbatch(kb_ws_state* ws, int batchCount)

@@ -233,9 +233,11 @@ int gemm_t(int ta, int tb, int m, int n, int k, T alpha, kb_mat_t A, kb_mat_t B,
            int upper = 0)
 {
     if constexpr (__is_same(T, double) || __is_same(T, float)) {
-        if (!vm && !vn && !vk) {   // uniform real batches: the matrix-core kernel of kb_gemm.hip
+        // real batches: the matrix-core kernels of kb_gemm.hip, uniform or with the
+        // non-uniform sizes bounded by m / n / k
+        if ((!vm && !vn && !vk) || (vm && vk && (syrk || vn) && m > 0 && n > 0 && k > 0)) {
             const int e = kb_launch_gemm_fast(sizeof(T) == 8 ? 'd' : 's', syrk, upper, ta, syrk ? !ta : tb, m, n, k,
-                                              (double)alpha, A, B, (double)beta, C, batch, s);
+                                              (double)alpha, A, B, (double)beta, C, vm, syrk ? vm : vn, vk, batch, s);
             if (e != (int)hipErrorNotSupported) return e;
         }
     }
@@ -262,9 +264,11 @@ int trsm_t(int side_left, int trans, int m, int n, T alpha, kb_mat_t A, kb_mat_t
            const int* vm, const int* vn, int batch, hipStream_t s, int upper = 0, int unit = 0)
 {
     if constexpr (__is_same(T, double) || __is_same(T, float)) {
-        if (!vm && !vn) {   // uniform real batches: the register / blocked kernels of kb_trsm.hip
+        // real batches: the register / matrix-core / blocked kernels of kb_trsm.hip,
+        // uniform or with the non-uniform sizes bounded by m / n
+        if ((!vm && !vn) || (vm && vn && m > 0 && n > 0)) {
             const int e = kb_launch_trsm_fast(sizeof(T) == 8 ? 'd' : 's', side_left, upper, trans, unit, m, n,
-                                              (double)alpha, A, B, batch, s);
+                                              (double)alpha, A, B, vm, vn, batch, s);
             if (e != (int)hipErrorNotSupported) return e;
         }
     }

@@ -579,10 +579,17 @@ static void launch_blk_mode(int n, int nrhs, int side_left, BatchArg<T> a, Batch
 // op(L)^T X^T = alpha B^T).
 template <typename T, bool FWD, int CB>
 __global__ void __launch_bounds__(KB_BLK_WG) __attribute__((amdgpu_waves_per_eu(CB == 2 ? 4 : 2)))
-k_blk_trsm(int n, int nrhs, int side_left, int unit, T alpha, BatchArg<T> A, BatchArg<T> B, int batch)
+k_blk_trsm(int n0, int nrhs0, int side_left, int unit, T alpha, BatchArg<T> A, BatchArg<T> B, int batch,
+           const int* vm, const int* vn)
 {
     __shared__ BlkSmem<T> S;
     for (long mat = blockIdx.x; mat < batch; mat += gridDim.x) {
+        int n = n0, nrhs = nrhs0;
+        if (vm) {   // non-uniform: this matrix's order and right-hand-side count
+            n = side_left ? vm[mat] : vn[mat];
+            nrhs = side_left ? vn[mat] : vm[mat];
+            if (n <= 0 || nrhs <= 0) continue;
+        }
         if constexpr (FWD)
             blk_solve_fwd<T, CB>(A.get(mat), A.ldim(mat), n, B.get(mat), B.ldim(mat), side_left, nrhs, S, alpha, unit);
         else
@@ -593,23 +600,23 @@ k_blk_trsm(int n, int nrhs, int side_left, int unit, T alpha, BatchArg<T> A, Bat
 
 template <typename T>
 static int launch_blk_trsm(int fwd, int side_left, int unit, int n, int nrhs, T alpha, kb_mat_t A, kb_mat_t B,
-                           int batch, hipStream_t s)
+                           const int* vm, const int* vn, int batch, hipStream_t s)
 {
     const int grid = batch < 2048 ? batch : 2048;
     BatchArg<T> a = to_arg<T>(A), b = to_arg<T>(B);
     const bool narrow = nrhs <= KB_BLK_NARROW;
     if (fwd && narrow)
         hipLaunchKernelGGL((k_blk_trsm<T, true, 2>), dim3(grid), dim3(KB_BLK_WG), 0, s, n, nrhs, side_left, unit,
-                           alpha, a, b, batch);
+                           alpha, a, b, batch, vm, vn);
     else if (fwd)
         hipLaunchKernelGGL((k_blk_trsm<T, true, 4>), dim3(grid), dim3(KB_BLK_WG), 0, s, n, nrhs, side_left, unit,
-                           alpha, a, b, batch);
+                           alpha, a, b, batch, vm, vn);
     else if (narrow)
         hipLaunchKernelGGL((k_blk_trsm<T, false, 2>), dim3(grid), dim3(KB_BLK_WG), 0, s, n, nrhs, side_left, unit,
-                           alpha, a, b, batch);
+                           alpha, a, b, batch, vm, vn);
     else
         hipLaunchKernelGGL((k_blk_trsm<T, false, 4>), dim3(grid), dim3(KB_BLK_WG), 0, s, n, nrhs, side_left, unit,
-                           alpha, a, b, batch);
+                           alpha, a, b, batch, vm, vn);
     return (int)hipGetLastError();
 }
 

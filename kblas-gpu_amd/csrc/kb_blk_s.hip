@@ -8,7 +8,7 @@ extern "C" int kb_blk_chol_s(int mode, int n, int nrhs, int side_left, kb_mat_t 
 }
 
 extern "C" int kb_blk_trsm_s(int fwd, int side_left, int unit, int n, int nrhs, double alpha, kb_mat_t A,
-                              kb_mat_t B, int batch, hipStream_t s)
+                              kb_mat_t B, const int* vm, const int* vn, int batch, hipStream_t s)
 {
-    return launch_blk_trsm<float>(fwd, side_left, unit, n, nrhs, (float)alpha, A, B, batch, s);
+    return launch_blk_trsm<float>(fwd, side_left, unit, n, nrhs, (float)alpha, A, B, vm, vn, batch, s);
 }

@@ -144,11 +144,13 @@ int kb_launch_syrk_x(char prec, int upper, int trans, int m, int k, const double
 int kb_launch_trsm_x(char prec, int side_left, int upper, int trans, int unit, int m, int n,
                      const double* alpha, kb_mat_t A, kb_mat_t B, const int* vm, const int* vn, int batch,
                      hipStream_t s);
-/* Real uniform TRSM (prec 'd' / 's', trans 0 = N, else T): orders up to 64 by
- * register substitution with the triangle in LDS, lower orders above 64 by the
- * blocked solve; hipErrorNotSupported for an upper triangle above 64. */
+/* Real TRSM (prec 'd' / 's', trans 0 = N, else T): orders up to 16 by register
+ * substitution with the triangle in LDS, up to 64 on the matrix cores, lower
+ * orders above 64 by the blocked solve; hipErrorNotSupported for an upper
+ * triangle above 64.  Non-uniform: per-matrix sizes in vm / vn, bounded by
+ * m / n (NULL: uniform). */
 int kb_launch_trsm_fast(char prec, int side_left, int upper, int trans, int unit, int m, int n, double alpha,
-                        kb_mat_t A, kb_mat_t B, int batch, hipStream_t s);
+                        kb_mat_t A, kb_mat_t B, const int* vm, const int* vn, int batch, hipStream_t s);
 /* (L L^T) X = B for 16 < n <= 64 ('d' / 's'): the two sweeps of the matrix-core
  * TRSM; matrices with skip[b] != 0 are left alone (skip may be NULL). */
 int kb_launch_trsm_solve(char prec, int side_left, int n, int nrhs, kb_mat_t A, kb_mat_t B, const int* skip,
@@ -156,11 +158,13 @@ int kb_launch_trsm_solve(char prec, int side_left, int n, int nrhs, kb_mat_t A, 
 /* Orders above 32 with at least this many right-hand sides: POSV / POTRS as a
  * factorization followed by kb_launch_trsm_solve. */
 #define KB_SOLVE_SPLIT_NRHS 32
-/* Real uniform GEMM / SYRK on the matrix cores (prec 'd' / 's', op 0 = N, else
- * T; syrk: C is m x m, B = A, tb = !ta, upper picks the triangle);
+/* Real GEMM / SYRK on the matrix cores (prec 'd' / 's', op 0 = N, else T;
+ * syrk: C is m x m, B = A, tb = !ta, upper picks the triangle).  Non-uniform:
+ * per-matrix sizes in vm / vn / vk, bounded by m / n / k (NULL: uniform).
  * hipErrorNotSupported when the batch is too large for one launch. */
 int kb_launch_gemm_fast(char prec, int syrk, int upper, int ta, int tb, int m, int n, int k, double alpha,
-                        kb_mat_t A, kb_mat_t B, double beta, kb_mat_t C, int batch, hipStream_t s);
+                        kb_mat_t A, kb_mat_t B, double beta, kb_mat_t C, const int* vm, const int* vn,
+                        const int* vk, int batch, hipStream_t s);
 /* Generic POTRF / POTRS / POSV (mode KB_MODE_*) for any precision and either
  * triangle; info per matrix for POTRF / POSV. */
 int kb_launch_gchol(char prec, int mode, int upper, int n, int nrhs, int side_left, kb_mat_t A, kb_mat_t B,

@@ -29,9 +29,9 @@
 #include "kb_launch.h"
 
 extern "C" int kb_blk_trsm_d(int fwd, int side_left, int unit, int n, int nrhs, double alpha, kb_mat_t A,
-                             kb_mat_t B, int batch, hipStream_t s);
+                             kb_mat_t B, const int* vm, const int* vn, int batch, hipStream_t s);
 extern "C" int kb_blk_trsm_s(int fwd, int side_left, int unit, int n, int nrhs, double alpha, kb_mat_t A,
-                             kb_mat_t B, int batch, hipStream_t s);
+                             kb_mat_t B, const int* vm, const int* vn, int batch, hipStream_t s);
 
 namespace {
 
@@ -63,7 +63,7 @@ KB_DEV T ts_fma(T a, T b, T c)
 template <typename T, int K, bool FWD>
 __global__ void __launch_bounds__(64)
 k_trsm_small(int k, int nvec, int lg, int chunks, long items, int side_left, int upper, int unit, T alpha,
-             BatchArg<T> A, BatchArg<T> B)
+             BatchArg<T> A, BatchArg<T> B, const int* vm, const int* vn)
 {
     extern __shared__ __align__(16) unsigned char ts_smem[];
     constexpr int PK = ts_pk(K);
@@ -77,7 +77,15 @@ k_trsm_small(int k, int nvec, int lg, int chunks, long items, int side_left, int
         const bool live = it < items;
         const long mat = live ? it / chunks : 0;
         const int vec = (int)(live ? it - mat * chunks : 0) * G + li;
-        const bool act = live && vec < nvec;
+        // this group's order and vector count (non-uniform: the matrix's own, at most K)
+        int kg = k, nvg = nvec;
+        if (vm && live) {
+            const int mm = vm[mat], nn = vn[mat];
+            kg = side_left ? mm : nn;
+            nvg = side_left ? nn : mm;
+            if (kg <= 0 || nvg <= 0) kg = nvg = 0;
+        }
+        const bool act = live && vec < nvg;
         // this lane's vector, loaded first so the loads overlap the staging below
         // both matrices' addresses first (in pointer-array mode they are loads, and a
         // wait for them would also wait for every load issued before)
@@ -99,7 +107,7 @@ k_trsm_small(int k, int nvec, int lg, int chunks, long items, int side_left, int
             for (int i = 0; i < K; ++i) {
                 // the address is opaque to the optimizer: otherwise it sees that a clamped
                 // load repeats the first one and moves each load under a branch again
-                const T* q = xp + (act && i < k ? i * es : 0);
+                const T* q = xp + (act && i < kg ? i * es : 0);
                 asm volatile("" : "+v"(q));
                 x[i] = gld(q);
             }
@@ -115,7 +123,7 @@ k_trsm_small(int k, int nvec, int lg, int chunks, long items, int side_left, int
                 for (int u = 0; u < TS_U; ++u) {
                     const int e = li + ((t0 + u) << lg);
                     const int i = e % K, j = e / K;
-                    const bool rd = t0 + u < cnt && i >= j && i < k && !(unit && i == j);
+                    const bool rd = t0 + u < cnt && i >= j && i < kg && !(unit && i == j);
                     const T x = gld(Ap + (rd ? (upper ? j + (long)i * lda : i + (long)j * lda) : 0));
                     v[u] = rd ? x : T(0);
                 }
@@ -123,14 +131,14 @@ k_trsm_small(int k, int nvec, int lg, int chunks, long items, int side_left, int
                 for (int u = 0; u < TS_U; ++u) {
                     const int e = li + ((t0 + u) << lg);
                     const int i = e % K, j = e / K;
-                    if (t0 + u < cnt && i >= j && j < k) Ls[ts_col(K, j) + i - j] = v[u];
+                    if (t0 + u < cnt && i >= j && j < kg) Ls[ts_col(K, j) + i - j] = v[u];
                 }
             }
         }
         __syncthreads();
         if (live) {
             // reciprocal diagonal in place (1 for a unit diagonal)
-            for (int d = li; d < k; d += G) Ls[ts_col(K, d)] = unit ? T(1) : T(1) / Ls[ts_col(K, d)];
+            for (int d = li; d < kg; d += G) Ls[ts_col(K, d)] = unit ? T(1) : T(1) / Ls[ts_col(K, d)];
         }
         __syncthreads();
 #pragma unroll
@@ -138,7 +146,7 @@ k_trsm_small(int k, int nvec, int lg, int chunks, long items, int side_left, int
         if constexpr (FWD) {
 #pragma unroll
             for (int j = 0; j < K; ++j) {
-                if (j < k) {
+                if (j < kg) {
                     const T* c = Ls + ts_col(K, j);
                     x[j] *= c[0];
 #pragma unroll
@@ -148,7 +156,7 @@ k_trsm_small(int k, int nvec, int lg, int chunks, long items, int side_left, int
         } else {
 #pragma unroll
             for (int j = K - 1; j >= 0; --j) {
-                if (j < k) {
+                if (j < kg) {
                     x[j] *= Ls[ts_col(K, j)];
 #pragma unroll
                     for (int i = 0; i < j; ++i) x[i] = ts_fma(-Ls[ts_col(K, i) + j - i], x[j], x[i]);
@@ -158,7 +166,7 @@ k_trsm_small(int k, int nvec, int lg, int chunks, long items, int side_left, int
         if (act) {
 #pragma unroll
             for (int i = 0; i < K; ++i)
-                if (i < k) gst(xp + i * es, x[i]);
+                if (i < kg) gst(xp + i * es, x[i]);
         }
         __syncthreads();
     }
@@ -190,8 +198,8 @@ constexpr int TM_NCB = 2;   // column blocks of 16 vectors per wavefront
 
 template <typename T, int NBK, bool FWD>
 __global__ void __launch_bounds__(256)
-k_trsm_mfma(int k, int nvec, int chunks, long items, int side_left, int upper, int unit, T alpha,
-            BatchArg<T> A, BatchArg<T> B, const int* skip)
+k_trsm_mfma(int k0, int nvec0, int chunks, long items, int side_left, int upper, int unit, T alpha,
+            BatchArg<T> A, BatchArg<T> B, const int* skip, const int* vm, const int* vn)
 {
     typedef typename TileV<T>::v4 v4;
     constexpr int NSLOT = NBK * (NBK + 1) / 2;
@@ -199,10 +207,18 @@ k_trsm_mfma(int k, int nvec, int chunks, long items, int side_left, int upper, i
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int li = lane & 15, lg = lane >> 4;
-    const int nb = (k + 15) >> 4;   // row blocks holding the order
     for (long it = blockIdx.x; it < items; it += gridDim.x) {
         const long mat = it / chunks;
         if (skip && skip[mat]) continue;   // (the whole workgroup: one matrix per item)
+        // this matrix's order and vector count (non-uniform: its own, at most 16 NBK)
+        int k = k0, nvec = nvec0;
+        if (vm) {
+            const int mm = vm[mat], nn = vn[mat];
+            k = side_left ? mm : nn;
+            nvec = side_left ? nn : mm;
+            if (k <= 0 || nvec <= 0) continue;
+        }
+        const int nb = (k + 15) >> 4;   // row blocks holding the order
         const int chunk = (int)(it - mat * chunks);
         const T* Ap = A.get(mat);
         const int lda = A.ldim(mat);
@@ -338,7 +354,7 @@ k_trsm_mfma(int k, int nvec, int chunks, long items, int side_left, int upper, i
 
 template <typename T, int NBK>
 int tm_launch(bool fwd, int k, int nvec, int side_left, int upper, int unit, T alpha, kb_mat_t A, kb_mat_t B,
-              int batch, hipStream_t s, const int* skip = nullptr)
+              int batch, hipStream_t s, const int* skip = nullptr, const int* vm = nullptr, const int* vn = nullptr)
 {
     const int per_wave = 16 * TM_NCB;
     const int waves = nvec >= 4 * per_wave ? 4 : (nvec + per_wave - 1) / per_wave;
@@ -348,16 +364,16 @@ int tm_launch(bool fwd, int k, int nvec, int side_left, int upper, int unit, T a
     BatchArg<T> a = to_arg<T>(A), b = to_arg<T>(B);
     if (fwd)
         hipLaunchKernelGGL((k_trsm_mfma<T, NBK, true>), dim3(grid), dim3(64 * waves), 0, s, k, nvec, chunks, items,
-                           side_left, upper, unit, alpha, a, b, skip);
+                           side_left, upper, unit, alpha, a, b, skip, vm, vn);
     else
         hipLaunchKernelGGL((k_trsm_mfma<T, NBK, false>), dim3(grid), dim3(64 * waves), 0, s, k, nvec, chunks,
-                           items, side_left, upper, unit, alpha, a, b, skip);
+                           items, side_left, upper, unit, alpha, a, b, skip, vm, vn);
     return (int)hipGetLastError();
 }
 
 template <typename T, int K>
 int ts_launch(bool fwd, int k, int nvec, int side_left, int upper, int unit, T alpha, kb_mat_t A, kb_mat_t B,
-              int batch, hipStream_t s)
+              int batch, const int* vm, const int* vn, hipStream_t s)
 {
     constexpr int PM = ts_pmax<T, K>();
     int lg = 0;
@@ -372,21 +388,21 @@ int ts_launch(bool fwd, int k, int nvec, int side_left, int upper, int unit, T a
     BatchArg<T> a = to_arg<T>(A), b = to_arg<T>(B);
     if (fwd)
         hipLaunchKernelGGL((k_trsm_small<T, K, true>), dim3(grid), dim3(64), lds, s, k, nvec, lg, chunks, items,
-                           side_left, upper, unit, alpha, a, b);
+                           side_left, upper, unit, alpha, a, b, vm, vn);
     else
         hipLaunchKernelGGL((k_trsm_small<T, K, false>), dim3(grid), dim3(64), lds, s, k, nvec, lg, chunks, items,
-                           side_left, upper, unit, alpha, a, b);
+                           side_left, upper, unit, alpha, a, b, vm, vn);
     return (int)hipGetLastError();
 }
 
 template <typename T>
 int ts_dispatch(bool fwd, int k, int nvec, int side_left, int upper, int unit, T alpha, kb_mat_t A, kb_mat_t B,
-                int batch, hipStream_t s)
+                int batch, const int* vm, const int* vn, hipStream_t s)
 {
-    if (k <= 8) return ts_launch<T, 8>(fwd, k, nvec, side_left, upper, unit, alpha, A, B, batch, s);
-    if (k <= 16) return ts_launch<T, 16>(fwd, k, nvec, side_left, upper, unit, alpha, A, B, batch, s);
-    if (k <= 32) return tm_launch<T, 2>(fwd, k, nvec, side_left, upper, unit, alpha, A, B, batch, s);
-    return tm_launch<T, 4>(fwd, k, nvec, side_left, upper, unit, alpha, A, B, batch, s);
+    if (k <= 8) return ts_launch<T, 8>(fwd, k, nvec, side_left, upper, unit, alpha, A, B, batch, vm, vn, s);
+    if (k <= 16) return ts_launch<T, 16>(fwd, k, nvec, side_left, upper, unit, alpha, A, B, batch, vm, vn, s);
+    if (k <= 32) return tm_launch<T, 2>(fwd, k, nvec, side_left, upper, unit, alpha, A, B, batch, s, nullptr, vm, vn);
+    return tm_launch<T, 4>(fwd, k, nvec, side_left, upper, unit, alpha, A, B, batch, s, nullptr, vm, vn);
 }
 
 }  // namespace
@@ -395,7 +411,8 @@ int ts_dispatch(bool fwd, int k, int nvec, int side_left, int upper, int unit, T
 // hipErrorNotSupported for what these kernels do not serve (an upper
 // triangle of order above 64): the caller then runs the generic kernel.
 extern "C" int kb_launch_trsm_fast(char prec, int side_left, int upper, int trans, int unit, int m, int n,
-                                   double alpha, kb_mat_t A, kb_mat_t B, int batch, hipStream_t s)
+                                   double alpha, kb_mat_t A, kb_mat_t B, const int* vm, const int* vn, int batch,
+                                   hipStream_t s)
 {
     if (batch < 1 || m < 1 || n < 1) return 0;
     const int k = side_left ? m : n, nvec = side_left ? n : m;
@@ -404,12 +421,13 @@ extern "C" int kb_launch_trsm_fast(char prec, int side_left, int upper, int tran
     const bool fwd = side_left ? !et : et;
     if (k > KB_REG_MAX_N) {
         if (upper) return (int)hipErrorNotSupported;
-        if (prec == 'd') return kb_blk_trsm_d(fwd, side_left, unit, k, nvec, alpha, A, B, batch, s);
-        if (prec == 's') return kb_blk_trsm_s(fwd, side_left, unit, k, nvec, alpha, A, B, batch, s);
+        if (prec == 'd') return kb_blk_trsm_d(fwd, side_left, unit, k, nvec, alpha, A, B, vm, vn, batch, s);
+        if (prec == 's') return kb_blk_trsm_s(fwd, side_left, unit, k, nvec, alpha, A, B, vm, vn, batch, s);
         return (int)hipErrorInvalidValue;
     }
-    if (prec == 'd') return ts_dispatch<double>(fwd, k, nvec, side_left, upper, unit, alpha, A, B, batch, s);
-    if (prec == 's') return ts_dispatch<float>(fwd, k, nvec, side_left, upper, unit, (float)alpha, A, B, batch, s);
+    if (prec == 'd') return ts_dispatch<double>(fwd, k, nvec, side_left, upper, unit, alpha, A, B, batch, vm, vn, s);
+    if (prec == 's')
+        return ts_dispatch<float>(fwd, k, nvec, side_left, upper, unit, (float)alpha, A, B, batch, vm, vn, s);
     return (int)hipErrorInvalidValue;
 }
 

@@ -133,3 +133,80 @@ def test_gemm_alpha_zero_reads_no_operands(h):
                                         2.0, kt.P(dC), m, m * n, batch)
     assert st == pykblas.KBLAS_Success
     assert np.array_equal(kt.to_host(dC), 2.0 * C)
+
+
+def dev_list(mats, dt):
+    return [kt.to_dev(np.asfortranarray(M).T.copy() if M.size else np.zeros((1, 1)), dt) for M in mats]
+
+
+@pytest.mark.parametrize("p", ["D", "S"])
+@pytest.mark.parametrize("ta", ["N", "T"])
+@pytest.mark.parametrize("tb", ["N", "T"])
+@pytest.mark.parametrize("maxs", [(20, 14, 9), (150, 90, 70)])
+def test_nonuniform_gemm_with_bounds(h, p, ta, tb, maxs):
+    # kblas{D,S}gemm_vbatch with max_m / max_n / max_k: the matrix-core kernels sized by the maxima (the
+    # workgroup tiles when both reach 64), each matrix at its own sizes, one with m = 0 skipped
+    dt, eps = PREC[p]
+    torch = kt.torch_dev()
+    rng = np.random.default_rng(sum(maxs) + 2 * (ta == "T") + (tb == "T"))
+    batch = 11
+    ms = rng.integers(1, maxs[0] + 1, batch).astype(np.int32)
+    ns = rng.integers(1, maxs[1] + 1, batch).astype(np.int32)
+    ks = rng.integers(1, maxs[2] + 1, batch).astype(np.int32)
+    ms[0], ns[0], ks[0] = maxs
+    ms[4] = 0
+    As = [rnd(rng, (m, k) if ta == "N" else (k, m), dt) for m, k in zip(ms, ks)]
+    Bs = [rnd(rng, (k, n) if tb == "N" else (n, k), dt) for n, k in zip(ns, ks)]
+    Cs = [rnd(rng, (m, n), dt) for m, n in zip(ms, ns)]
+    dA, dB, dC = dev_list(As, dt), dev_list(Bs, dt), dev_list(Cs, dt)
+    lda = kt.to_dev(np.array([max(A.shape[0], 1) for A in As], np.int32))
+    ldb = kt.to_dev(np.array([max(B.shape[0], 1) for B in Bs], np.int32))
+    ldc = kt.to_dev(np.maximum(ms, 1).astype(np.int32))
+    P = lambda L: torch.tensor([t.data_ptr() for t in L], dtype=torch.int64, device="cuda")
+    pA, pB, pC = P(dA), P(dB), P(dC)                            # held until the call has run
+    dm, dn, dk = kt.to_dev(ms), kt.to_dev(ns), kt.to_dev(ks)
+    st = getattr(h.lib, "kblas%sgemm_vbatch" % p)(h.h, ta.encode(), tb.encode(), kt.P(dm), kt.P(dn), kt.P(dk),
+                                                  maxs[0], maxs[1], maxs[2], -0.5, kt.P(pA), kt.P(lda), kt.P(pB),
+                                                  kt.P(ldb), 2.0, kt.P(pC), kt.P(ldc), batch)
+    assert st == pykblas.KBLAS_Success
+    for i in range(batch):
+        if ms[i] == 0:
+            continue
+        X = kt.to_host(dC[i]).T.astype(np.float64)
+        Co = po.gemm(ta, tb, -0.5, As[i], Bs[i], 2.0, Cs[i])
+        err = np.max(np.abs(X - Co)) / max(np.max(np.abs(Co)), 1e-300)
+        assert err <= 16 * int(ks[i]) * eps, (i, err)
+
+
+@pytest.mark.parametrize("p", ["D", "S"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("maxs", [(24, 10), (130, 70)])
+def test_nonuniform_syrk_with_bounds(h, p, trans, maxs):
+    dt, eps = PREC[p]
+    torch = kt.torch_dev()
+    rng = np.random.default_rng(sum(maxs) + (trans == "T"))
+    batch = 10
+    ms = rng.integers(1, maxs[0] + 1, batch).astype(np.int32)
+    ks = rng.integers(1, maxs[1] + 1, batch).astype(np.int32)
+    ms[0], ks[0] = maxs
+    As = [rnd(rng, (m, k) if trans == "N" else (k, m), dt) for m, k in zip(ms, ks)]
+    Cs = [rnd(rng, (m, m), dt) for m in ms]
+    for C in Cs:
+        C[np.triu_indices(C.shape[0], 1)] = np.nan
+    dA, dC = dev_list(As, dt), dev_list(Cs, dt)
+    lda = kt.to_dev(np.array([A.shape[0] for A in As], np.int32))
+    ldc = kt.to_dev(ms.copy())
+    P = lambda L: torch.tensor([t.data_ptr() for t in L], dtype=torch.int64, device="cuda")
+    pA, pC = P(dA), P(dC)                                       # held until the call has run
+    dm, dk = kt.to_dev(ms), kt.to_dev(ks)
+    st = getattr(h.lib, "kblas%ssyrk_vbatch" % p)(h.h, b"L", trans.encode(), kt.P(dm), kt.P(dk), maxs[0], maxs[1],
+                                                  0.75, kt.P(pA), kt.P(lda), -1.0, kt.P(pC), kt.P(ldc), batch)
+    assert st == pykblas.KBLAS_Success
+    for i in range(batch):
+        m = int(ms[i])
+        X = kt.to_host(dC[i]).T.astype(np.float64)
+        Co = po.syrk(trans, 0.75, As[i], -1.0, np.nan_to_num(Cs[i]))
+        tri = np.tril_indices(m)
+        err = np.max(np.abs(X[tri] - Co[tri])) / max(np.max(np.abs(Co[tri])), 1e-300)
+        assert err <= 16 * int(ks[i]) * eps, (i, err)
+        assert np.all(np.isnan(X[np.triu_indices(m, 1)])), i

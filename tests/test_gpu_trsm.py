@@ -129,3 +129,53 @@ def test_blocked_trsm_unit_and_upper(h, side, trans):
     # lower unit: the blocked sweep with a unit diagonal; upper above 64: the generic kernel
     run_case(h, "D", side, False, trans, True, *((100, 20) if side == "L" else (20, 100)), batch=3, seed=5, ptrs=False)
     run_case(h, "D", side, True, trans, False, *((90, 12) if side == "L" else (12, 90)), batch=2, seed=6, ptrs=True)
+
+
+def ptrs(torch, tensors):
+    return torch.tensor([t.data_ptr() for t in tensors], dtype=torch.int64, device="cuda")
+
+
+@pytest.mark.parametrize("p", ["D", "S"])
+@pytest.mark.parametrize("side", ["L", "R"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("max_order", [12, 50, 150])
+def test_nonuniform_trsm_with_bounds(h, p, side, trans, max_order):
+    # kblas{D,S}trsm_vbatch with max_m / max_n given: the register (<= 16), matrix-core (<= 64) or blocked
+    # kernel chosen by the largest order, each matrix solved at its own sizes; one matrix has a zero size
+    dt, eps = PREC[p]
+    torch = kt.torch_dev()
+    rng = np.random.default_rng(max_order + 7 * (side == "L") + 3 * (trans == "T"))
+    batch = 9
+    orders = rng.integers(1, max_order + 1, batch)
+    orders[0] = max_order
+    nvecs = rng.integers(1, 70, batch)
+    nvecs[3] = 0
+    ms = np.where(side == "L", orders, nvecs).astype(np.int32)
+    ns = np.where(side == "L", nvecs, orders).astype(np.int32)
+    Ts, Bs, dT, dB = [], [], [], []
+    for i in range(batch):
+        k = int(orders[i])
+        T = triangle(rng, k, False, False)
+        B = rng.standard_normal((int(ms[i]), int(ns[i])))
+        if dt == np.float32:
+            T = T.astype(np.float32).astype(np.float64)
+            B = B.astype(np.float32).astype(np.float64)
+        Ts.append(T)
+        Bs.append(B)
+        dT.append(kt.to_dev(np.asfortranarray(T).T.copy(), dt))
+        dB.append(kt.to_dev(np.asfortranarray(B).T.copy() if B.size else np.zeros((1, 1)), dt))
+    lda = kt.to_dev(orders.astype(np.int32))
+    ldb = kt.to_dev(np.maximum(ms, 1).astype(np.int32))
+    dm, dn = kt.to_dev(ms), kt.to_dev(ns)
+    pT, pB = ptrs(torch, dT), ptrs(torch, dB)   # held until the call has run
+    st = getattr(h.lib, "kblas%strsm_vbatch" % p)(h.h, side.encode(), b"L", trans.encode(), b"N", kt.P(dm), kt.P(dn),
+                                                  int(ms.max()), int(ns.max()), 0.5, kt.P(pT), kt.P(lda), kt.P(pB),
+                                                  kt.P(ldb), batch)
+    assert st == pykblas.KBLAS_Success
+    for i in range(batch):
+        if ms[i] == 0 or ns[i] == 0:
+            continue
+        X = kt.to_host(dB[i]).T.astype(np.float64)
+        Xo = po.trsm(side, trans, 0.5, np.nan_to_num(Ts[i]), Bs[i])
+        err = np.max(np.abs(X - Xo)) / max(np.max(np.abs(Xo)), 1e-300)
+        assert err <= 64 * int(orders[i]) * eps, (i, err)
