@@ -590,6 +590,14 @@ k_blk_trsm(int n0, int nrhs0, int side_left, int unit, T alpha, BatchArg<T> A, B
             nrhs = side_left ? vn[mat] : vm[mat];
             if (n <= 0 || nrhs <= 0) continue;
         }
+        if (alpha == T(0)) {
+            // B := 0 without reading it (BLAS semantics for alpha = 0)
+            T* Bp = B.get(mat);
+            const int ldb = B.ldim(mat);
+            for (long e = threadIdx.x; e < (long)n * nrhs; e += KB_BLK_WG)
+                gst(rhs_elem(Bp, ldb, side_left, (int)(e % n), (int)(e / n)), T(0));
+            continue;
+        }
         if constexpr (FWD)
             blk_solve_fwd<T, CB>(A.get(mat), A.ldim(mat), n, B.get(mat), B.ldim(mat), side_left, nrhs, S, alpha, unit);
         else

@@ -210,3 +210,16 @@ def test_nonuniform_syrk_with_bounds(h, p, trans, maxs):
         err = np.max(np.abs(X[tri] - Co[tri])) / max(np.max(np.abs(Co[tri])), 1e-300)
         assert err <= 16 * int(ks[i]) * eps, (i, err)
         assert np.all(np.isnan(X[np.triu_indices(m, 1)])), i
+
+
+def test_gemm_workgroup_tiles_alpha_zero_reads_no_operands(h):
+    # the 64 x 64 workgroup-tile kernel: alpha = 0 leaves C := beta C without reading A or B
+    m, n, k, batch = 70, 66, 20, 3
+    A = np.full((batch, k, m), np.nan)
+    B = np.full((batch, n, k), np.nan)
+    C = np.random.default_rng(2).standard_normal((batch, n, m))
+    dA, dB, dC = kt.to_dev(A), kt.to_dev(B), kt.to_dev(C)
+    st = h.lib.kblasDgemm_batch_strided(h.h, b"N", b"N", m, n, k, 0.0, kt.P(dA), m, m * k, kt.P(dB), k, k * n,
+                                        -1.5, kt.P(dC), m, m * n, batch)
+    assert st == pykblas.KBLAS_Success
+    assert np.array_equal(kt.to_host(dC), -1.5 * C)

@@ -179,3 +179,20 @@ def test_nonuniform_trsm_with_bounds(h, p, side, trans, max_order):
         Xo = po.trsm(side, trans, 0.5, np.nan_to_num(Ts[i]), Bs[i])
         err = np.max(np.abs(X - Xo)) / max(np.max(np.abs(Xo)), 1e-300)
         assert err <= 64 * int(orders[i]) * eps, (i, err)
+
+
+@pytest.mark.parametrize("k", [8, 40, 100])
+@pytest.mark.parametrize("side", ["L", "R"])
+def test_trsm_alpha_zero_sets_b_without_reading(h, k, side):
+    # alpha = 0: B := 0 and B is not read (NaN and Inf in B leave zeros), in each kernel bucket
+    rng = np.random.default_rng(k)
+    batch, nv = 5, 7
+    m, n = (k, nv) if side == "L" else (nv, k)
+    T = [triangle(rng, k, False, False) for _ in range(batch)]
+    B = np.full((batch, n, m), np.nan)
+    B[:, 0, 0] = np.inf
+    dT, dB = kt.to_dev(np.stack([t.T for t in T])), kt.to_dev(B)
+    st = h.lib.kblasDtrsm_batch_strided(h.h, side.encode(), b"L", b"N", b"N", m, n, 0.0, kt.P(dT), k, k * k,
+                                        kt.P(dB), m, m * n, batch)
+    assert st == pykblas.KBLAS_Success
+    assert np.array_equal(kt.to_host(dB), np.zeros_like(B))
