@@ -134,8 +134,10 @@ k_ara_syrk(const int* __restrict__ ms, const int* __restrict__ ns, const float* 
         for (int t = 0; t < 4; ++t) {
             const int r = r0 + 4 * lg + t;
             const bool in = r < m;
-            x0[t] = (in && c0) ? (double)gld(p0 + r) : 0.0;
-            x1[t] = (in && c1 && two) ? (double)gld(p1 + r) : 0.0;
+            const bool i0 = in && c0, i1 = in && c1 && two;
+            const float v0 = gld_opaque(i0 ? p0 + r : a), v1 = gld_opaque(i1 ? p1 + r : a);
+            x0[t] = i0 ? (double)v0 : 0.0;
+            x1[t] = i1 ? (double)v1 : 0.0;
         }
 #pragma unroll
         for (int t = 0; t < 4; ++t) {

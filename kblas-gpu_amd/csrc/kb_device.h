@@ -65,6 +65,18 @@ KB_DEV T gld(const T* p)
 {
     return *(const __attribute__((address_space(1))) T*)p;
 }
+
+// gld for one of a group of loads meant to be in flight together: the caller
+// clamps p to a valid address where the value goes unused and selects after
+// the load; the address is kept opaque to the optimizer, which would
+// otherwise see clamped loads repeat one another and put each load under a
+// branch of its own (one wait per element).
+template <typename T>
+KB_DEV T gld_opaque(const T* p)
+{
+    asm volatile("" : "+v"(p));
+    return gld(p);
+}
 template <typename T>
 KB_DEV void gst(T* p, T v)
 {

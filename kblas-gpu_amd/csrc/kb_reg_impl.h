@@ -190,9 +190,9 @@ KB_DEV void reg_chol_one(int n, int nrhs, int side_left, T* Ap, int lda, T* Bp, 
             T x[RC];
 #pragma unroll
             for (int c = 0; c < RC; ++c) {
-                T v = T(0);
-                if (r < n && c0 + c < nrhs) v = gld(rhs_elem(Bp, ldb, side_left, r, c0 + c));
-                x[c] = v;
+                const bool in = r < n && c0 + c < nrhs;
+                const T v = gld_opaque(rhs_elem(Bp, ldb, side_left, in ? r : 0, in ? c0 + c : 0));
+                x[c] = in ? v : T(0);
             }
             reg_forward<T, NB, G, RC, EX>(a, x, n, my_inv);
             if (Lt) reg_backward<T, NB, G, RC, EX>(Lt, x, n, r, my_inv);
@@ -250,9 +250,9 @@ KB_DEV void reg_sposv_one(int n, int nrhs, int side_left, double* Ap, int lda, d
     double ad[NB];
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
-        double v = 0.0;
-        if (r < n && k < n) v = gld(Ap + ((k <= r) ? r + (long)k * lda : k + (long)r * lda));
-        ad[k] = v;
+        const bool in = r < n && k < n;
+        const double v = gld_opaque(Ap + (in ? ((k <= r) ? r + (long)k * lda : k + (long)r * lda) : 0));
+        ad[k] = in ? v : 0.0;
     }
     double rowsum = 0.0;
 #pragma unroll
@@ -262,8 +262,17 @@ KB_DEV void reg_sposv_one(int n, int nrhs, int side_left, double* Ap, int lda, d
 
     // FP32 range of B, then of A (dsposv's DLAG2S / DLAT2S checks): ITER = -2
     int small = 1;
-    for (int c = 0; c < nrhs; ++c)
-        if (r < n) small &= !(__builtin_fabs(gld(rhs_elem(Bp, ldb, side_left, r, c))) > 3.4028234663852886e38);
+    for (int c0 = 0; c0 < nrhs; c0 += 4) {   // four loads in flight per step
+        double v[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const bool in = r < n && c0 + c < nrhs;
+            v[c] = gld_opaque(rhs_elem(Bp, ldb, side_left, in ? r : 0, in ? c0 + c : 0));
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            if (r < n && c0 + c < nrhs) small &= !(__builtin_fabs(v[c]) > 3.4028234663852886e38);
+    }
 #pragma unroll
     for (int k = 0; k < NB; ++k) small &= !(k <= r && __builtin_fabs(ad[k]) > 3.4028234663852886e38);
     const bool ovf = !group_all<G>(small);
@@ -285,8 +294,9 @@ KB_DEV void reg_sposv_one(int n, int nrhs, int side_left, double* Ap, int lda, d
             double b[RC], x[RC], res[RC];
 #pragma unroll
             for (int c = 0; c < RC; ++c) {
-                double v = 0.0;
-                if (r < n && c0 + c < nrhs) v = gld(rhs_elem(Bp, ldb, side_left, r, c0 + c));
+                const bool in = r < n && c0 + c < nrhs;
+                const double lv = gld_opaque(rhs_elem(Bp, ldb, side_left, in ? r : 0, in ? c0 + c : 0));
+                const double v = in ? lv : 0.0;
                 b[c] = v;
                 x[c] = 0.0;
                 res[c] = v;
@@ -354,9 +364,9 @@ KB_DEV void reg_sposv_one(int n, int nrhs, int side_left, double* Ap, int lda, d
                 double x[RC];
 #pragma unroll
                 for (int c = 0; c < RC; ++c) {
-                    double v = 0.0;
-                    if (r < n && c0 + c < nrhs) v = gld(rhs_elem(Bp, ldb, side_left, r, c0 + c));
-                    x[c] = v;
+                    const bool in = r < n && c0 + c < nrhs;
+                    const double lv = gld_opaque(rhs_elem(Bp, ldb, side_left, in ? r : 0, in ? c0 + c : 0));
+                    x[c] = in ? lv : 0.0;
                 }
                 reg_forward<double, NB, G, RC>(ad, x, n, dinv);
                 // backward sweep reads L(k,r) from the factor just written to A
