@@ -43,19 +43,19 @@ static int opof(char prec, char c)
 static kb_mat_t mat_strided(const void* p, int ld, long stride)
 {
     kb_mat_t m;
-    m.base = (void*)p; m.stride = stride; m.ptrs = NULL; m.lds = NULL; m.ld = ld;
+    m.base = (void*)p; m.stride = stride; m.ptrs = NULL; m.lds = NULL; m.ld = ld; m.off = 0;
     return m;
 }
 static kb_mat_t mat_ptrs(const void* pp, int ld)
 {
     kb_mat_t m;
-    m.base = NULL; m.stride = 0; m.ptrs = (void* const*)pp; m.lds = NULL; m.ld = ld;
+    m.base = NULL; m.stride = 0; m.ptrs = (void* const*)pp; m.lds = NULL; m.ld = ld; m.off = 0;
     return m;
 }
 static kb_mat_t mat_vptrs(const void* pp, const int* lds)
 {
     kb_mat_t m;
-    m.base = NULL; m.stride = 0; m.ptrs = (void* const*)pp; m.lds = lds; m.ld = 0;
+    m.base = NULL; m.stride = 0; m.ptrs = (void* const*)pp; m.lds = lds; m.ld = 0; m.off = 0;
     return m;
 }
 
@@ -81,6 +81,18 @@ static int chol_dispatch(kblasHandle_t h, char prec, int mode, int n, int nrhs, 
     /* the upper triangle and the complex types: the generic kernels */
     if (upper || cplx(prec)) return kb_launch_gchol(prec, mode, upper, n, nrhs, side_left, A, B, info, bc, h->stream);
     const int v = kb_tune_variant(prec, mode, n);
+    if (mode != KB_MODE_POTRF && n > KB_REG_MAX_N && nrhs >= KB_TRSM_REC_NVEC) {
+        /* many right-hand sides above the register range: the blocked
+         * factorization, then the two solves by half-order recursion with
+         * matrix-core GEMM updates (a failed factorization leaves that
+         * matrix's B as it was) */
+        if (mode == KB_MODE_POSV) {
+            const int e = kb_launch_blk_chol(prec, KB_MODE_POTRF, n, 0, side_left, A, B, info, bc, h->stream);
+            if (e) return e;
+        }
+        return kb_launch_trsm_rec_solve(prec, side_left, n, nrhs, A, B, mode == KB_MODE_POSV ? info : NULL, bc,
+                                        h->stream);
+    }
     if (v == KB_VAR_BLK || n > KB_REG_MAX_N)
         return kb_launch_blk_chol(prec, mode, n, nrhs, side_left, A, B, info, bc, h->stream);
     if (mode != KB_MODE_POTRF && n > 32 && nrhs >= KB_SOLVE_SPLIT_NRHS) {

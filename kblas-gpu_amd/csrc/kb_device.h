@@ -27,9 +27,10 @@ struct BatchArg {
     T* const* ptrs;      // pointer-array mode (device array of device pointers)
     const int* lds;      // non-uniform leading dimensions (device), or nullptr
     int ld;              // uniform leading dimension
+    long off;            // element offset into every matrix (a sub-block)
     KB_DEV T* get(long b) const
     {
-        return ptrs ? ptrs[b] : base + b * stride;
+        return (ptrs ? ptrs[b] : base + b * stride) + off;
     }
     KB_DEV int ldim(long b) const { return lds ? lds[b] : ld; }
 };
@@ -43,6 +44,7 @@ static BatchArg<T> to_arg(kb_mat_t m)
     a.ptrs = (T* const*)m.ptrs;
     a.lds = m.lds;
     a.ld = m.ld;
+    a.off = m.off;
     return a;
 }
 

@@ -69,7 +69,7 @@ KB_DEV void gm_load(const T* Ap, int lda, const T* Bp, int ldb, int m, int n, in
 template <typename T, int TM, bool TA, bool TB>
 __global__ void __launch_bounds__(64)
 k_gemm_mfma(int m, int n, int k, T alpha, BatchArg<T> A, BatchArg<T> B, T beta, BatchArg<T> C, int batch,
-            int tiles, int tm, int syrk, int upper, const int* vm, const int* vn, const int* vk)
+            int tiles, int tm, int syrk, int upper, const int* vm, const int* vn, const int* vk, const int* skip)
 {
     constexpr int NB = TM / 16, KU = gm_ku<T, TM>();
     typedef typename TileV<T>::v4 v4;
@@ -80,7 +80,7 @@ k_gemm_mfma(int m, int n, int k, T alpha, BatchArg<T> A, BatchArg<T> B, T beta, 
     const int r8 = (int)(w - grp * 8L * tiles);
     const int t = r8 >> 3;
     const long mat = grp * 8 + (r8 & 7);
-    if (mat >= batch) return;
+    if (mat >= batch || (skip && skip[mat])) return;   // skip: C left as it is
     int I, J;
     if (syrk) {
         // lower tiles (I, J <= I) in row order; upper: the transposed tile
@@ -210,7 +210,7 @@ KB_DEV void gl_store(T (*sA)[GL_LD], T (*sB)[GL_LD], int m, int n, int k, int I0
 template <typename T, bool TA, bool TB>
 __global__ void __launch_bounds__(256)
 k_gemm_lds(int m, int n, int k, T alpha, BatchArg<T> A, BatchArg<T> B, T beta, BatchArg<T> C, int batch, int tiles,
-           int tm, int syrk, int upper, const int* vm, const int* vn, const int* vk)
+           int tm, int syrk, int upper, const int* vm, const int* vn, const int* vk, const int* skip)
 {
     typedef typename TileV<T>::v4 v4;
     __shared__ T sA[2][GL_KC][GL_LD];
@@ -222,7 +222,7 @@ k_gemm_lds(int m, int n, int k, T alpha, BatchArg<T> A, BatchArg<T> B, T beta, B
     const int r8 = (int)(w - grp * 8L * tiles);
     const int t = r8 >> 3;
     const long mat = grp * 8 + (r8 & 7);
-    if (mat >= batch) return;
+    if (mat >= batch || (skip && skip[mat])) return;   // skip: C left as it is
     int I, J;
     if (syrk) {
         I = 0;
@@ -305,20 +305,20 @@ k_gemm_lds(int m, int n, int k, T alpha, BatchArg<T> A, BatchArg<T> B, T beta, B
 
 template <typename T, bool TA, bool TB>
 int gl_launch(int m, int n, int k, T alpha, kb_mat_t A, kb_mat_t B, T beta, kb_mat_t C, int batch, int syrk,
-              int upper, const int* vm, const int* vn, const int* vk, hipStream_t s)
+              int upper, const int* vm, const int* vn, const int* vk, const int* skip, hipStream_t s)
 {
     const int tm = (m + GL_TM - 1) / GL_TM, tn = (n + GL_TM - 1) / GL_TM;
     const int tiles = syrk ? tm * (tm + 1) / 2 : tm * tn;
     const long wgs = (batch + 7) / 8 * 8L * tiles;
     if (wgs >= (1L << 24)) return (int)hipErrorNotSupported;   // grid of at most 2^32 work-items
     hipLaunchKernelGGL((k_gemm_lds<T, TA, TB>), dim3((unsigned)wgs), dim3(256), 0, s, m, n, k, alpha, to_arg<T>(A),
-                       to_arg<T>(B), beta, to_arg<T>(C), batch, tiles, tm, syrk, upper, vm, vn, vk);
+                       to_arg<T>(B), beta, to_arg<T>(C), batch, tiles, tm, syrk, upper, vm, vn, vk, skip);
     return (int)hipGetLastError();
 }
 
 template <typename T, int TM, bool TA, bool TB>
 int gm_launch(int m, int n, int k, T alpha, kb_mat_t A, kb_mat_t B, T beta, kb_mat_t C, int batch, int syrk,
-              int upper, const int* vm, const int* vn, const int* vk, hipStream_t s)
+              int upper, const int* vm, const int* vn, const int* vk, const int* skip, hipStream_t s)
 {
     const int tm = (m + TM - 1) / TM, tn = (n + TM - 1) / TM;
     const int tiles = syrk ? tm * (tm + 1) / 2 : tm * tn;
@@ -326,43 +326,45 @@ int gm_launch(int m, int n, int k, T alpha, kb_mat_t A, kb_mat_t B, T beta, kb_m
     const long wgs = groups * 8L * tiles;
     if (wgs >= (1L << 26)) return (int)hipErrorNotSupported;   // grid of at most 2^32 work-items
     hipLaunchKernelGGL((k_gemm_mfma<T, TM, TA, TB>), dim3((unsigned)wgs), dim3(64), 0, s, m, n, k, alpha,
-                       to_arg<T>(A), to_arg<T>(B), beta, to_arg<T>(C), batch, tiles, tm, syrk, upper, vm, vn, vk);
+                       to_arg<T>(A), to_arg<T>(B), beta, to_arg<T>(C), batch, tiles, tm, syrk, upper, vm, vn, vk,
+                       skip);
     return (int)hipGetLastError();
 }
 
 template <typename T, int TM>
 int gm_trans(int ta, int tb, int m, int n, int k, T alpha, kb_mat_t A, kb_mat_t B, T beta, kb_mat_t C, int batch,
-             int syrk, int upper, const int* vm, const int* vn, const int* vk, hipStream_t s)
+             int syrk, int upper, const int* vm, const int* vn, const int* vk, const int* skip, hipStream_t s)
 {
     if constexpr (TM == GL_TM) {
         if (!ta && !tb) return gl_launch<T, false, false>(m, n, k, alpha, A, B, beta, C,
-            batch, syrk, upper, vm, vn, vk, s);
+            batch, syrk, upper, vm, vn, vk, skip, s);
         if (!ta && tb) return gl_launch<T, false, true>(m, n, k, alpha, A, B, beta, C,
-            batch, syrk, upper, vm, vn, vk, s);
+            batch, syrk, upper, vm, vn, vk, skip, s);
         if (ta && !tb) return gl_launch<T, true, false>(m, n, k, alpha, A, B, beta, C,
-            batch, syrk, upper, vm, vn, vk, s);
-        return gl_launch<T, true, true>(m, n, k, alpha, A, B, beta, C, batch, syrk, upper, vm, vn, vk, s);
+            batch, syrk, upper, vm, vn, vk, skip, s);
+        return gl_launch<T, true, true>(m, n, k, alpha, A, B, beta, C, batch, syrk, upper, vm, vn, vk, skip, s);
     }
     if (!ta && !tb) return gm_launch<T, TM, false, false>(m, n, k, alpha, A, B, beta, C,
-        batch, syrk, upper, vm, vn, vk, s);
+        batch, syrk, upper, vm, vn, vk, skip, s);
     if (!ta && tb) return gm_launch<T, TM, false, true>(m, n, k, alpha, A, B, beta, C,
-        batch, syrk, upper, vm, vn, vk, s);
+        batch, syrk, upper, vm, vn, vk, skip, s);
     if (ta && !tb) return gm_launch<T, TM, true, false>(m, n, k, alpha, A, B, beta, C,
-        batch, syrk, upper, vm, vn, vk, s);
-    return gm_launch<T, TM, true, true>(m, n, k, alpha, A, B, beta, C, batch, syrk, upper, vm, vn, vk, s);
+        batch, syrk, upper, vm, vn, vk, skip, s);
+    return gm_launch<T, TM, true, true>(m, n, k, alpha, A, B, beta, C, batch, syrk, upper, vm, vn, vk, skip, s);
 }
 
 template <typename T>
 int gm_dispatch(int ta, int tb, int m, int n, int k, T alpha, kb_mat_t A, kb_mat_t B, T beta, kb_mat_t C,
-                int batch, int syrk, int upper, const int* vm, const int* vn, const int* vk, hipStream_t s)
+                int batch, int syrk, int upper, const int* vm, const int* vn, const int* vk, const int* skip,
+                hipStream_t s)
 {
     if (m <= 16 || n <= 16) return gm_trans<T, 16>(ta, tb, m, n, k, alpha, A, B, beta, C,
-        batch, syrk, upper, vm, vn, vk, s);
+        batch, syrk, upper, vm, vn, vk, skip, s);
     // SYRK keeps the wavefront tiles: they skip the quarters above the diagonal,
     // which a 64 x 64 workgroup tile computes and discards (measured slower)
     if (syrk || m < GM_LDS_MIN || n < GM_LDS_MIN)
-        return gm_trans<T, 32>(ta, tb, m, n, k, alpha, A, B, beta, C, batch, syrk, upper, vm, vn, vk, s);
-    return gm_trans<T, GL_TM>(ta, tb, m, n, k, alpha, A, B, beta, C, batch, syrk, upper, vm, vn, vk, s);
+        return gm_trans<T, 32>(ta, tb, m, n, k, alpha, A, B, beta, C, batch, syrk, upper, vm, vn, vk, skip, s);
+    return gm_trans<T, GL_TM>(ta, tb, m, n, k, alpha, A, B, beta, C, batch, syrk, upper, vm, vn, vk, skip, s);
 }
 
 }  // namespace
@@ -373,17 +375,26 @@ int gm_dispatch(int ta, int tb, int m, int n, int k, T alpha, kb_mat_t A, kb_mat
 // (device arrays) and their maxima in m / n / k, which must bound them.
 // hipErrorNotSupported when the grid would be too large (the caller then runs
 // the general kernel).
-extern "C" int kb_launch_gemm_fast(char prec, int syrk, int upper, int ta, int tb, int m, int n, int k,
+// (skip: optional per-matrix flags, a matrix with skip[b] != 0 keeps its C.)
+extern "C" int kb_launch_gemm_skip(char prec, int syrk, int upper, int ta, int tb, int m, int n, int k,
                                    double alpha, kb_mat_t A, kb_mat_t B, double beta, kb_mat_t C, const int* vm,
-                                   const int* vn, const int* vk, int batch, hipStream_t s)
+                                   const int* vn, const int* vk, const int* skip, int batch, hipStream_t s)
 {
     if (batch < 1 || m < 1 || n < 1) return 0;
     ta = ta != 0;
     tb = tb != 0;
     if (prec == 'd')
-        return gm_dispatch<double>(ta, tb, m, n, k, alpha, A, B, beta, C, batch, syrk, upper, vm, vn, vk, s);
+        return gm_dispatch<double>(ta, tb, m, n, k, alpha, A, B, beta, C, batch, syrk, upper, vm, vn, vk, skip, s);
     if (prec == 's')
         return gm_dispatch<float>(ta, tb, m, n, k, (float)alpha, A, B, (float)beta, C, batch, syrk, upper, vm, vn,
-                                  vk, s);
+                                  vk, skip, s);
     return (int)hipErrorInvalidValue;
+}
+
+extern "C" int kb_launch_gemm_fast(char prec, int syrk, int upper, int ta, int tb, int m, int n, int k,
+                                   double alpha, kb_mat_t A, kb_mat_t B, double beta, kb_mat_t C, const int* vm,
+                                   const int* vn, const int* vk, int batch, hipStream_t s)
+{
+    return kb_launch_gemm_skip(prec, syrk, upper, ta, tb, m, n, k, alpha, A, B, beta, C, vm, vn, vk, nullptr, batch,
+                               s);
 }

@@ -14,13 +14,16 @@ extern "C" {
 #endif
 
 /* A batch operand: strided (base + b*stride), pointer array (ptrs[b]) and a
- * uniform (ld) or per-matrix (lds[b]) leading dimension.  All device memory. */
+ * uniform (ld) or per-matrix (lds[b]) leading dimension, plus an element
+ * offset into every matrix (a sub-block; 0 for the caller's matrices).  All
+ * device memory. */
 typedef struct {
     void* base;
     long stride;
     void* const* ptrs;
     const int* lds;
     int ld;
+    long off;
 } kb_mat_t;
 
 enum { KB_MODE_POTRF = 0, KB_MODE_POSV = 1, KB_MODE_POTRS = 2 };
@@ -158,6 +161,12 @@ int kb_launch_trsm_solve(char prec, int side_left, int n, int nrhs, kb_mat_t A, 
 /* Orders above 32 with at least this many right-hand sides: POSV / POTRS as a
  * factorization followed by kb_launch_trsm_solve. */
 #define KB_SOLVE_SPLIT_NRHS 32
+/* Orders above 64 with at least this many vectors: TRSM (lower, uniform) and
+ * the solves of POSV / POTRS by the recursion into half-order solves and one
+ * matrix-core GEMM per level (kb_trsm.hip). */
+#define KB_TRSM_REC_NVEC 64
+int kb_launch_trsm_rec_solve(char prec, int side_left, int n, int nrhs, kb_mat_t A, kb_mat_t B, const int* skip,
+                             int batch, hipStream_t s);
 /* Real GEMM / SYRK on the matrix cores (prec 'd' / 's', op 0 = N, else T;
  * syrk: C is m x m, B = A, tb = !ta, upper picks the triangle).  Non-uniform:
  * per-matrix sizes in vm / vn / vk, bounded by m / n / k (NULL: uniform).
@@ -165,6 +174,11 @@ int kb_launch_trsm_solve(char prec, int side_left, int n, int nrhs, kb_mat_t A, 
 int kb_launch_gemm_fast(char prec, int syrk, int upper, int ta, int tb, int m, int n, int k, double alpha,
                         kb_mat_t A, kb_mat_t B, double beta, kb_mat_t C, const int* vm, const int* vn,
                         const int* vk, int batch, hipStream_t s);
+/* The same with optional per-matrix skip flags (a matrix with skip[b] != 0
+ * keeps its C). */
+int kb_launch_gemm_skip(char prec, int syrk, int upper, int ta, int tb, int m, int n, int k, double alpha,
+                        kb_mat_t A, kb_mat_t B, double beta, kb_mat_t C, const int* vm, const int* vn,
+                        const int* vk, const int* skip, int batch, hipStream_t s);
 /* Generic POTRF / POTRS / POSV (mode KB_MODE_*) for any precision and either
  * triangle; info per matrix for POTRF / POSV. */
 int kb_launch_gchol(char prec, int mode, int upper, int n, int nrhs, int side_left, kb_mat_t A, kb_mat_t B,

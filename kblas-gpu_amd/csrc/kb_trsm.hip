@@ -405,6 +405,55 @@ int ts_dispatch(bool fwd, int k, int nvec, int side_left, int upper, int unit, T
     return tm_launch<T, 4>(fwd, k, nvec, side_left, upper, unit, alpha, A, B, batch, s, nullptr, vm, vn);
 }
 
+// Lower triangle of order k > 64 with many vectors: split the order at k1,
+// solve one part (alpha applied), update the other part of B with one batched
+// GEMM on the matrix cores, solve the other part; the parts recurse until
+// each leaf, of order 33..64 (the split balances the leaves), runs the
+// matrix-core solve.  In terms of the vectors y of B (side_left: columns,
+// else rows):
+//   forward  L y = alpha b:    y1 = L11^-1 alpha b1;   b2 := alpha b2 - L21 y1;    y2 = L22^-1 b2
+//   backward L^T y = alpha b:  y2 = L22^-T alpha b2;   b1 := alpha b1 - L21^T y2;  y1 = L11^-T b1
+// with the update as B2 := alpha B2 - L21 Y1 (left) or B2 := alpha B2 - Y1 L21^T
+// (right), and the backward forms B1 := alpha B1 - L21^T Y2 / - Y2 L21.  The
+// reference runs the same recursion as separate TRSM and GEMM launches
+// (src/batch_triangular/Xtrsm_batch_drivers.cuh:145-171).  skip: optional
+// per-matrix flags, a matrix with skip[b] != 0 keeps its B.
+template <typename T>
+int tr_rec(bool fwd, int side_left, int unit, int k, int nvec, T alpha, kb_mat_t A, kb_mat_t B, int batch,
+           const int* skip, hipStream_t s)
+{
+    if (k <= KB_REG_MAX_N) {
+        if (k <= 32) return tm_launch<T, 2>(fwd, k, nvec, side_left, 0, unit, alpha, A, B, batch, s, skip);
+        return tm_launch<T, 4>(fwd, k, nvec, side_left, 0, unit, alpha, A, B, batch, s, skip);
+    }
+    const int parts = (k + KB_REG_MAX_N - 1) / KB_REG_MAX_N;
+    const int leaf = (k + parts - 1) / parts;
+    const int k1 = (parts / 2) * leaf, k2 = k - k1;
+    const long lda = A.ld, ldb = B.ld;
+    kb_mat_t L11 = A, L21 = A, L22 = A, B1 = B, B2 = B;
+    L21.off += k1;
+    L22.off += k1 + (long)k1 * lda;
+    B2.off += side_left ? (long)k1 : (long)k1 * ldb;
+    const char prec = sizeof(T) == 8 ? 'd' : 's';
+    int e;
+    if (fwd) {
+        if ((e = tr_rec<T>(true, side_left, unit, k1, nvec, alpha, L11, B1, batch, skip, s))) return e;
+        e = side_left ? kb_launch_gemm_skip(prec, 0, 0, 0, 0, k2, nvec, k1, -1.0, L21, B1, (double)alpha, B2, nullptr,
+                                            nullptr, nullptr, skip, batch, s)
+                      : kb_launch_gemm_skip(prec, 0, 0, 0, 1, nvec, k2, k1, -1.0, B1, L21, (double)alpha, B2, nullptr,
+                                            nullptr, nullptr, skip, batch, s);
+        if (e) return e;
+        return tr_rec<T>(true, side_left, unit, k2, nvec, T(1), L22, B2, batch, skip, s);
+    }
+    if ((e = tr_rec<T>(false, side_left, unit, k2, nvec, alpha, L22, B2, batch, skip, s))) return e;
+    e = side_left ? kb_launch_gemm_skip(prec, 0, 0, 1, 0, k1, nvec, k2, -1.0, L21, B2, (double)alpha, B1, nullptr,
+                                        nullptr, nullptr, skip, batch, s)
+                  : kb_launch_gemm_skip(prec, 0, 0, 0, 0, nvec, k1, k2, -1.0, B2, L21, (double)alpha, B1, nullptr,
+                                        nullptr, nullptr, skip, batch, s);
+    if (e) return e;
+    return tr_rec<T>(false, side_left, unit, k1, nvec, T(1), L11, B1, batch, skip, s);
+}
+
 }  // namespace
 
 // Real uniform TRSM ('d' or 's'; trans 0 = N, 1 or 2 = T).  B is m x n.
@@ -421,6 +470,11 @@ extern "C" int kb_launch_trsm_fast(char prec, int side_left, int upper, int tran
     const bool fwd = side_left ? !et : et;
     if (k > KB_REG_MAX_N) {
         if (upper) return (int)hipErrorNotSupported;
+        if (!vm && !vn && !A.lds && !B.lds && nvec >= KB_TRSM_REC_NVEC && alpha != 0.0) {
+            if (prec == 'd') return tr_rec<double>(fwd, side_left, unit, k, nvec, alpha, A, B, batch, nullptr, s);
+            if (prec == 's')
+                return tr_rec<float>(fwd, side_left, unit, k, nvec, (float)alpha, A, B, batch, nullptr, s);
+        }
         if (prec == 'd') return kb_blk_trsm_d(fwd, side_left, unit, k, nvec, alpha, A, B, vm, vn, batch, s);
         if (prec == 's') return kb_blk_trsm_s(fwd, side_left, unit, k, nvec, alpha, A, B, vm, vn, batch, s);
         return (int)hipErrorInvalidValue;
@@ -448,6 +502,26 @@ int tm_solve(int side_left, int n, int nrhs, kb_mat_t A, kb_mat_t B, const int* 
 // (L L^T) X = B with the lower factor L of order 16 < n <= 64 ('d' or 's'):
 // the forward and the backward sweep of the matrix-core TRSM; matrices with
 // skip[b] != 0 (a failed factorization) are left as they are.
+// (L L^T) X = B with the lower factor L of order n > 64 and at least
+// KB_TRSM_REC_NVEC right-hand sides ('d' or 's'): the forward and the backward
+// solve by the recursion above; matrices with skip[b] != 0 keep their B.
+extern "C" int kb_launch_trsm_rec_solve(char prec, int side_left, int n, int nrhs, kb_mat_t A, kb_mat_t B,
+                                        const int* skip, int batch, hipStream_t s)
+{
+    if (batch < 1 || nrhs < 1) return 0;
+    if (n <= KB_REG_MAX_N || A.lds || B.lds) return (int)hipErrorInvalidValue;
+    int e;
+    if (prec == 'd') {
+        if ((e = tr_rec<double>(true, side_left, 0, n, nrhs, 1.0, A, B, batch, skip, s))) return e;
+        return tr_rec<double>(false, side_left, 0, n, nrhs, 1.0, A, B, batch, skip, s);
+    }
+    if (prec == 's') {
+        if ((e = tr_rec<float>(true, side_left, 0, n, nrhs, 1.0f, A, B, batch, skip, s))) return e;
+        return tr_rec<float>(false, side_left, 0, n, nrhs, 1.0f, A, B, batch, skip, s);
+    }
+    return (int)hipErrorInvalidValue;
+}
+
 extern "C" int kb_launch_trsm_solve(char prec, int side_left, int n, int nrhs, kb_mat_t A, kb_mat_t B,
                                     const int* skip, int batch, hipStream_t s)
 {

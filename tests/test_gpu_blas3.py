@@ -275,3 +275,49 @@ def test_dpotrf_blocked_info_in_second_panel(h, fail_col):
     for i in (0, 2):
         assert inf[i] == 0
         assert po.potrf_berr(po.col(A, i), po.col(L, i)) <= 4 * n * EPS
+
+
+@pytest.mark.parametrize("n,nrhs,side", [(100, 64, "L"), (150, 80, "R"), (256, 64, "L")])
+def test_dposv_recursive_solve_failed_matrix_keeps_b(h, n, nrhs, side):
+    # orders above 64 with 64 or more right-hand sides: blocked factorization,
+    # then the recursive solves; the matrix whose factorization fails keeps
+    # its B bit for bit, the others match LAPACK
+    batch = 4
+    A = po.gen_spd_shift_batch(31, n, batch)
+    A[2, n // 2, n // 2] = -1.0
+    B = po.gen_rhs_batch(32, n, nrhs, batch) if side == "L" else po.gen_rhs_batch(32, nrhs, n, batch)
+    dA, dB = kt.to_dev(A), kt.to_dev(B)
+    info = kt.to_dev(np.full(batch, -7, np.int32))
+    rows, cols = B.shape[2], B.shape[1]
+    st = h.lib.kblasDposv_batch_strided(h.h, side.encode(), b"L", rows, cols, kt.P(dA), n, n * n, kt.P(dB), rows,
+                                        rows * cols, batch, kt.P(info))
+    assert st == pykblas.KBLAS_Success
+    inf, X = kt.to_host(info), kt.to_host(dB)
+    _, info_o = po.potrf(po.col(A, 2))
+    assert inf[2] == info_o and inf[2] > 0
+    assert np.array_equal(X[2], B[2])
+    for i in (0, 1, 3):
+        assert inf[i] == 0
+        Ai = po.col(A, i)
+        assert po.solve_berr(side, Ai, po.col(X, i), po.col(B, i)) <= 4 * n * EPS
+        _, Xo, _ = po.posv(side, Ai, po.col(B, i))
+        assert rel(po.col(X, i), Xo) <= 64 * n * EPS
+
+
+def test_dpotrs_recursive_pointer_array(h):
+    n, nrhs, batch = 130, 70, 3
+    A = po.gen_spd_shift_batch(33, n, batch)
+    Lh = np.zeros_like(A)
+    for i in range(batch):
+        Lh[i] = np.tril(po.potrf(po.col(A, i))[0]).T
+    B = po.gen_rhs_batch(34, nrhs, n, batch)     # right side: nrhs x n
+    torch = kt.torch_dev()
+    dL, dB = kt.to_dev(Lh), kt.to_dev(B)
+    pL = torch.tensor([dL[i].data_ptr() for i in range(batch)], dtype=torch.int64, device="cuda")
+    pB = torch.tensor([dB[i].data_ptr() for i in range(batch)], dtype=torch.int64, device="cuda")
+    st = h.lib.kblasDpotrs_batch(h.h, b"R", b"L", nrhs, n, kt.P(pL), n, kt.P(pB), nrhs, batch)
+    assert st == pykblas.KBLAS_Success
+    X = kt.to_host(dB)
+    for i in range(batch):
+        Xo = po.potrs("R", po.col(Lh, i), po.col(B, i))
+        assert np.max(np.abs(po.col(X, i) - Xo)) <= 64 * n * EPS * max(np.max(np.abs(Xo)), 1)

@@ -196,3 +196,21 @@ def test_trsm_alpha_zero_sets_b_without_reading(h, k, side):
                                         kt.P(dB), m, m * n, batch)
     assert st == pykblas.KBLAS_Success
     assert np.array_equal(kt.to_host(dB), np.zeros_like(B))
+
+
+@pytest.mark.parametrize("p", ["D", "S"])
+@pytest.mark.parametrize("side", ["L", "R"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("shape", [(72, 64), (129, 65), (200, 96)])
+def test_recursive_trsm_lower(h, p, side, trans, shape):
+    # orders above 64 with at least 64 vectors: half-order recursion with
+    # matrix-core GEMM updates, leaves of order 33..64; pointer arrays for
+    # odd orders (sub-block offsets on top of each matrix's pointer)
+    k, nv = shape
+    m, n = (k, nv) if side == "L" else (nv, k)
+    run_case(h, p, side, False, trans, False, m, n, batch=5, seed=3 * k + nv, ptrs=(k % 2 == 1))
+
+
+@pytest.mark.parametrize("side", ["L", "R"])
+def test_recursive_trsm_unit_diagonal(h, side):
+    run_case(h, "D", side, False, "T", True, *((150, 70) if side == "L" else (70, 150)), batch=3, seed=11, ptrs=True)
