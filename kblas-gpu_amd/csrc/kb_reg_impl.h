@@ -190,9 +190,9 @@ KB_DEV void reg_chol_one(int n, int nrhs, int side_left, T* Ap, int lda, T* Bp, 
             T x[RC];
 #pragma unroll
             for (int c = 0; c < RC; ++c) {
-                const bool in = r < n && c0 + c < nrhs;
-                const T v = gld_opaque(rhs_elem(Bp, ldb, side_left, in ? r : 0, in ? c0 + c : 0));
-                x[c] = in ? v : T(0);
+                T v = T(0);
+                if (r < n && c0 + c < nrhs) v = gld(rhs_elem(Bp, ldb, side_left, r, c0 + c));
+                x[c] = v;
             }
             reg_forward<T, NB, G, RC, EX>(a, x, n, my_inv);
             if (Lt) reg_backward<T, NB, G, RC, EX>(Lt, x, n, r, my_inv);
