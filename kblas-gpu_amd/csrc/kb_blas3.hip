@@ -30,7 +30,7 @@ template <typename T>
 BatchArg<T> arg(kb_mat_t m)
 {
     BatchArg<T> a;
-    a.base = (T*)m.base; a.stride = m.stride; a.ptrs = (T* const*)m.ptrs; a.lds = m.lds; a.ld = m.ld; a.off = m.off;
+    a.base = (T*)m.base; a.stride = m.stride; a.ptrs = (T* const*)m.ptrs; a.lds = m.lds; a.ld = m.ld;
     return a;
 }
 

@@ -27,7 +27,25 @@ struct BatchArg {
     T* const* ptrs;      // pointer-array mode (device array of device pointers)
     const int* lds;      // non-uniform leading dimensions (device), or nullptr
     int ld;              // uniform leading dimension
-    long off;            // element offset into every matrix (a sub-block)
+    KB_DEV T* get(long b) const
+    {
+        return ptrs ? ptrs[b] : base + b * stride;
+    }
+    KB_DEV int ldim(long b) const { return lds ? lds[b] : ld; }
+};
+
+// The same with an element offset into every matrix: sub-blocks of a batch,
+// which a pointer array cannot take into its base.  Only the kernels that
+// work on sub-blocks (matrix-core TRSM and GEMM) take this form, so the others
+// keep two scalar registers per operand free.
+template <typename T>
+struct BatchArgO {
+    T* base;
+    long stride;
+    T* const* ptrs;
+    const int* lds;
+    int ld;
+    long off;
     KB_DEV T* get(long b) const
     {
         return (ptrs ? ptrs[b] : base + b * stride) + off;
@@ -35,10 +53,23 @@ struct BatchArg {
     KB_DEV int ldim(long b) const { return lds ? lds[b] : ld; }
 };
 
+// A strided operand takes its offset into the base; a pointer array with an
+// offset needs to_arg_off.
 template <typename T>
 static BatchArg<T> to_arg(kb_mat_t m)
 {
     BatchArg<T> a;
+    a.base = (T*)m.base + (m.ptrs ? 0 : m.off);
+    a.stride = m.stride;
+    a.ptrs = (T* const*)m.ptrs;
+    a.lds = m.lds;
+    a.ld = m.ld;
+    return a;
+}
+template <typename T>
+static BatchArgO<T> to_arg_off(kb_mat_t m)
+{
+    BatchArgO<T> a;
     a.base = (T*)m.base;
     a.stride = m.stride;
     a.ptrs = (T* const*)m.ptrs;

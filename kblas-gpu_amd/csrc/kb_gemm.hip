@@ -68,7 +68,7 @@ KB_DEV void gm_load(const T* Ap, int lda, const T* Bp, int ldb, int m, int n, in
 
 template <typename T, int TM, bool TA, bool TB>
 __global__ void __launch_bounds__(64)
-k_gemm_mfma(int m, int n, int k, T alpha, BatchArg<T> A, BatchArg<T> B, T beta, BatchArg<T> C, int batch,
+k_gemm_mfma(int m, int n, int k, T alpha, BatchArgO<T> A, BatchArgO<T> B, T beta, BatchArgO<T> C, int batch,
             int tiles, int tm, int syrk, int upper, const int* vm, const int* vn, const int* vk, const int* skip)
 {
     constexpr int NB = TM / 16, KU = gm_ku<T, TM>();
@@ -209,8 +209,8 @@ KB_DEV void gl_store(T (*sA)[GL_LD], T (*sB)[GL_LD], int m, int n, int k, int I0
 
 template <typename T, bool TA, bool TB>
 __global__ void __launch_bounds__(256)
-k_gemm_lds(int m, int n, int k, T alpha, BatchArg<T> A, BatchArg<T> B, T beta, BatchArg<T> C, int batch, int tiles,
-           int tm, int syrk, int upper, const int* vm, const int* vn, const int* vk, const int* skip)
+k_gemm_lds(int m, int n, int k, T alpha, BatchArgO<T> A, BatchArgO<T> B, T beta, BatchArgO<T> C, int batch,
+           int tiles, int tm, int syrk, int upper, const int* vm, const int* vn, const int* vk, const int* skip)
 {
     typedef typename TileV<T>::v4 v4;
     __shared__ T sA[2][GL_KC][GL_LD];
@@ -311,8 +311,8 @@ int gl_launch(int m, int n, int k, T alpha, kb_mat_t A, kb_mat_t B, T beta, kb_m
     const int tiles = syrk ? tm * (tm + 1) / 2 : tm * tn;
     const long wgs = (batch + 7) / 8 * 8L * tiles;
     if (wgs >= (1L << 24)) return (int)hipErrorNotSupported;   // grid of at most 2^32 work-items
-    hipLaunchKernelGGL((k_gemm_lds<T, TA, TB>), dim3((unsigned)wgs), dim3(256), 0, s, m, n, k, alpha, to_arg<T>(A),
-                       to_arg<T>(B), beta, to_arg<T>(C), batch, tiles, tm, syrk, upper, vm, vn, vk, skip);
+    hipLaunchKernelGGL((k_gemm_lds<T, TA, TB>), dim3((unsigned)wgs), dim3(256), 0, s, m, n, k, alpha, to_arg_off<T>(A),
+                       to_arg_off<T>(B), beta, to_arg_off<T>(C), batch, tiles, tm, syrk, upper, vm, vn, vk, skip);
     return (int)hipGetLastError();
 }
 
@@ -326,7 +326,7 @@ int gm_launch(int m, int n, int k, T alpha, kb_mat_t A, kb_mat_t B, T beta, kb_m
     const long wgs = groups * 8L * tiles;
     if (wgs >= (1L << 26)) return (int)hipErrorNotSupported;   // grid of at most 2^32 work-items
     hipLaunchKernelGGL((k_gemm_mfma<T, TM, TA, TB>), dim3((unsigned)wgs), dim3(64), 0, s, m, n, k, alpha,
-                       to_arg<T>(A), to_arg<T>(B), beta, to_arg<T>(C), batch, tiles, tm, syrk, upper, vm, vn, vk,
+                       to_arg_off<T>(A), to_arg_off<T>(B), beta, to_arg_off<T>(C), batch, tiles, tm, syrk, upper, vm, vn, vk,
                        skip);
     return (int)hipGetLastError();
 }

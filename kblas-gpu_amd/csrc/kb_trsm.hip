@@ -199,7 +199,7 @@ constexpr int TM_NCB = 2;   // column blocks of 16 vectors per wavefront
 template <typename T, int NBK, bool FWD>
 __global__ void __launch_bounds__(256)
 k_trsm_mfma(int k0, int nvec0, int chunks, long items, int side_left, int upper, int unit, T alpha,
-            BatchArg<T> A, BatchArg<T> B, const int* skip, const int* vm, const int* vn)
+            BatchArgO<T> A, BatchArgO<T> B, const int* skip, const int* vm, const int* vn)
 {
     typedef typename TileV<T>::v4 v4;
     constexpr int NSLOT = NBK * (NBK + 1) / 2;
@@ -361,7 +361,7 @@ int tm_launch(bool fwd, int k, int nvec, int side_left, int upper, int unit, T a
     const int chunks = (nvec + waves * per_wave - 1) / (waves * per_wave);
     const long items = (long)batch * chunks;
     const unsigned grid = (unsigned)(items < (1L << 22) ? items : (1L << 22));
-    BatchArg<T> a = to_arg<T>(A), b = to_arg<T>(B);
+    BatchArgO<T> a = to_arg_off<T>(A), b = to_arg_off<T>(B);
     if (fwd)
         hipLaunchKernelGGL((k_trsm_mfma<T, NBK, true>), dim3(grid), dim3(64 * waves), 0, s, k, nvec, chunks, items,
                            side_left, upper, unit, alpha, a, b, skip, vm, vn);
