@@ -321,3 +321,24 @@ def test_dpotrs_recursive_pointer_array(h):
     for i in range(batch):
         Xo = po.potrs("R", po.col(Lh, i), po.col(B, i))
         assert np.max(np.abs(po.col(X, i) - Xo)) <= 64 * n * EPS * max(np.max(np.abs(Xo)), 1)
+
+
+@pytest.mark.parametrize("p,n,nrhs,side", [("D", 300, 70, "L"), ("S", 130, 64, "R")])
+def test_posv_recursive_solve_orders(h, p, n, nrhs, side):
+    # the recursive solves above order 256 (FP64) and in FP32: backward error
+    # of every system within the working precision's bar
+    dt, eps = (np.float64, kt.EPS64) if p == "D" else (np.float32, kt.EPS32)
+    batch = 3
+    A = po.gen_spd_shift_batch(35, n, batch)
+    B = po.gen_rhs_batch(36, n, nrhs, batch) if side == "L" else po.gen_rhs_batch(36, nrhs, n, batch)
+    A, B = A.astype(dt).astype(np.float64), B.astype(dt).astype(np.float64)
+    dA, dB = kt.to_dev(A.astype(dt)), kt.to_dev(B.astype(dt))
+    info = kt.to_dev(np.full(batch, -7, np.int32))
+    rows, cols = B.shape[2], B.shape[1]
+    st = getattr(h.lib, "kblas%sposv_batch_strided" % p)(h.h, side.encode(), b"L", rows, cols, kt.P(dA), n, n * n,
+                                                         kt.P(dB), rows, rows * cols, batch, kt.P(info))
+    assert st == pykblas.KBLAS_Success
+    assert np.all(kt.to_host(info) == 0)
+    X = kt.to_host(dB).astype(np.float64)
+    for i in range(batch):
+        assert po.solve_berr(side, po.col(A, i), po.col(X, i), po.col(B, i)) <= 4 * n * eps
